@@ -1,0 +1,332 @@
+"""MI355X ISSL off-target scorer (MIT + CFD specificity).
+
+Python face of libissl_gpu.so, used by the tests and the benchmark.  The
+scorer itself is HIP/C++ behind the C ABI in include/issl_gpu.h; this module
+only moves numpy arrays across it.
+
+Drop-in surface of the reference:
+  * ISSLScoringModule::run  (lib/ISSLScoringModule.cpp:52-466)  -> Index.score
+  * ISSLScoreOfftargets CLI (ISSLScoreOfftargets.cpp:36-647)    -> bin/ISSLScoreOfftargets
+  * ISSLCreateIndex          (ISSLCreateIndex.cpp:284-493)       -> build_lists / write_index
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_uint64, c_void_p
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _capi
+from ._capi import (FLAG_DISTINCT, FORM_BATCHED, FORM_PLAIN, METHOD_AVG_MIT_CFD, METHOD_CFD,
+                    METHOD_MIT, METHOD_MIT_AND_CFD, METHOD_MIT_OR_CFD, PREC_FP32C, PREC_FP64,
+                    IsslError, check, lib)
+
+__all__ = [
+    "Index", "IndexArrays", "Results", "IsslError", "encode", "decode", "build_lists",
+    "write_index", "read_index", "parse_masks", "count_sorted_sites", "score_tables",
+    "synth", "device_count", "METHOD_MIT", "METHOD_CFD", "METHOD_MIT_AND_CFD",
+    "METHOD_MIT_OR_CFD", "METHOD_AVG_MIT_CFD", "FORM_PLAIN", "FORM_BATCHED", "PREC_FP64",
+    "PREC_FP32C", "FLAG_DISTINCT", "METHOD_NAMES", "CLI_METHODS",
+]
+
+# include/util.hpp:20 and lib/util.cpp:13-19
+METHOD_NAMES = {"mit": METHOD_MIT, "cfd": METHOD_CFD, "mitAndCfd": METHOD_MIT_AND_CFD,
+                "mitOrCfd": METHOD_MIT_OR_CFD, "avgMitCfd": METHOD_AVG_MIT_CFD}
+# ISSLScoreOfftargets.cpp:69-91
+CLI_METHODS = {"and": METHOD_MIT_AND_CFD, "or": METHOD_MIT_OR_CFD, "avg": METHOD_AVG_MIT_CFD,
+               "mit": METHOD_MIT, "cfd": METHOD_CFD}
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def device_count() -> int:
+    return int(lib().issl_device_count())
+
+
+# ------------------------------------------------------------------ encoding
+
+_CODE = np.zeros(256, dtype=np.uint64)
+_CODE[ord("C")] = 1
+_CODE[ord("G")] = 2
+_CODE[ord("T")] = 3
+
+
+def encode(seqs) -> np.ndarray:
+    """20-mers -> signatures through issl_encode (ISSLScoringModule.cpp:469-476)."""
+    seqs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+    n = len(seqs)
+    out = np.zeros(n, dtype=np.uint64)
+    if n == 0:
+        return out
+    if any(len(s) < 20 for s in seqs):
+        raise ValueError("guides must have at least 20 characters")
+    buf = b"".join(s[:20] for s in seqs)
+    check(lib().issl_encode(buf, n, 20, _ptr(out)))
+    return out
+
+
+def decode(sigs) -> list:
+    """Signatures -> 20-mers (ISSLScoringModule.cpp:478-485)."""
+    out = []
+    for s in np.asarray(sigs, dtype=np.uint64).tolist():
+        out.append("".join("ACGT"[(s >> (2 * j)) & 3] for j in range(20)))
+    return out
+
+
+# ------------------------------------------------------------------ index data
+
+@dataclass
+class IndexArrays:
+    """Host arrays of an index in the .issl layout (ISSLCreateIndex.cpp:428-484)."""
+    sigs: np.ndarray          # uint64[N]
+    masks: np.ndarray         # uint64[S]
+    list_sizes: np.ndarray    # uint64[sum 4^w]
+    entries: np.ndarray       # uint64[S*N]  occ<<32 | id
+    seq_len: int = 20
+
+    @property
+    def n_sites(self) -> int:
+        return int(self.sigs.shape[0])
+
+    @property
+    def n_slices(self) -> int:
+        return int(self.masks.shape[0])
+
+    def occurrences(self) -> np.ndarray:
+        """occ per id, read from slice 0's entries."""
+        occ = np.zeros(self.n_sites, dtype=np.uint32)
+        if self.n_slices:
+            e = self.entries[: self.n_sites]
+            occ[(e & np.uint64(0xFFFFFFFF)).astype(np.int64)] = (e >> np.uint64(32)).astype(np.uint32)
+        return occ
+
+
+def list_count(masks, seq_len: int = 20) -> int:
+    m = np.ascontiguousarray(masks, dtype=np.uint64)
+    return int(lib().issl_list_count(_ptr(m), m.shape[0], seq_len))
+
+
+def build_lists(sigs, occ, masks, seq_len: int = 20, threads: int = 0) -> IndexArrays:
+    """Slice lists of distinct sites (ISSLCreateIndex.cpp:450-484)."""
+    sigs = np.ascontiguousarray(sigs, dtype=np.uint64)
+    occ = np.ascontiguousarray(occ, dtype=np.uint32)
+    masks = np.ascontiguousarray(masks, dtype=np.uint64)
+    n, s = sigs.shape[0], masks.shape[0]
+    nl = list_count(masks, seq_len) if s else 0
+    if s and nl == 0:
+        raise ValueError("slice wider than 16 positions")
+    sizes = np.zeros(max(nl, 1), dtype=np.uint64)[:nl]
+    entries = np.zeros(max(n * s, 1), dtype=np.uint64)[: n * s]
+    check(lib().issl_build_lists(_ptr(sigs), _ptr(occ), n, seq_len, _ptr(masks), s,
+                                 _ptr(sizes), _ptr(entries), threads))
+    return IndexArrays(sigs, masks, sizes, entries, seq_len)
+
+
+def write_index(path: str, ix: IndexArrays) -> None:
+    check(lib().issl_write_index(path.encode(), ix.n_sites, ix.seq_len, ix.n_slices,
+                                 _ptr(ix.sigs), _ptr(ix.masks), _ptr(ix.list_sizes),
+                                 _ptr(ix.entries)))
+
+
+def read_index(path: str) -> IndexArrays:
+    n, L, S, nl = c_uint64(), c_uint64(), c_uint64(), c_uint64()
+    check(lib().issl_read_header(path.encode(), byref(n), byref(L), byref(S), byref(nl)))
+    sigs = np.zeros(n.value, dtype=np.uint64)
+    masks = np.zeros(S.value, dtype=np.uint64)
+    sizes = np.zeros(nl.value, dtype=np.uint64)
+    entries = np.zeros(n.value * S.value, dtype=np.uint64)
+    check(lib().issl_read_index(path.encode(), _ptr(sigs), _ptr(masks), _ptr(sizes),
+                                _ptr(entries)))
+    return IndexArrays(sigs, masks, sizes, entries, int(L.value))
+
+
+def parse_masks(text: str, seq_len: int = 20) -> np.ndarray:
+    b = text.encode()
+    n = c_uint64()
+    check(lib().issl_parse_masks(b, len(b), seq_len, None, 0, byref(n)))
+    out = np.zeros(n.value, dtype=np.uint64)
+    check(lib().issl_parse_masks(b, len(b), seq_len, _ptr(out), n.value, byref(n)))
+    return out
+
+
+def count_sorted_sites(text: bytes, seq_len: int = 20, line_len: int = 21):
+    n = c_uint64()
+    check(lib().issl_count_sorted_sites(text, len(text), seq_len, line_len, None, None, 0,
+                                        byref(n)))
+    sigs = np.zeros(n.value, dtype=np.uint64)
+    occ = np.zeros(n.value, dtype=np.uint32)
+    check(lib().issl_count_sorted_sites(text, len(text), seq_len, line_len, _ptr(sigs),
+                                        _ptr(occ), n.value, byref(n)))
+    return sigs, occ
+
+
+def score_tables():
+    """(mit_keys[6195], mit_vals[6195], cfd_pos[320], cfd_pam[16]) as the product holds them."""
+    k = np.zeros(6195, dtype=np.uint64)
+    v = np.zeros(6195, dtype=np.float64)
+    p = np.zeros(320, dtype=np.float64)
+    a = np.zeros(16, dtype=np.float64)
+    check(lib().issl_score_tables(_ptr(k), _ptr(v), _ptr(p), _ptr(a)))
+    return k, v, p, a
+
+
+def synth(cfg: dict, threads: int = 0):
+    """Generate a seeded workload: (sigs, occ, masks, guides, kinds)."""
+    c = _capi.issl_synth_cfg()
+    for key, val in cfg.items():
+        setattr(c, key, val)
+    q = (c.n_guides_site + c.n_guides_unique + c.n_guides_planted + c.n_guides_repeat
+         + c.n_guides_random)
+    sigs = np.zeros(c.n_sites, dtype=np.uint64)
+    occ = np.zeros(c.n_sites, dtype=np.uint32)
+    masks = np.zeros(c.n_slices, dtype=np.uint64)
+    guides = np.zeros(q, dtype=np.uint64)
+    kinds = np.zeros(q, dtype=np.uint8)
+    check(lib().issl_synth(byref(c), _ptr(sigs), _ptr(occ), _ptr(masks), _ptr(guides),
+                           _ptr(kinds), threads))
+    return sigs, occ, masks, guides, kinds
+
+
+# ------------------------------------------------------------------ scoring
+
+@dataclass
+class Results:
+    mit: np.ndarray
+    cfd: np.ndarray
+    occ_count: np.ndarray
+    site_count: np.ndarray
+    passed: np.ndarray      # bytes b'1' / b'0' as uint8
+    mit_sum: np.ndarray
+    cfd_sum: np.ndarray
+
+
+class Index:
+    """A .issl index resident on one GPU (issl_index_open / issl_index_from_arrays)."""
+
+    def __init__(self, handle: int):
+        self._h = c_void_p(handle)
+
+    @classmethod
+    def open(cls, path: str, device: int = 0) -> "Index":
+        h = c_void_p()
+        check(lib().issl_index_open(path.encode(), device, byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, ix: IndexArrays, device: int = 0) -> "Index":
+        h = c_void_p()
+        check(lib().issl_index_from_arrays(ix.n_sites, ix.seq_len, ix.n_slices, _ptr(ix.sigs),
+                                           _ptr(ix.masks), _ptr(ix.list_sizes),
+                                           _ptr(ix.entries), device, byref(h)))
+        return cls(h.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            check(lib().issl_index_close(self._h))
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> _capi.issl_index_info:
+        out = _capi.issl_index_info()
+        check(lib().issl_index_get_info(self._h, byref(out)))
+        return out
+
+    def validate(self) -> int:
+        bad = c_uint64()
+        check(lib().issl_index_validate(self._h, byref(bad)))
+        return int(bad.value)
+
+    def distinct_count(self) -> int:
+        c = c_uint64()
+        check(lib().issl_distinct_count(self._h, byref(c)))
+        return int(c.value)
+
+    def distinct_reset(self) -> None:
+        check(lib().issl_distinct_reset(self._h))
+
+    @staticmethod
+    def params(method: int = METHOD_MIT_AND_CFD, threshold: float = 0.0,
+               form: int = FORM_PLAIN, precision: int = PREC_FP64, flags: int = 0,
+               chunk: int = 0, max_dist: int = 4) -> _capi.issl_params:
+        return _capi.issl_params(max_dist, method, threshold, form, precision, flags, chunk)
+
+    def score(self, guides, method: int = METHOD_MIT_AND_CFD, threshold: float = 0.0,
+              form: int = FORM_PLAIN, precision: int = PREC_FP64, flags: int = 0,
+              chunk: int = 0, max_dist: int = 4) -> Results:
+        g = np.ascontiguousarray(guides, dtype=np.uint64)
+        n = g.shape[0]
+        r = Results(np.zeros(n), np.zeros(n), np.zeros(n, np.uint64), np.zeros(n, np.uint64),
+                    np.zeros(n, np.uint8), np.zeros(n), np.zeros(n))
+        res = _capi.issl_results(_ptr(r.mit), _ptr(r.cfd), _ptr(r.occ_count),
+                                 _ptr(r.site_count), _ptr(r.passed), _ptr(r.mit_sum),
+                                 _ptr(r.cfd_sum))
+        p = self.params(method, threshold, form, precision, flags, chunk, max_dist)
+        check(lib().issl_score(self._h, _ptr(g), n, byref(p), byref(res)))
+        return r
+
+    def candidates(self, guides):
+        """CSR candidate sets (offsets[n+1], ids) in the reference's visit order."""
+        g = np.ascontiguousarray(guides, dtype=np.uint64)
+        n = g.shape[0]
+        off = np.zeros(n + 1, dtype=np.uint64)
+        rc = lib().issl_candidates(self._h, _ptr(g), n, _ptr(off), None, 0)
+        if rc not in (_capi.ISSL_OK, _capi.ISSL_ERR_CAPACITY):
+            check(rc)
+        total = int(off[-1])
+        ids = np.zeros(max(total, 1), dtype=np.uint32)
+        check(lib().issl_candidates(self._h, _ptr(g), n, _ptr(off), _ptr(ids), total))
+        return off, ids[:total]
+
+    def autotune(self, sample, cache_path: Optional[str] = None) -> _capi.issl_tuning:
+        g = np.ascontiguousarray(sample, dtype=np.uint64)
+        t = _capi.issl_tuning()
+        check(lib().issl_autotune(self._h, _ptr(g), g.shape[0],
+                                  cache_path.encode() if cache_path else None, byref(t)))
+        return t
+
+    def tuning(self) -> _capi.issl_tuning:
+        t = _capi.issl_tuning()
+        check(lib().issl_get_tuning(self._h, byref(t)))
+        return t
+
+    def timing_enable(self, enable: bool = True) -> None:
+        check(lib().issl_timing_enable(self._h, 1 if enable else 0))
+
+    def timing_read(self):
+        """(summed dominant-kernel ms, launches) since timing_enable."""
+        ms, n = ctypes.c_double(), c_uint64()
+        check(lib().issl_timing_read(self._h, byref(ms), byref(n)))
+        return ms.value, int(n.value)
+
+    def score_device(self, d_guides, n: int, d_out: dict, method: int = METHOD_MIT_AND_CFD,
+                     threshold: float = 0.0, form: int = FORM_PLAIN,
+                     precision: int = PREC_FP64, flags: int = 0, chunk: int = 0,
+                     stream=None) -> None:
+        """Asynchronous scoring of device-resident guides into device buffers.
+
+        d_guides / d_out values are hiprt.DeviceArray (or None per output)."""
+        def dp(x):
+            return None if x is None else x.ptr
+        res = _capi.issl_results(dp(d_out.get("mit")), dp(d_out.get("cfd")),
+                                 dp(d_out.get("occ_count")), dp(d_out.get("site_count")),
+                                 dp(d_out.get("pass")), dp(d_out.get("mit_sum")),
+                                 dp(d_out.get("cfd_sum")))
+        p = self.params(method, threshold, form, precision, flags, chunk)
+        check(lib().issl_score_device(self._h, d_guides.ptr, n, byref(p), byref(res), stream))
+
+    def set_tuning(self, chunk: int, depth: int, grid_per_cu: int) -> None:
+        t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, 0.0)
+        check(lib().issl_set_tuning(self._h, byref(t)))

@@ -1,0 +1,689 @@
+// batched.hip -- BATCHED scoring form: a persistent, work-stealing kernel over
+// (guide, chunk) tiles of each guide's slice-major candidate sequence.
+//
+// Pipeline (all on one stream, capturable in a hipGraph; no memsets):
+//   k_plan     one thread per guide: slice keys -> list ranges, prefix P_g,
+//              tile length C_g = max(chunk, ceil(P_g/kMaxTiles)) and tiles K_g
+//   k_scan     one workgroup: exclusive scan of K_g -> tile_base, total T,
+//              queue counter reset
+//   k_fillmap  tile -> (guide, k) map
+//   k_tiles    persistent waves pop tiles from the atomic queue; each streams
+//              its tile's site words (sig | occ<<40, no signature gather),
+//              folds, filters, dedups by the slice-order rule and sums its
+//              hits; one partial per tile, reduced in a fixed lane order
+//   k_finalize one wave per guide: adds the tile partials in tile order and,
+//              when the early exit is armed, rescans the tile in which the
+//              running total first meets the exit test, hit by hit, to stop
+//              exactly where the reference's loop stops
+//              (lib/ISSLScoringModule.cpp:365-401)
+// Results therefore depend only on the guide and the tile length, never on
+// which wave ran which tile or on what else was in the batch.
+//
+// Precision: ISSL_PREC_FP32C forms each site's MIT/CFD product in FP32 and
+// carries every sum as an unevaluated pair (hi, lo) of FP32 built with
+// error-free TwoSum; FP64 appears only for hi+lo, the 10000/(100+S)
+// transform and the threshold tests.  ISSL_PREC_FP64 uses FP64 throughout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <type_traits>
+
+#include "device.hpp"
+
+namespace issl {
+
+constexpr int kMaxTiles = 64;        // tiles per guide at most
+constexpr int kTileBlock = 256;      // threads per persistent workgroup
+constexpr int kTileWaves = kTileBlock / 64;
+
+struct TilePart {
+    union {
+        struct {
+            float mh, ml, ch, cl;
+        } f;
+        struct {
+            double m, c;
+        } d;
+    };
+    uint64_t occ;
+    uint64_t sites;
+};
+
+struct BatchWs {
+    uint64_t *pre;       // [n][S+1]
+    uint64_t *delta;     // [n][S]   posting index = position + delta
+    uint32_t *gchunk;    // [n]
+    uint32_t *gtiles;    // [n]
+    uint32_t *tile_base; // [n+1]
+    uint32_t *ctrl;      // [0] queue counter, [1] total tiles
+    uint64_t *tile_map;  // [n*kMaxTiles] guide<<32 | k
+    TilePart *part;      // [n*kMaxTiles]
+};
+
+// ------------------------------------------------------------ FP helpers
+
+struct F2 {
+    float s, c;
+};
+
+// Knuth's TwoSum: s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ F2 f2_add(F2 x, float t) {
+    float s, e;
+    two_sum(x.s, t, s, e);
+    return F2{s, x.c + e};
+}
+
+__device__ __forceinline__ F2 f2_merge(F2 x, F2 y) {
+    float s, e;
+    two_sum(x.s, y.s, s, e);
+    return F2{s, (x.c + y.c) + e};
+}
+
+__device__ __forceinline__ double f2_value(F2 x) {
+    return static_cast<double>(x.s) + static_cast<double>(x.c);
+}
+
+__device__ __forceinline__ uint64_t bfold(uint64_t x) {
+    return ((x >> 1) & kFoldLo) | (x & kFoldLo);
+}
+
+__device__ __forceinline__ uint32_t brank(uint64_t f, int k) {
+    uint32_t r = (k == 1) ? 0u : (k == 2) ? 20u : (k == 3) ? 210u : 1350u;
+    for (int i = 1; i <= k; ++i) {
+        const uint32_t p = static_cast<uint32_t>(__builtin_ctzll(f)) >> 1;
+        f &= f - 1;
+        const uint32_t c = (i == 1)   ? p
+                           : (i == 2) ? p * (p - 1) / 2
+                           : (i == 3) ? p * (p - 1) * (p - 2) / 6
+                                      : p * (p - 1) * (p - 2) * (p - 3) / 24;
+        r += c;
+    }
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T cfd_product(uint64_t q, uint64_t site, uint64_t f, int dist,
+                                         const T *cfd) {
+    if (dist == 0) return T(1);
+    T c = cfd[320 + 10];
+    while (f) {
+        const uint32_t p = static_cast<uint32_t>(__builtin_ctzll(f)) >> 1;
+        f &= f - 1;
+        const uint32_t g = static_cast<uint32_t>(q >> (2 * p)) & 3u;
+        const uint32_t o = static_cast<uint32_t>(site >> (2 * p)) & 3u;
+        c *= cfd[(p << 4) | (g << 2) | (o ^ 3u)];
+    }
+    return c;
+}
+
+__device__ __forceinline__ bool bexit(int method, double m, double c, double cut) {
+    switch (method) {
+    case ISSL_METHOD_MIT_AND_CFD: return m > cut && c > cut;
+    case ISSL_METHOD_MIT_OR_CFD: return m > cut || c > cut;
+    case ISSL_METHOD_AVG_MIT_CFD: return ((m + c) / 2.0) > cut;
+    case ISSL_METHOD_MIT: return m > cut;
+    default: return c > cut;
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(v), src));
+    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(v >> 32), src));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// ------------------------------------------------------------ plan / scan
+
+struct PlanArgs {
+    const uint64_t *guides;
+    uint64_t n;
+    const uint32_t *lstart;
+    const SliceMeta *meta;
+    int S;
+    uint64_t n_sites;
+    uint32_t chunk;
+    BatchWs ws;
+};
+
+__global__ void k_plan(PlanArgs a) {
+    const uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (g >= a.n) return;
+    const uint64_t q = a.guides[g] & kSigBits;
+    uint64_t run = 0;
+    uint64_t *pre = a.ws.pre + g * (a.S + 1);
+    uint64_t *delta = a.ws.delta + g * a.S;
+    for (int s = 0; s < a.S; ++s) {
+        const SliceMeta m = a.meta[s];
+        uint32_t key = 0;
+        for (uint32_t j = 0; j < m.width; ++j)
+            key |= static_cast<uint32_t>((q >> (2 * m.pos[j])) & 3ULL) << (2 * j);
+        const uint32_t b = a.lstart[m.lbase + key];
+        const uint32_t e = a.lstart[m.lbase + key + 1];
+        pre[s] = run;
+        // posting index of position p in slice s: s*N + b + (p - run)
+        delta[s] = static_cast<uint64_t>(s) * a.n_sites + b - run;
+        run += e - b;
+    }
+    pre[a.S] = run;
+    uint64_t c = (run + kMaxTiles - 1) / kMaxTiles;
+    c = (c + 63) / 64 * 64;
+    if (c < a.chunk) c = a.chunk;
+    a.ws.gchunk[g] = static_cast<uint32_t>(c);
+    a.ws.gtiles[g] = static_cast<uint32_t>((run + c - 1) / c);
+}
+
+constexpr int kScanBlock = 1024;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan(BatchWs ws, uint64_t n) {
+    __shared__ uint32_t s_part[kScanBlock];
+    const int t = threadIdx.x;
+    const uint64_t per = (n + kScanBlock - 1) / kScanBlock;
+    const uint64_t lo = per * t, hi = std::min<uint64_t>(n, lo + per);
+    uint32_t sum = 0;
+    for (uint64_t i = lo; i < hi; ++i) sum += ws.gtiles[i];
+    s_part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        const uint32_t v = t >= off ? s_part[t - off] : 0u;
+        __syncthreads();
+        s_part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = s_part[t] - sum;
+    for (uint64_t i = lo; i < hi; ++i) {
+        ws.tile_base[i] = run;
+        run += ws.gtiles[i];
+    }
+    if (t == kScanBlock - 1) {
+        ws.tile_base[n] = s_part[t];
+        ws.ctrl[0] = 0;            // queue counter
+        ws.ctrl[1] = s_part[t];    // total tiles
+    }
+}
+
+__global__ void k_fillmap(BatchWs ws, uint64_t n) {
+    const uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t b = ws.tile_base[g], k = ws.gtiles[g];
+    for (uint32_t i = 0; i < k; ++i) ws.tile_map[b + i] = (g << 32) | i;
+}
+
+// ------------------------------------------------------------ tiles
+
+struct TileArgs {
+    const uint64_t *guides;
+    const uint64_t *site;   // signature-inline words
+    const uint64_t *post;   // postings (escape occ, ids)
+    const SliceMeta *meta;
+    const void *mit;        // float or double by precision
+    const void *cfd;
+    int S;
+    int use_cut_distinct;   // mark distinct sites here (no early exit armed)
+    int calc_mit, calc_cfd;
+    uint64_t *distinct;
+    BatchWs ws;
+};
+
+// Postings in flight per lane.
+template <int PREC, int DEPTH>
+__global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
+    using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
+    __shared__ Real s_cfd[336];
+    __shared__ uint64_t s_fold[kMaxSlices];
+    __shared__ uint64_t s_pre[kTileWaves][kMaxSlices + 1];
+    __shared__ uint64_t s_delta[kTileWaves][kMaxSlices];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const Real *cfdg = static_cast<const Real *>(a.cfd);
+    const Real *mitg = static_cast<const Real *>(a.mit);
+    for (int i = tid; i < 336; i += kTileBlock) s_cfd[i] = cfdg[i];
+    if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
+    __syncthreads();
+    const int S = a.S;
+    const uint32_t total = a.ws.ctrl[1];
+
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(a.ws.ctrl, 1u);
+        t = static_cast<uint32_t>(__shfl(static_cast<int>(t), 0));
+        if (t >= total) break;
+        const uint64_t gm = a.ws.tile_map[t];
+        const uint64_t g = gm >> 32;
+        const uint32_t k = static_cast<uint32_t>(gm);
+        const uint64_t q = a.guides[g] & kSigBits;
+        const uint64_t C = a.ws.gchunk[g];
+        if (lane <= S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
+        if (lane < S) s_delta[wave][lane] = a.ws.delta[g * S + lane];
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t P = s_pre[wave][S];
+        const uint64_t a0 = k * C;
+        const uint64_t a1 = std::min<uint64_t>(P, a0 + C);
+
+        Real am = 0, ac = 0;
+        F2 fm{0.f, 0.f}, fc{0.f, 0.f};
+        uint64_t occ_sum = 0, sites = 0;
+        int cur = 0;
+        {
+            const uint64_t p0 = a0 + lane;
+            while (cur < S - 1 && p0 >= s_pre[wave][cur + 1]) ++cur;
+        }
+        for (uint64_t base = a0; base < a1; base += 64 * DEPTH) {
+            uint64_t w[DEPTH], addr[DEPTH];
+            int sl[DEPTH];
+#pragma unroll
+            for (int d = 0; d < DEPTH; ++d) {
+                const uint64_t pos = base + 64 * d + lane;
+                sl[d] = -1;
+                addr[d] = 0;
+                w[d] = 0;
+                if (pos < a1) {
+                    while (pos >= s_pre[wave][cur + 1]) ++cur;
+                    sl[d] = cur;
+                    addr[d] = pos + s_delta[wave][cur];
+                    w[d] = a.site[addr[d]];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < DEPTH; ++d) {
+                if (sl[d] < 0) continue;
+                const uint64_t sg = w[d] & kSigBits;
+                const uint64_t f = bfold(q ^ sg);
+                const int dist = __popcll(f);
+                if (dist > 4) continue;
+                bool first = true;
+                for (int s2 = 0; s2 < sl[d]; ++s2)
+                    if ((f & s_fold[s2]) == 0) {
+                        first = false;
+                        break;
+                    }
+                if (!first) continue;
+                uint32_t occ = static_cast<uint32_t>(w[d] >> 40);
+                if (occ == kOccEscape || a.use_cut_distinct) {
+                    const uint64_t e = a.post[addr[d]];
+                    occ = static_cast<uint32_t>(e >> 32);
+                    if (a.use_cut_distinct)
+                        atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
+                                     (static_cast<uint32_t>(e) >> 6),
+                                 1ULL << (static_cast<uint32_t>(e) & 63));
+                }
+                occ_sum += occ;
+                sites += 1;
+                const Real ro = static_cast<Real>(occ);
+                Real tm = 0, tc = 0;
+                if (a.calc_mit && dist > 0) tm = mitg[brank(f, dist)] * ro;
+                if (a.calc_cfd) tc = cfd_product<Real>(q, sg, f, dist, s_cfd) * ro;
+                if constexpr (PREC == ISSL_PREC_FP32C) {
+                    fm = f2_add(fm, tm);
+                    fc = f2_add(fc, tc);
+                } else {
+                    am += tm;
+                    ac += tc;
+                }
+            }
+        }
+        // Fixed butterfly: every lane ends with the same, order-determined value.
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            occ_sum += shfl64(occ_sum, lane ^ off);
+            sites += shfl64(sites, lane ^ off);
+            if constexpr (PREC == ISSL_PREC_FP32C) {
+                F2 om{__shfl_xor(fm.s, off), __shfl_xor(fm.c, off)};
+                F2 oc{__shfl_xor(fc.s, off), __shfl_xor(fc.c, off)};
+                fm = f2_merge(fm, om);
+                fc = f2_merge(fc, oc);
+            } else {
+                am += __shfl_xor(am, off);
+                ac += __shfl_xor(ac, off);
+            }
+        }
+        if (lane == 0) {
+            TilePart tp;
+            if constexpr (PREC == ISSL_PREC_FP32C) {
+                tp.f.mh = fm.s;
+                tp.f.ml = fm.c;
+                tp.f.ch = fc.s;
+                tp.f.cl = fc.c;
+            } else {
+                tp.d.m = am;
+                tp.d.c = ac;
+            }
+            tp.occ = occ_sum;
+            tp.sites = sites;
+            a.ws.part[t] = tp;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ------------------------------------------------------------ finalize
+
+struct FinArgs {
+    const uint64_t *guides;
+    const uint64_t *site;
+    const uint64_t *post;
+    const SliceMeta *meta;
+    const void *mit;
+    const void *cfd;
+    int S;
+    uint64_t n;
+    ScoreScalars sc;
+    DevResults out;
+    uint64_t *distinct;     // mark sites up to the exit (early exit armed)
+    BatchWs ws;
+};
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
+    using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
+    constexpr int W = 256 / 64;
+    __shared__ uint64_t s_fold[kMaxSlices];
+    __shared__ uint64_t s_pre[W][kMaxSlices + 1];
+    __shared__ uint64_t s_delta[W][kMaxSlices];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
+    __syncthreads();
+    const uint64_t g = blockIdx.x * static_cast<uint64_t>(W) + wave;
+    if (g >= a.n) return;
+    const int S = a.S;
+    const uint32_t base = a.ws.tile_base[g];
+    const uint32_t K = a.ws.tile_base[g + 1] - base;
+    TilePart mine{};
+    if (lane < static_cast<int>(K)) mine = a.ws.part[base + lane];
+
+    F2 fm{0.f, 0.f}, fc{0.f, 0.f};
+    double dm = 0, dc = 0;
+    uint64_t occ_sum = 0, sites = 0;
+    int exit_tile = -1;
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint64_t o = shfl64(mine.occ, k);
+        const uint64_t si = shfl64(mine.sites, k);
+        F2 pm = fm, pc = fc;
+        double qm = dm, qc = dc;
+        if constexpr (PREC == ISSL_PREC_FP32C) {
+            fm = f2_merge(fm, F2{__shfl(mine.f.mh, k), __shfl(mine.f.ml, k)});
+            fc = f2_merge(fc, F2{__shfl(mine.f.ch, k), __shfl(mine.f.cl, k)});
+        } else {
+            dm += __shfl(mine.d.m, k);
+            dc += __shfl(mine.d.c, k);
+        }
+        const double vm = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
+        const double vc = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
+        if (a.sc.use_cut && bexit(a.sc.method, vm, vc, a.sc.cut)) {
+            // Restore the sums before this tile and rescan it hit by hit.
+            fm = pm;
+            fc = pc;
+            dm = qm;
+            dc = qc;
+            exit_tile = static_cast<int>(k);
+            break;
+        }
+        occ_sum += o;
+        sites += si;
+    }
+
+    const uint64_t q = a.guides[g] & kSigBits;
+    if (exit_tile >= 0 || a.distinct) {
+        if (lane <= S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
+        if (lane < S) s_delta[wave][lane] = a.ws.delta[g * S + lane];
+        __builtin_amdgcn_wave_barrier();
+    }
+    const uint64_t C = a.ws.gchunk[g];
+    if (exit_tile >= 0) {
+        const Real *mitg = static_cast<const Real *>(a.mit);
+        const Real *cfdg = static_cast<const Real *>(a.cfd);
+        const uint64_t P = s_pre[wave][S];
+        const uint64_t a0 = static_cast<uint64_t>(exit_tile) * C;
+        const uint64_t a1 = std::min<uint64_t>(P, a0 + C);
+        int cur = 0;
+        bool done = false;
+        for (uint64_t b0 = a0; b0 < a1 && !done; b0 += 64) {
+            const uint64_t pos = b0 + lane;
+            bool hit = false;
+            Real tm = 0, tc = 0;
+            uint32_t occ = 0;
+            uint64_t addr = 0;
+            if (pos < a1) {
+                while (pos >= s_pre[wave][cur + 1]) ++cur;
+                addr = pos + s_delta[wave][cur];
+                const uint64_t w = a.site[addr];
+                const uint64_t sg = w & kSigBits;
+                const uint64_t f = bfold(q ^ sg);
+                const int dist = __popcll(f);
+                if (dist <= 4) {
+                    hit = true;
+                    for (int s2 = 0; s2 < cur; ++s2)
+                        if ((f & s_fold[s2]) == 0) {
+                            hit = false;
+                            break;
+                        }
+                }
+                if (hit) {
+                    occ = static_cast<uint32_t>(w >> 40);
+                    if (occ == kOccEscape) occ = static_cast<uint32_t>(a.post[addr] >> 32);
+                    const Real ro = static_cast<Real>(occ);
+                    if (a.sc.calc_mit && dist > 0) tm = mitg[brank(f, dist)] * ro;
+                    if (a.sc.calc_cfd) tc = cfd_product<Real>(q, sg, f, dist, cfdg) * ro;
+                }
+            }
+            uint64_t bal = __ballot(hit);
+            while (bal) {
+                const int h = __builtin_ctzll(bal);
+                bal &= bal - 1;
+                const Real hm = __shfl(tm, h), hc = __shfl(tc, h);
+                const uint32_t ho = static_cast<uint32_t>(__shfl(static_cast<int>(occ), h));
+                double vm, vc;
+                if constexpr (PREC == ISSL_PREC_FP32C) {
+                    fm = f2_add(fm, hm);
+                    fc = f2_add(fc, hc);
+                    vm = f2_value(fm);
+                    vc = f2_value(fc);
+                } else {
+                    dm += hm;
+                    dc += hc;
+                    vm = dm;
+                    vc = dc;
+                }
+                occ_sum += ho;
+                sites += 1;
+                if (a.distinct && lane == h)
+                    atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
+                                 (static_cast<uint32_t>(a.post[addr]) >> 6),
+                             1ULL << (static_cast<uint32_t>(a.post[addr]) & 63));
+                if (bexit(a.sc.method, vm, vc, a.sc.cut)) {
+                    done = true;
+                    break;
+                }
+            }
+        }
+    }
+    // Distinct sites of the tiles before the exit tile (or all tiles).
+    if (a.distinct) {
+        const uint64_t P = s_pre[wave][S];
+        const uint64_t end = exit_tile >= 0 ? static_cast<uint64_t>(exit_tile) * C : P;
+        int cur = 0;
+        for (uint64_t b0 = 0; b0 < end; b0 += 64) {
+            const uint64_t pos = b0 + lane;
+            if (pos >= end) continue;
+            while (pos >= s_pre[wave][cur + 1]) ++cur;
+            const uint64_t addr = pos + s_delta[wave][cur];
+            const uint64_t f = bfold(q ^ (a.site[addr] & kSigBits));
+            if (__popcll(f) > 4) continue;
+            bool first = true;
+            for (int s2 = 0; s2 < cur; ++s2)
+                if ((f & s_fold[s2]) == 0) {
+                    first = false;
+                    break;
+                }
+            if (!first) continue;
+            const uint32_t id = static_cast<uint32_t>(a.post[addr]);
+            atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (id >> 6), 1ULL << (id & 63));
+        }
+    }
+    if (lane != 0) return;
+    const double smit = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
+    const double scfd = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
+    const double vm = 10000.0 / (100.0 + smit);
+    const double vc = 10000.0 / (100.0 + scfd);
+    const double th = a.sc.threshold;
+    bool reject;
+    switch (a.sc.method) {
+    case ISSL_METHOD_MIT_AND_CFD: reject = vm < th && vc < th; break;
+    case ISSL_METHOD_MIT_OR_CFD: reject = vm < th || vc < th; break;
+    case ISSL_METHOD_AVG_MIT_CFD: reject = (vm + vc) / 2 < th; break;
+    case ISSL_METHOD_MIT: reject = vm < th; break;
+    default: reject = vc < th; break;
+    }
+    const DevResults &o = a.out;
+    if (o.mit) o.mit[g] = a.sc.calc_mit ? vm : -1.0;
+    if (o.cfd) o.cfd[g] = a.sc.calc_cfd ? vc : -1.0;
+    if (o.mit_sum) o.mit_sum[g] = smit;
+    if (o.cfd_sum) o.cfd_sum[g] = scfd;
+    if (o.occ) o.occ[g] = occ_sum;
+    if (o.sites) o.sites[g] = sites;
+    if (o.pass) o.pass[g] = reject ? '0' : '1';
+}
+
+// ------------------------------------------------------------ launcher
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+int carve(Workspace &ws, uint64_t n, int S, BatchWs *b) {
+    const size_t sz_pre = align256(n * (S + 1) * 8);
+    const size_t sz_delta = align256(n * std::max(S, 1) * 8);
+    const size_t sz_u32 = align256(n * 4);
+    const size_t sz_base = align256((n + 1) * 4);
+    const size_t sz_ctrl = 256;
+    const size_t sz_map = align256(n * kMaxTiles * 8);
+    const size_t sz_part = align256(n * kMaxTiles * sizeof(TilePart));
+    const size_t total = sz_pre + sz_delta + 2 * sz_u32 + sz_base + sz_ctrl + sz_map + sz_part;
+    int rc = ws.tiles.reserve(total);
+    if (rc != ISSL_OK) return rc;
+    char *p = static_cast<char *>(ws.tiles.p);
+    b->pre = reinterpret_cast<uint64_t *>(p);
+    p += sz_pre;
+    b->delta = reinterpret_cast<uint64_t *>(p);
+    p += sz_delta;
+    b->gchunk = reinterpret_cast<uint32_t *>(p);
+    p += sz_u32;
+    b->gtiles = reinterpret_cast<uint32_t *>(p);
+    p += sz_u32;
+    b->tile_base = reinterpret_cast<uint32_t *>(p);
+    p += sz_base;
+    b->ctrl = reinterpret_cast<uint32_t *>(p);
+    p += sz_ctrl;
+    b->tile_map = reinterpret_cast<uint64_t *>(p);
+    p += sz_map;
+    b->part = reinterpret_cast<TilePart *>(p);
+    return ISSL_OK;
+}
+
+template <int PREC, int DEPTH>
+void launch_tiles(const TileArgs &ta, int grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_tiles<PREC, DEPTH>), dim3(grid), dim3(kTileBlock), 0, st, ta);
+}
+
+template <int PREC>
+int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st) {
+    switch (depth) {
+    case 1: launch_tiles<PREC, 1>(ta, grid, st); break;
+    case 2: launch_tiles<PREC, 2>(ta, grid, st); break;
+    case 4: launch_tiles<PREC, 4>(ta, grid, st); break;
+    case 8: launch_tiles<PREC, 8>(ta, grid, st); break;
+    case 16: launch_tiles<PREC, 16>(ta, grid, st); break;
+    default: return fail(ISSL_ERR_ARG, "tuning depth must be 1, 2, 4, 8 or 16");
+    }
+    return ISSL_OK;
+}
+
+int cu_count(int device) {
+    static int cached[64] = {0};
+    if (device >= 0 && device < 64 && cached[device]) return cached[device];
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (device >= 0 && device < 64) cached[device] = cus;
+    return cus;
+}
+
+}  // namespace
+
+int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
+                   int precision, int chunk, const issl_tuning &tn_in, const DevResults &out,
+                   uint64_t *distinct, Workspace &ws, hipStream_t st) {
+    if (n == 0) return ISSL_OK;
+    if (n > (0xFFFFFFFFULL / kMaxTiles)) return fail(ISSL_ERR_ARG, "too many guides for one launch");
+    issl_tuning tn = tn_in;
+    if (tn.chunk <= 0) tn.chunk = 512;
+    if (tn.depth <= 0) tn.depth = 4;
+    if (tn.grid_per_cu <= 0) tn.grid_per_cu = 8;
+    if (chunk > 0) tn.chunk = chunk;
+    if (tn.chunk % 64 != 0 || tn.chunk > (1 << 20))
+        return fail(ISSL_ERR_ARG, "chunk must be a multiple of 64 up to 2^20");
+    const int S = static_cast<int>(ix->S);
+    BatchWs b;
+    int rc = carve(ws, n, S, &b);
+    if (rc != ISSL_OK) return rc;
+
+    PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk), b};
+    const uint32_t gb = static_cast<uint32_t>((n + 255) / 256);
+    hipLaunchKernelGGL(k_plan, dim3(gb), dim3(256), 0, st, pa);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, st, b, n);
+    hipLaunchKernelGGL(k_fillmap, dim3(gb), dim3(256), 0, st, b, n);
+
+    const bool fp32 = precision == ISSL_PREC_FP32C;
+    TileArgs ta;
+    ta.guides = d_guides;
+    ta.site = ix->d_inline;
+    ta.post = ix->d_post;
+    ta.meta = ix->d_meta;
+    ta.mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
+    ta.cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
+    ta.S = S;
+    ta.use_cut_distinct = (distinct && !sc.use_cut) ? 1 : 0;
+    ta.calc_mit = sc.calc_mit;
+    ta.calc_cfd = sc.calc_cfd;
+    ta.distinct = distinct;
+    ta.ws = b;
+    const uint64_t max_tiles = n * kMaxTiles;
+    uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
+    const uint64_t waves_needed = std::max<uint64_t>(1, std::min<uint64_t>(max_tiles, grid * kTileWaves));
+    grid = std::min<uint64_t>(grid, (waves_needed + kTileWaves - 1) / kTileWaves);
+    timing_mark(ix, st);
+    rc = fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st)
+              : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st);
+    if (rc != ISSL_OK) return rc;
+    timing_mark(ix, st);
+
+    FinArgs fa;
+    fa.guides = d_guides;
+    fa.site = ix->d_inline;
+    fa.post = ix->d_post;
+    fa.meta = ix->d_meta;
+    fa.mit = ta.mit;
+    fa.cfd = ta.cfd;
+    fa.S = S;
+    fa.n = n;
+    fa.sc = sc;
+    fa.out = out;
+    fa.distinct = (distinct && sc.use_cut) ? distinct : nullptr;
+    fa.ws = b;
+    const uint32_t fb = static_cast<uint32_t>((n + 3) / 4);
+    if (fp32)
+        hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP32C>), dim3(fb), dim3(256), 0, st, fa);
+    else
+        hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP64>), dim3(fb), dim3(256), 0, st, fa);
+    ISSL_HIP(hipGetLastError());
+    return ISSL_OK;
+}
+
+}  // namespace issl

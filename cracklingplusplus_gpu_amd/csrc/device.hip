@@ -1,0 +1,918 @@
+// device.hip -- resident ISSL index on one MI355X (gfx950), the PLAIN scoring
+// kernel and the C ABI of issl_gpu.h.
+//
+// Device layout of an index (all in HBM, resident until issl_index_close):
+//   d_sig    u64[N]        signatures, id order (the .issl sig block)
+//   d_post   u64[S*N]      posting entries occ<<32 | id, the .issl order
+//   d_inline u64[S*N]      same order, sig | occ<<40 (BATCHED form)
+//   d_lstart u32[sum(4^w+1)] per slice, start of each key's list within the
+//                          slice's N entries, plus an end sentinel
+// A guide's candidate sequence is the concatenation, slice by slice, of the
+// lists its keys select: exactly the order of the reference's nested loops
+// (lib/ISSLScoringModule.cpp:231-250).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "device.hpp"
+#include "issl_index.h"
+
+namespace issl {
+
+// ------------------------------------------------------------ buffers
+
+int DevBuf::reserve(size_t want) {
+    if (want <= bytes) return ISSL_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    const size_t sz = std::max<size_t>(want, 4096);
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        p = nullptr;
+        return fail(ISSL_ERR_NOMEM, "device allocation of " + std::to_string(sz) + " bytes failed");
+    }
+    bytes = sz;
+    return ISSL_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int HostBuf::reserve(size_t want) {
+    if (want <= bytes) return ISSL_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    const size_t sz = std::max<size_t>(want, 4096);
+    if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        return fail(ISSL_ERR_NOMEM, "pinned host allocation failed");
+    }
+    bytes = sz;
+    return ISSL_OK;
+}
+
+void HostBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// ------------------------------------------------------------ device helpers
+
+__device__ __forceinline__ uint64_t dev_fold(uint64_t x) {
+    return ((x >> 1) & kFoldLo) | (x & kFoldLo);
+}
+
+// Combinadic rank of the mismatch position set of a fold mask with 1..4 bits.
+__device__ __forceinline__ uint32_t mit_rank(uint64_t f, int k) {
+    uint32_t r = (k == 1) ? 0u : (k == 2) ? 20u : (k == 3) ? 210u : 1350u;
+#pragma unroll
+    for (int i = 1; i <= 4; ++i) {
+        if (i > k) break;
+        const uint32_t p = static_cast<uint32_t>(__builtin_ctzll(f)) >> 1;
+        f &= f - 1;
+        uint32_t c;
+        if (i == 1)
+            c = p;
+        else if (i == 2)
+            c = p * (p - 1) / 2;
+        else if (i == 3)
+            c = p * (p - 1) * (p - 2) / 6;
+        else
+            c = p * (p - 1) * (p - 2) * (p - 3) / 24;
+        r += c;
+    }
+    return r;
+}
+
+// CFD factor of one site in FP64 (lib/ISSLScoringModule.cpp:308-361): the
+// NGG PAM penalty times the position penalty of every differing position,
+// in ascending position order.  Exact matches score 1.
+__device__ __forceinline__ double cfd_site64(uint64_t q, uint64_t site, uint64_t f, int dist,
+                                             const double *__restrict__ cfd) {
+    if (dist == 0) return 1.0;
+    double c = cfd[320 + 10];
+    while (f) {
+        const uint32_t p = static_cast<uint32_t>(__builtin_ctzll(f)) >> 1;
+        f &= f - 1;
+        const uint32_t g = static_cast<uint32_t>(q >> (2 * p)) & 3u;
+        const uint32_t o = static_cast<uint32_t>(site >> (2 * p)) & 3u;
+        c *= cfd[(p << 4) | (g << 2) | (o ^ 3u)];
+    }
+    return c;
+}
+
+// Exit tests of lib/ISSLScoringModule.cpp:365-395.
+__device__ __forceinline__ bool exit_test(int method, double m, double c, double cut) {
+    switch (method) {
+    case ISSL_METHOD_MIT_AND_CFD: return m > cut && c > cut;
+    case ISSL_METHOD_MIT_OR_CFD: return m > cut || c > cut;
+    case ISSL_METHOD_AVG_MIT_CFD: return ((m + c) / 2.0) > cut;
+    case ISSL_METHOD_MIT: return m > cut;
+    default: return c > cut;
+    }
+}
+
+// Transform and call (lib/ISSLScoringModule.cpp:404-455).
+__device__ void write_result(const DevResults &o, uint64_t g, const ScoreScalars &sc, double smit,
+                             double scfd, uint64_t occ_sum, uint64_t site_sum) {
+    const double vm = 10000.0 / (100.0 + smit);
+    const double vc = 10000.0 / (100.0 + scfd);
+    const double t = sc.threshold;
+    bool reject;
+    switch (sc.method) {
+    case ISSL_METHOD_MIT_AND_CFD: reject = vm < t && vc < t; break;
+    case ISSL_METHOD_MIT_OR_CFD: reject = vm < t || vc < t; break;
+    case ISSL_METHOD_AVG_MIT_CFD: reject = (vm + vc) / 2 < t; break;
+    case ISSL_METHOD_MIT: reject = vm < t; break;
+    default: reject = vc < t; break;
+    }
+    if (o.mit) o.mit[g] = sc.calc_mit ? vm : -1.0;
+    if (o.cfd) o.cfd[g] = sc.calc_cfd ? vc : -1.0;
+    if (o.mit_sum) o.mit_sum[g] = smit;
+    if (o.cfd_sum) o.cfd_sum[g] = scfd;
+    if (o.occ) o.occ[g] = occ_sum;
+    if (o.sites) o.sites[g] = site_sum;
+    if (o.pass) o.pass[g] = reject ? '0' : '1';
+}
+
+// ------------------------------------------------------------ PLAIN kernel
+
+struct PlainArgs {
+    const uint64_t *sig;
+    const uint64_t *post;
+    const uint32_t *lstart;
+    const SliceMeta *meta;
+    const double *mit64;
+    const double *cfd64;
+    const uint64_t *guides;
+    uint64_t n_sites;
+    int n_slices;
+    ScoreScalars sc;
+    DevResults out;
+    uint64_t *distinct;
+    const uint64_t *cand_off;
+    uint32_t *cand_ids;
+};
+
+constexpr int kPlainBlock = 256;
+constexpr int kPlainPer = 4;  // postings per thread per chunk
+
+// One workgroup per guide.  Each chunk of BLOCK*PER postings of the guide's
+// slice-major sequence is read with coalesced loads (consecutive threads,
+// consecutive entries), the site signatures are gathered, and sites within
+// 4 mismatches that no earlier slice lists (the reference's first-seen
+// toggle, ISSLScoringModule.cpp:252-258, as a stateless rule: slice s'
+// lists the site iff the fold misses all of s'’s positions) become hits.
+// Hits are compacted into LDS in sequence order with wave ballots; their
+// FP64 terms are formed in parallel and one lane adds them in order and
+// applies the early exit after each, so sums, counts and the exit point are
+// the reference's.
+template <int BLOCK, int PER>
+__global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
+    constexpr int WAVES = BLOCK / 64;
+    constexpr int CH = BLOCK * PER;
+    __shared__ uint64_t s_beg[kMaxSlices];
+    __shared__ uint64_t s_pre[kMaxSlices + 1];
+    __shared__ uint64_t s_fold[kMaxSlices];
+    __shared__ uint32_t s_wcnt[2][PER][WAVES];
+    __shared__ double s_hmit[CH];
+    __shared__ double s_hcfd[CH];
+    __shared__ uint32_t s_hocc[CH];
+    __shared__ uint32_t s_accept;
+    __shared__ int s_exited;
+
+    const uint64_t g = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    // Signatures are 40-bit (20 positions); higher bits are ignored.
+    const uint64_t q = a.guides[g] & kSigBits;
+    const int S = a.n_slices;
+
+    if (tid < S) {
+        const SliceMeta m = a.meta[tid];
+        uint32_t key = 0;
+        for (uint32_t j = 0; j < m.width; ++j)
+            key |= static_cast<uint32_t>((q >> (2 * m.pos[j])) & 3ULL) << (2 * j);
+        const uint32_t b = a.lstart[m.lbase + key];
+        const uint32_t e = a.lstart[m.lbase + key + 1];
+        s_beg[tid] = static_cast<uint64_t>(tid) * a.n_sites + b;
+        s_pre[tid + 1] = e - b;
+        s_fold[tid] = m.fold;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        s_pre[0] = 0;
+        for (int s = 0; s < S; ++s) s_pre[s + 1] += s_pre[s];
+        s_exited = 0;
+    }
+    __syncthreads();
+    const uint64_t P = s_pre[S];
+
+    double smit = 0.0, scfd = 0.0;  // meaningful in thread 0
+    uint64_t occ_sum = 0, site_sum = 0;
+    uint64_t emitted = 0;
+    const uint64_t cand_base = a.cand_ids ? a.cand_off[g] : 0;
+    int cur = 0;
+    int parity = 0;
+
+    for (uint64_t base = 0; base < P; base += CH) {
+        uint64_t ent[PER], site[PER];
+        int sl[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint64_t i = base + static_cast<uint64_t>(k) * BLOCK + tid;
+            sl[k] = -1;
+            ent[k] = 0;
+            if (i < P) {
+                while (i >= s_pre[cur + 1]) ++cur;
+                sl[k] = cur;
+                ent[k] = a.post[s_beg[cur] + (i - s_pre[cur])];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            site[k] = sl[k] >= 0 ? (a.sig[static_cast<uint32_t>(ent[k])] & kSigBits) : 0;
+
+        bool hit[PER];
+        uint64_t fold[PER];
+        uint32_t rank[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            hit[k] = false;
+            fold[k] = 0;
+            if (sl[k] >= 0) {
+                const uint64_t f = dev_fold(q ^ site[k]);
+                fold[k] = f;
+                if (__popcll(f) <= 4) {
+                    bool first = true;
+                    for (int t = 0; t < sl[k]; ++t)
+                        if ((f & s_fold[t]) == 0) {
+                            first = false;
+                            break;
+                        }
+                    hit[k] = first;
+                }
+            }
+            const uint64_t b = __ballot(hit[k]);
+            rank[k] = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+            if (lane == 0) s_wcnt[parity][k][wave] = static_cast<uint32_t>(__popcll(b));
+        }
+        __syncthreads();
+        uint32_t H = 0;
+        uint32_t off[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            uint32_t before = 0, all = 0;
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) {
+                const uint32_t c = s_wcnt[parity][k][w];
+                before += (w < wave) ? c : 0u;
+                all += c;
+            }
+            off[k] = H + before + rank[k];
+            H += all;
+        }
+        parity ^= 1;
+        if (H == 0) continue;  // uniform across the workgroup
+
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (!hit[k]) continue;
+            const int dist = __popcll(fold[k]);
+            const uint32_t occ = static_cast<uint32_t>(ent[k] >> 32);
+            double tm = 0.0, tc = 0.0;
+            if (a.sc.calc_mit && dist > 0) tm = a.mit64[mit_rank(fold[k], dist)] * static_cast<double>(occ);
+            if (a.sc.calc_cfd) tc = cfd_site64(q, site[k], fold[k], dist, a.cfd64) * static_cast<double>(occ);
+            s_hmit[off[k]] = tm;
+            s_hcfd[off[k]] = tc;
+            s_hocc[off[k]] = occ;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = H;
+            int ex = 0;
+            for (uint32_t h = 0; h < H; ++h) {
+                smit += s_hmit[h];
+                scfd += s_hcfd[h];
+                occ_sum += s_hocc[h];
+                site_sum += 1;
+                if (a.sc.use_cut && exit_test(a.sc.method, smit, scfd, a.sc.cut)) {
+                    acc = h + 1;
+                    ex = 1;
+                    break;
+                }
+            }
+            s_accept = acc;
+            s_exited = ex;
+        }
+        __syncthreads();
+        const uint32_t acc = s_accept;
+        if (a.distinct || a.cand_ids) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                if (!hit[k] || off[k] >= acc) continue;
+                const uint32_t id = static_cast<uint32_t>(ent[k]);
+                if (a.distinct)
+                    atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (id >> 6),
+                             1ULL << (id & 63));
+                if (a.cand_ids) a.cand_ids[cand_base + emitted + off[k]] = id;
+            }
+        }
+        emitted += acc;
+        if (s_exited) break;
+    }
+    if (tid == 0) write_result(a.out, g, a.sc, smit, scfd, occ_sum, site_sum);
+}
+
+int launch_plain(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
+                 const DevResults &out, uint64_t *distinct, const uint64_t *cand_off,
+                 uint32_t *cand_ids, hipStream_t stream) {
+    if (n == 0) return ISSL_OK;
+    if (n > 0x7FFFFFFFULL) return fail(ISSL_ERR_ARG, "too many guides for one launch");
+    PlainArgs a;
+    a.sig = ix->d_sig;
+    a.post = ix->d_post;
+    a.lstart = ix->d_lstart;
+    a.meta = ix->d_meta;
+    a.mit64 = ix->d_mit64;
+    a.cfd64 = ix->d_cfd64;
+    a.guides = d_guides;
+    a.n_sites = ix->n;
+    a.n_slices = static_cast<int>(ix->S);
+    a.sc = sc;
+    a.out = out;
+    a.distinct = distinct;
+    a.cand_off = cand_off;
+    a.cand_ids = cand_ids;
+    timing_mark(ix, stream);
+    hipLaunchKernelGGL((k_plain<kPlainBlock, kPlainPer>), dim3(static_cast<uint32_t>(n)),
+                       dim3(kPlainBlock), 0, stream, a);
+    ISSL_HIP(hipGetLastError());
+    timing_mark(ix, stream);
+    return ISSL_OK;
+}
+
+// ------------------------------------------------------------ index kernels
+
+// Signature-inline site words: sig | min(occ, escape)<<40, same order as the
+// postings.
+__global__ void k_inline(const uint64_t *__restrict__ post, const uint64_t *__restrict__ sig,
+                         uint64_t *__restrict__ out, uint64_t total) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t e = post[i];
+        const uint32_t occ = static_cast<uint32_t>(e >> 32);
+        const uint64_t o = occ < kOccEscape ? occ : kOccEscape;
+        out[i] = (sig[static_cast<uint32_t>(e)] & kSigBits) | (o << 40);
+    }
+}
+
+// Index check: each entry of slice s sits in the list of its own key, its id
+// is < N, and every id occurs exactly once in the slice.
+__global__ void k_validate(const uint64_t *__restrict__ post, const uint64_t *__restrict__ sig,
+                           const uint32_t *__restrict__ lstart, SliceMeta m, uint64_t s,
+                           uint64_t n, uint32_t *__restrict__ seen,
+                           unsigned long long *__restrict__ bad) {
+    const uint32_t nkeys = 1u << (2 * m.width);
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        // Largest key whose list starts at or before i.
+        uint32_t lo = 0, hi = nkeys;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (lstart[m.lbase + mid] <= i)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint64_t e = post[s * n + i];
+        const uint64_t id = e & 0xFFFFFFFFULL;
+        bool good = id < n;
+        if (good) {
+            uint32_t key = 0;
+            const uint64_t sg = sig[id];
+            for (uint32_t j = 0; j < m.width; ++j)
+                key |= static_cast<uint32_t>((sg >> (2 * m.pos[j])) & 3ULL) << (2 * j);
+            good = key == lo;
+            if (atomicAdd(seen + id, 1u) != 0u) good = false;
+        }
+        if (!good) atomicAdd(bad, 1ULL);
+    }
+}
+
+void timing_mark(issl_index *ix, hipStream_t stream) {
+    if (!ix->timing) return;
+    if (ix->ev_used == ix->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        ix->ev_pool.push_back(e);
+    }
+    (void)hipEventRecord(ix->ev_pool[ix->ev_used++], stream);
+}
+
+int build_inline_layout(issl_index *ix) {
+    const uint64_t total = ix->S * ix->n;
+    if (total == 0) return ISSL_OK;
+    hipLaunchKernelGGL(k_inline, dim3(4096), dim3(256), 0, ix->stream, ix->d_post, ix->d_sig,
+                       ix->d_inline, total);
+    ISSL_HIP(hipGetLastError());
+    return ISSL_OK;
+}
+
+}  // namespace issl
+
+using namespace issl;
+
+// ============================================================ C ABI
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+void free_index(issl_index *ix) {
+    if (!ix) return;
+    (void)hipSetDevice(ix->device);
+    if (ix->stream) (void)hipStreamSynchronize(ix->stream);
+    void *ptrs[] = {ix->d_sig, ix->d_post, ix->d_inline, ix->d_lstart, ix->d_meta, ix->d_mit64,
+                    ix->d_mit32, ix->d_cfd64, ix->d_cfd32, ix->d_distinct};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    ix->ws.guides.release();
+    ix->ws.out.release();
+    ix->ws.tiles.release();
+    ix->ws.scratch.release();
+    ix->ws.pinned.release();
+    for (hipEvent_t e : ix->ev_pool) (void)hipEventDestroy(e);
+    if (ix->stream) (void)hipStreamDestroy(ix->stream);
+    delete ix;
+}
+
+template <typename T>
+int dev_alloc(issl_index *ix, T **p, uint64_t count) {
+    const size_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+    void *q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess)
+        return fail(ISSL_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    *p = static_cast<T *>(q);
+    ix->device_bytes += bytes;
+    return ISSL_OK;
+}
+
+// Host -> device copy of a large pageable array through pinned staging.
+int upload(issl_index *ix, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return ISSL_OK;
+    const size_t stage = size_t(64) << 20;
+    if (bytes <= stage) {
+        ISSL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return ISSL_OK;
+    }
+    int rc = ix->ws.pinned.reserve(2 * stage);
+    if (rc != ISSL_OK) {
+        ISSL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return ISSL_OK;
+    }
+    char *pin = static_cast<char *>(ix->ws.pinned.p);
+    hipEvent_t ev[2];
+    ISSL_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    ISSL_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    bool used[2] = {false, false};
+    size_t off = 0;
+    int slot = 0;
+    int status = ISSL_OK;
+    while (off < bytes && status == ISSL_OK) {
+        const size_t c = std::min(stage, bytes - off);
+        if (used[slot] && hipEventSynchronize(ev[slot]) != hipSuccess) status = ISSL_ERR_DEVICE;
+        std::memcpy(pin + slot * stage, static_cast<const char *>(src) + off, c);
+        if (hipMemcpyAsync(static_cast<char *>(dst) + off, pin + slot * stage, c,
+                           hipMemcpyHostToDevice, ix->stream) != hipSuccess ||
+            hipEventRecord(ev[slot], ix->stream) != hipSuccess)
+            status = ISSL_ERR_DEVICE;
+        used[slot] = true;
+        off += c;
+        slot ^= 1;
+    }
+    (void)hipStreamSynchronize(ix->stream);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (status != ISSL_OK) return fail(status, "index upload failed");
+    return ISSL_OK;
+}
+
+int check_params(const issl_index *ix, const issl_params *p, ScoreScalars *sc) {
+    if (!ix) return fail(ISSL_ERR_ARG, "null index");
+    if (!p) return fail(ISSL_ERR_ARG, "null params");
+    if (p->max_dist != 4)
+        return fail(ISSL_ERR_ARG,
+                    "max_dist must be 4: the reference scores dist <= 4 regardless of its argument");
+    if (p->method < 0 || p->method > 4) return fail(ISSL_ERR_ARG, "Scoring method is not valid.");
+    if (p->form != ISSL_FORM_PLAIN && p->form != ISSL_FORM_BATCHED)
+        return fail(ISSL_ERR_ARG, "unknown form");
+    if (p->precision != ISSL_PREC_FP64 && p->precision != ISSL_PREC_FP32C)
+        return fail(ISSL_ERR_ARG, "unknown precision");
+    if (p->form == ISSL_FORM_PLAIN && p->precision != ISSL_PREC_FP64)
+        return fail(ISSL_ERR_ARG, "the PLAIN form computes in FP64 only");
+    sc->method = p->method;
+    sc->calc_mit = p->method != ISSL_METHOD_CFD;
+    sc->calc_cfd = p->method != ISSL_METHOD_MIT;
+    sc->threshold = p->threshold;
+    // maximum_sum of ISSLScoringModule.cpp:225, formed in host FP64.
+    sc->cut = (10000.0 - p->threshold * 100) / p->threshold;
+    // t = 0 gives +inf and never exits; the GPU skips the tests then.
+    sc->use_cut = !(sc->cut == HUGE_VAL) && !(sc->cut != sc->cut);
+    return ISSL_OK;
+}
+
+}  // namespace
+
+extern "C" int issl_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" const char *issl_version(void) { return "issl-mi355x 0.1.0 (gfx950)"; }
+
+extern "C" int issl_encode(const char *text, uint64_t n, uint64_t stride, uint64_t *out) {
+    if ((n && (!text || !out)) || stride < 20) return fail(ISSL_ERR_ARG, "issl_encode: bad arguments");
+    for (uint64_t i = 0; i < n; ++i) {
+        const char *p = text + i * stride;
+        uint64_t s = 0;
+        for (unsigned j = 0; j < 20; ++j) {
+            const char c = p[j];
+            const uint64_t v = c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : 0;
+            s |= v << (2 * j);
+        }
+        out[i] = s;
+    }
+    return ok();
+}
+
+extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+                                      const uint64_t *sigs, const uint64_t *slice_masks,
+                                      const uint64_t *list_sizes, const uint64_t *entries,
+                                      int device, issl_index **out) {
+    if (!out) return fail(ISSL_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (n_sites >= (1ULL << 32)) return fail(ISSL_ERR_FORMAT, "ids are 32-bit: N must be < 2^32");
+    if (n_slices > static_cast<uint64_t>(kMaxSlices))
+        return fail(ISSL_ERR_ARG, "at most 64 slices are supported");
+    if ((n_sites && !sigs) || (n_slices && (!slice_masks || !list_sizes)) ||
+        (n_sites && n_slices && !entries))
+        return fail(ISSL_ERR_ARG, "null index array");
+    int ndev = issl_device_count();
+    if (ndev <= 0) return fail(ISSL_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(ISSL_ERR_ARG, "device ordinal out of range");
+
+    std::unique_ptr<issl_index, void (*)(issl_index *)> ix(new (std::nothrow) issl_index, free_index);
+    if (!ix) return fail(ISSL_ERR_NOMEM, "host allocation failed");
+    ix->device = device;
+    ix->n = n_sites;
+    ix->seq_len = seq_len;
+    ix->S = n_slices;
+    ix->masks.assign(slice_masks, slice_masks + n_slices);
+
+    // Slice metadata and per-slice list starts (ISSLScoringModule.cpp:100-198).
+    std::vector<uint32_t> lstart;
+    uint64_t sz_base = 0;
+    for (uint64_t s = 0; s < n_slices; ++s) {
+        uint8_t pos[64];
+        const int w = mask_positions(slice_masks[s], seq_len, pos);
+        if (w > kMaxWidth) return fail(ISSL_ERR_FORMAT, "slice wider than 16 positions");
+        SliceMeta m{};
+        m.width = static_cast<uint32_t>(w);
+        m.lbase = static_cast<uint32_t>(lstart.size());
+        m.fold = 0;
+        for (int j = 0; j < w; ++j) {
+            m.pos[j] = pos[j];
+            m.fold |= 1ULL << (2 * pos[j]);
+        }
+        const uint64_t nk = 1ULL << (2 * w);
+        if (lstart.size() + nk + 1 > 0xFFFFFFFFULL)
+            return fail(ISSL_ERR_FORMAT, "too many lists for 32-bit list offsets");
+        uint64_t run = 0;
+        for (uint64_t k = 0; k < nk; ++k) {
+            lstart.push_back(static_cast<uint32_t>(run));
+            run += list_sizes[sz_base + k];
+        }
+        if (run != n_sites)
+            return fail(ISSL_ERR_FORMAT, "slice " + std::to_string(s) + " list sizes sum to " +
+                                             std::to_string(run) + ", not N");
+        lstart.push_back(static_cast<uint32_t>(run));
+        sz_base += nk;
+        ix->max_w = std::max(ix->max_w, w);
+        ix->meta_h.push_back(m);
+    }
+    ix->n_lists = sz_base;
+    if (n_slices && count_bad_ids(entries, n_slices * n_sites, n_sites) != 0)
+        return fail(ISSL_ERR_FORMAT, "posting entries hold ids >= N");
+
+    DeviceGuard guard(device);
+    ISSL_HIP(hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking));
+    int rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_sig, n_sites)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_post, n_slices * n_sites)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_inline, n_slices * n_sites)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_lstart, lstart.size())) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_meta, std::max<uint64_t>(n_slices, 1))) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_mit64, kMitEntries)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_mit32, kMitEntries)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_cfd64, 336)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_cfd32, 336)) != ISSL_OK) return rc;
+    ix->distinct_words = n_sites / 64 + 1;
+    if ((rc = dev_alloc(ix.get(), &ix->d_distinct, ix->distinct_words)) != ISSL_OK) return rc;
+
+    if ((rc = upload(ix.get(), ix->d_sig, sigs, n_sites * 8)) != ISSL_OK) return rc;
+    if ((rc = upload(ix.get(), ix->d_post, entries, n_slices * n_sites * 8)) != ISSL_OK) return rc;
+    if ((rc = upload(ix.get(), ix->d_lstart, lstart.data(), lstart.size() * 4)) != ISSL_OK) return rc;
+    if (n_slices)
+        ISSL_HIP(hipMemcpy(ix->d_meta, ix->meta_h.data(), n_slices * sizeof(SliceMeta),
+                           hipMemcpyHostToDevice));
+    std::vector<float> m32(kMitEntries), c32(336);
+    std::vector<double> c64(336);
+    for (int r = 0; r < kMitEntries; ++r) m32[r] = static_cast<float>(mit_by_rank()[r]);
+    std::memcpy(c64.data(), cfd_pos_table(), 320 * 8);
+    std::memcpy(c64.data() + 320, cfd_pam_table(), 16 * 8);
+    for (int i = 0; i < 336; ++i) c32[i] = static_cast<float>(c64[i]);
+    ISSL_HIP(hipMemcpy(ix->d_mit64, mit_by_rank(), kMitEntries * 8, hipMemcpyHostToDevice));
+    ISSL_HIP(hipMemcpy(ix->d_mit32, m32.data(), kMitEntries * 4, hipMemcpyHostToDevice));
+    ISSL_HIP(hipMemcpy(ix->d_cfd64, c64.data(), 336 * 8, hipMemcpyHostToDevice));
+    ISSL_HIP(hipMemcpy(ix->d_cfd32, c32.data(), 336 * 4, hipMemcpyHostToDevice));
+    ISSL_HIP(hipMemset(ix->d_distinct, 0, ix->distinct_words * 8));
+    if ((rc = build_inline_layout(ix.get())) != ISSL_OK) return rc;
+    ISSL_HIP(hipStreamSynchronize(ix->stream));
+    // Release the staging buffer used for the upload.
+    ix->ws.pinned.release();
+    *out = ix.release();
+    return ok();
+}
+
+extern "C" int issl_index_open(const char *path, int device, issl_index **out) {
+    if (!out) return fail(ISSL_ERR_ARG, "null output handle");
+    uint64_t n, L, S, nl;
+    int rc = issl_read_header(path, &n, &L, &S, &nl);
+    if (rc != ISSL_OK) return rc;
+    std::vector<uint64_t> sigs, masks, sizes, entries;
+    try {
+        sigs.resize(n);
+        masks.resize(S);
+        sizes.resize(nl);
+        entries.resize(S * n);
+    } catch (...) {
+        return fail(ISSL_ERR_NOMEM, "host allocation for the index failed");
+    }
+    rc = issl_read_index(path, sigs.data(), masks.data(), sizes.data(), entries.data());
+    if (rc != ISSL_OK) return rc;
+    return issl_index_from_arrays(n, L, S, sigs.data(), masks.data(), sizes.data(), entries.data(),
+                                  device, out);
+}
+
+extern "C" int issl_index_close(issl_index *ix) {
+    if (!ix) return fail(ISSL_ERR_ARG, "null index");
+    free_index(ix);
+    return ok();
+}
+
+extern "C" int issl_index_get_info(const issl_index *ix, issl_index_info *out) {
+    if (!ix || !out) return fail(ISSL_ERR_ARG, "null argument");
+    out->n_sites = ix->n;
+    out->seq_len = ix->seq_len;
+    out->n_slices = ix->S;
+    out->n_lists = ix->n_lists;
+    out->device_bytes = ix->device_bytes;
+    out->device = ix->device;
+    out->max_width = ix->max_w;
+    return ok();
+}
+
+extern "C" int issl_index_validate(issl_index *ix, uint64_t *bad_entries) {
+    if (!ix || !bad_entries) return fail(ISSL_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    int rc = ix->ws.scratch.reserve(ix->n * 4 + 64);
+    if (rc != ISSL_OK) return rc;
+    uint32_t *seen = static_cast<uint32_t *>(ix->ws.scratch.p);
+    unsigned long long *bad = reinterpret_cast<unsigned long long *>(
+        static_cast<char *>(ix->ws.scratch.p) + ((ix->n * 4 + 15) / 16) * 16);
+    ISSL_HIP(hipMemsetAsync(bad, 0, 8, ix->stream));
+    for (uint64_t s = 0; s < ix->S; ++s) {
+        ISSL_HIP(hipMemsetAsync(seen, 0, ix->n * 4, ix->stream));
+        hipLaunchKernelGGL(k_validate, dim3(2048), dim3(256), 0, ix->stream, ix->d_post, ix->d_sig,
+                           ix->d_lstart, ix->meta_h[s], s, ix->n, seen, bad);
+        ISSL_HIP(hipGetLastError());
+    }
+    unsigned long long h = 0;
+    ISSL_HIP(hipMemcpyAsync(&h, bad, 8, hipMemcpyDeviceToHost, ix->stream));
+    ISSL_HIP(hipStreamSynchronize(ix->stream));
+    *bad_entries = h;
+    return ok();
+}
+
+extern "C" int issl_distinct_count(issl_index *ix, uint64_t *out) {
+    if (!ix || !out) return fail(ISSL_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    std::vector<uint64_t> words(ix->distinct_words);
+    ISSL_HIP(hipMemcpyAsync(words.data(), ix->d_distinct, ix->distinct_words * 8,
+                            hipMemcpyDeviceToHost, ix->stream));
+    ISSL_HIP(hipStreamSynchronize(ix->stream));
+    uint64_t c = 0;
+    for (uint64_t w : words) c += static_cast<uint64_t>(__builtin_popcountll(w));
+    *out = c;
+    return ok();
+}
+
+extern "C" int issl_distinct_reset(issl_index *ix) {
+    if (!ix) return fail(ISSL_ERR_ARG, "null index");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    ISSL_HIP(hipMemsetAsync(ix->d_distinct, 0, ix->distinct_words * 8, ix->stream));
+    ISSL_HIP(hipStreamSynchronize(ix->stream));
+    return ok();
+}
+
+namespace {
+
+int score_on_device(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_params *p,
+                    const ScoreScalars &sc, const DevResults &o, hipStream_t st) {
+    uint64_t *distinct = (p->flags & ISSL_FLAG_DISTINCT) ? ix->d_distinct : nullptr;
+    if (p->form == ISSL_FORM_PLAIN) return launch_plain(ix, d_guides, n, sc, o, distinct, nullptr, nullptr, st);
+    issl_tuning tn = ix->tuning;
+    return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, tn, o, distinct, ix->ws, st);
+}
+
+// Byte layout of the per-call device outputs for n guides.
+struct OutLayout {
+    size_t mit, cfd, msum, csum, occ, sites, pass, total;
+    explicit OutLayout(uint64_t n) {
+        const size_t d = n * 8;
+        mit = 0;
+        cfd = mit + d;
+        msum = cfd + d;
+        csum = msum + d;
+        occ = csum + d;
+        sites = occ + d;
+        pass = sites + d;
+        total = pass + ((n + 255) / 256) * 256;
+    }
+};
+
+}  // namespace
+
+extern "C" int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
+                          const issl_params *params, issl_results *out) {
+    ScoreScalars sc;
+    int rc = check_params(ix, params, &sc);
+    if (rc != ISSL_OK) return rc;
+    if (!out) return fail(ISSL_ERR_ARG, "null results");
+    if (n == 0) return ok();
+    if (!guides) return fail(ISSL_ERR_ARG, "null guides");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    const OutLayout L(n);
+    if ((rc = ix->ws.guides.reserve(n * 8)) != ISSL_OK) return rc;
+    if ((rc = ix->ws.out.reserve(L.total)) != ISSL_OK) return rc;
+    if ((rc = ix->ws.pinned.reserve(n * 8 + L.total)) != ISSL_OK) return rc;
+    char *pin = static_cast<char *>(ix->ws.pinned.p);
+    std::memcpy(pin, guides, n * 8);
+    hipStream_t st = ix->stream;
+    ISSL_HIP(hipMemcpyAsync(ix->ws.guides.p, pin, n * 8, hipMemcpyHostToDevice, st));
+    char *ob = static_cast<char *>(ix->ws.out.p);
+    DevResults o;
+    o.mit = reinterpret_cast<double *>(ob + L.mit);
+    o.cfd = reinterpret_cast<double *>(ob + L.cfd);
+    o.mit_sum = reinterpret_cast<double *>(ob + L.msum);
+    o.cfd_sum = reinterpret_cast<double *>(ob + L.csum);
+    o.occ = reinterpret_cast<uint64_t *>(ob + L.occ);
+    o.sites = reinterpret_cast<uint64_t *>(ob + L.sites);
+    o.pass = ob + L.pass;
+    if ((rc = score_on_device(ix, static_cast<const uint64_t *>(ix->ws.guides.p), n, params, sc, o,
+                              st)) != ISSL_OK)
+        return rc;
+    char *hout = pin + n * 8;
+    ISSL_HIP(hipMemcpyAsync(hout, ob, L.total, hipMemcpyDeviceToHost, st));
+    ISSL_HIP(hipStreamSynchronize(st));
+    if (out->mit) std::memcpy(out->mit, hout + L.mit, n * 8);
+    if (out->cfd) std::memcpy(out->cfd, hout + L.cfd, n * 8);
+    if (out->mit_sum) std::memcpy(out->mit_sum, hout + L.msum, n * 8);
+    if (out->cfd_sum) std::memcpy(out->cfd_sum, hout + L.csum, n * 8);
+    if (out->occ_count) std::memcpy(out->occ_count, hout + L.occ, n * 8);
+    if (out->site_count) std::memcpy(out->site_count, hout + L.sites, n * 8);
+    if (out->pass) std::memcpy(out->pass, hout + L.pass, n);
+    return ok();
+}
+
+extern "C" int issl_score_device(issl_index *ix, const uint64_t *guides_device, uint64_t n,
+                                 const issl_params *params, issl_results *out_device, void *stream) {
+    ScoreScalars sc;
+    int rc = check_params(ix, params, &sc);
+    if (rc != ISSL_OK) return rc;
+    if (!out_device) return fail(ISSL_ERR_ARG, "null results");
+    if (n == 0) return ok();
+    if (!guides_device) return fail(ISSL_ERR_ARG, "null guides");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    DevResults o;
+    o.mit = out_device->mit;
+    o.cfd = out_device->cfd;
+    o.mit_sum = out_device->mit_sum;
+    o.cfd_sum = out_device->cfd_sum;
+    o.occ = out_device->occ_count;
+    o.sites = out_device->site_count;
+    o.pass = out_device->pass;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+    rc = score_on_device(ix, guides_device, n, params, sc, o, st);
+    if (rc != ISSL_OK) return rc;
+    return ok();
+}
+
+extern "C" int issl_candidates(issl_index *ix, const uint64_t *guides, uint64_t n, uint64_t *offsets,
+                               uint32_t *ids, uint64_t capacity) {
+    if (!ix || !offsets) return fail(ISSL_ERR_ARG, "null argument");
+    if (n && !guides) return fail(ISSL_ERR_ARG, "null guides");
+    offsets[0] = 0;
+    if (n == 0) return ok();
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    int rc;
+    if ((rc = ix->ws.guides.reserve(n * 8)) != ISSL_OK) return rc;
+    if ((rc = ix->ws.out.reserve(n * 8 * 2 + 64)) != ISSL_OK) return rc;
+    hipStream_t st = ix->stream;
+    ISSL_HIP(hipMemcpyAsync(ix->ws.guides.p, guides, n * 8, hipMemcpyHostToDevice, st));
+    ScoreScalars sc{};
+    sc.method = ISSL_METHOD_MIT_AND_CFD;
+    sc.calc_mit = 0;
+    sc.calc_cfd = 0;
+    sc.use_cut = 0;
+    sc.threshold = 0;
+    DevResults o;
+    o.sites = static_cast<uint64_t *>(ix->ws.out.p);
+    const uint64_t *dg = static_cast<const uint64_t *>(ix->ws.guides.p);
+    if ((rc = launch_plain(ix, dg, n, sc, o, nullptr, nullptr, nullptr, st)) != ISSL_OK) return rc;
+    std::vector<uint64_t> counts(n);
+    ISSL_HIP(hipMemcpyAsync(counts.data(), o.sites, n * 8, hipMemcpyDeviceToHost, st));
+    ISSL_HIP(hipStreamSynchronize(st));
+    for (uint64_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + counts[i];
+    const uint64_t total = offsets[n];
+    if (total > capacity)
+        return fail(ISSL_ERR_CAPACITY, "candidate capacity " + std::to_string(capacity) +
+                                           " < " + std::to_string(total));
+    if (total == 0) return ok();
+    if (!ids) return fail(ISSL_ERR_ARG, "null ids");
+    if ((rc = ix->ws.scratch.reserve(total * 4 + (n + 1) * 8)) != ISSL_OK) return rc;
+    uint64_t *d_off = static_cast<uint64_t *>(ix->ws.scratch.p);
+    uint32_t *d_ids = reinterpret_cast<uint32_t *>(d_off + n + 1);
+    ISSL_HIP(hipMemcpyAsync(d_off, offsets, (n + 1) * 8, hipMemcpyHostToDevice, st));
+    if ((rc = launch_plain(ix, dg, n, sc, o, nullptr, d_off, d_ids, st)) != ISSL_OK) return rc;
+    ISSL_HIP(hipMemcpyAsync(ids, d_ids, total * 4, hipMemcpyDeviceToHost, st));
+    ISSL_HIP(hipStreamSynchronize(st));
+    return ok();
+}
+
+extern "C" int issl_timing_enable(issl_index *ix, int enable) {
+    if (!ix) return fail(ISSL_ERR_ARG, "null index");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    for (size_t i = 0; i < ix->ev_used; ++i) ISSL_HIP(hipEventSynchronize(ix->ev_pool[i]));
+    ix->ev_used = 0;
+    ix->timing = enable != 0;
+    return ok();
+}
+
+extern "C" int issl_timing_read(issl_index *ix, double *kernel_ms, uint64_t *launches) {
+    if (!ix || !kernel_ms || !launches) return fail(ISSL_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    double total = 0;
+    for (size_t i = 0; i + 1 < ix->ev_used; i += 2) {
+        ISSL_HIP(hipEventSynchronize(ix->ev_pool[i + 1]));
+        float ms = 0;
+        ISSL_HIP(hipEventElapsedTime(&ms, ix->ev_pool[i], ix->ev_pool[i + 1]));
+        total += ms;
+    }
+    *kernel_ms = total;
+    *launches = ix->ev_used / 2;
+    return ok();
+}

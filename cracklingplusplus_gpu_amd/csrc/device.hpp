@@ -1,0 +1,121 @@
+// device.hpp -- resident device index and kernel interfaces of libissl_gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "issl_gpu.h"
+
+namespace issl {
+
+#define ISSL_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::issl::fail(ISSL_ERR_DEVICE, std::string(#call) + ": " +             \
+                                                     hipGetErrorString(e_));             \
+    } while (0)
+
+// Per slice: fold mask (bit 2j per position j), first list-start index,
+// width and positions.
+struct SliceMeta {
+    uint64_t fold;
+    uint32_t lbase;
+    uint32_t width;
+    uint8_t pos[16];
+};
+
+// Site word of the signature-inline layout: signature in bits 0..39,
+// occurrences in bits 40..63; kOccEscape means "read occ from the posting".
+constexpr uint64_t kSigBits = (1ULL << 40) - 1;
+constexpr uint32_t kOccEscape = 0xFFFFFF;
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t want);
+    void release();
+};
+
+// Grow-only pinned host buffer.
+struct HostBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t want);
+    void release();
+};
+
+// Device outputs of one scoring call (each may be null).
+struct DevResults {
+    double *mit = nullptr, *cfd = nullptr, *mit_sum = nullptr, *cfd_sum = nullptr;
+    uint64_t *occ = nullptr, *sites = nullptr;
+    char *pass = nullptr;
+};
+
+// Scalar parameters shared by the kernels.
+struct ScoreScalars {
+    int method;
+    int calc_mit;
+    int calc_cfd;
+    int use_cut;       // the early exit is armed (threshold != 0)
+    double cut;        // (10000 - 100 t)/t, formed on the host in FP64
+    double threshold;
+};
+
+struct Workspace {
+    DevBuf guides, out, tiles, scratch;
+    HostBuf pinned;
+};
+
+}  // namespace issl
+
+struct issl_index {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t n = 0, seq_len = 0, S = 0, n_lists = 0;
+    int max_w = 0;
+    std::vector<uint64_t> masks;
+    std::vector<issl::SliceMeta> meta_h;
+    // device arrays
+    uint64_t *d_sig = nullptr;      // [n] signatures
+    uint64_t *d_post = nullptr;     // [S*n] occ<<32 | id, .issl order
+    uint64_t *d_inline = nullptr;   // [S*n] sig | occ<<40 (same order)
+    uint32_t *d_lstart = nullptr;   // per slice: 4^w + 1 list starts
+    issl::SliceMeta *d_meta = nullptr;
+    double *d_mit64 = nullptr;      // [6195] by combinadic rank
+    float *d_mit32 = nullptr;
+    double *d_cfd64 = nullptr;      // [320 pos | 16 pam]
+    float *d_cfd32 = nullptr;
+    uint64_t *d_distinct = nullptr; // [n/64 + 1]
+    uint64_t distinct_words = 0;
+    uint64_t device_bytes = 0;
+    issl_tuning tuning{};
+    bool tuned = false;
+    // Kernel timing: event pairs around the dominant kernel of each launch.
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::mutex mu;
+    issl::Workspace ws;
+};
+
+namespace issl {
+
+// Launchers (device.hip / batched.hip).  All asynchronous on `stream`.
+int launch_plain(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
+                 const DevResults &out, uint64_t *distinct, const uint64_t *cand_off,
+                 uint32_t *cand_ids, hipStream_t stream);
+int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
+                   int precision, int chunk, const issl_tuning &tn, const DevResults &out,
+                   uint64_t *distinct, Workspace &ws, hipStream_t stream);
+int build_inline_layout(issl_index *ix);
+// Record a timing event on `stream` when timing is enabled (no-op otherwise).
+void timing_mark(issl_index *ix, hipStream_t stream);
+
+}  // namespace issl

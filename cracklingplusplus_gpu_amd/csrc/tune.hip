@@ -1,0 +1,159 @@
+// tune.hip -- launch-shape autotuner for the BATCHED form.
+//
+// Candidates: tile length (chunk), postings in flight per lane (depth) and
+// persistent workgroups per CU.  Each is timed with HIP events on a sample of
+// the caller's guides against the resident index; the fastest is kept on the
+// index and, when a cache path is given, appended to a text cache keyed by
+// the index shape and the device, so later processes skip the sweep.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+
+using namespace issl;
+
+namespace {
+
+std::string shape_key(const issl_index *ix) {
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    int cus = 0;
+    if (hipGetDeviceProperties(&prop, ix->device) == hipSuccess) {
+        arch = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+    }
+    for (char &c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    std::ostringstream k;
+    k << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
+    for (const SliceMeta &m : ix->meta_h) k << m.width << ',';
+    return k.str();
+}
+
+bool cache_lookup(const char *path, const std::string &key, issl_tuning *t) {
+    if (!path) return false;
+    std::ifstream in(path);
+    std::string line;
+    bool found = false;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string k;
+        issl_tuning v{};
+        if (ls >> k >> v.chunk >> v.block >> v.depth >> v.grid_per_cu >> v.us_per_guide && k == key) {
+            *t = v;
+            found = true;  // the last entry for a key wins
+        }
+    }
+    return found;
+}
+
+}  // namespace
+
+extern "C" int issl_get_tuning(issl_index *ix, issl_tuning *out) {
+    if (!ix || !out) return fail(ISSL_ERR_ARG, "null argument");
+    *out = ix->tuning;
+    if (out->chunk <= 0) out->chunk = 512;
+    if (out->depth <= 0) out->depth = 4;
+    if (out->grid_per_cu <= 0) out->grid_per_cu = 8;
+    out->block = 256;
+    return ok();
+}
+
+extern "C" int issl_set_tuning(issl_index *ix, const issl_tuning *in) {
+    if (!ix || !in) return fail(ISSL_ERR_ARG, "null argument");
+    const int d = in->depth;
+    if (in->chunk < 64 || in->chunk % 64 != 0 || in->chunk > (1 << 20) ||
+        !(d == 1 || d == 2 || d == 4 || d == 8 || d == 16) || in->grid_per_cu < 1 ||
+        in->grid_per_cu > 64)
+        return fail(ISSL_ERR_ARG, "tuning out of range");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    ix->tuning = *in;
+    ix->tuning.block = 256;
+    ix->tuned = true;
+    return ok();
+}
+
+extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
+                             const char *cache_path, issl_tuning *best) {
+    if (!ix || !best) return fail(ISSL_ERR_ARG, "null argument");
+    if (n == 0 || !sample) return fail(ISSL_ERR_ARG, "autotune needs sample guides");
+    const std::string key = shape_key(ix);
+    issl_tuning cached{};
+    if (cache_lookup(cache_path, key, &cached)) {
+        int rc = issl_set_tuning(ix, &cached);
+        if (rc == ISSL_OK) {
+            *best = ix->tuning;
+            return ok();
+        }
+    }
+    std::lock_guard<std::mutex> lk(ix->mu);
+    ISSL_HIP(hipSetDevice(ix->device));
+    int rc;
+    if ((rc = ix->ws.guides.reserve(n * 8)) != ISSL_OK) return rc;
+    if ((rc = ix->ws.out.reserve(n * 8 * 2 + 4096)) != ISSL_OK) return rc;
+    hipStream_t st = ix->stream;
+    ISSL_HIP(hipMemcpyAsync(ix->ws.guides.p, sample, n * 8, hipMemcpyHostToDevice, st));
+    const uint64_t *dg = static_cast<const uint64_t *>(ix->ws.guides.p);
+    DevResults o;
+    o.mit = static_cast<double *>(ix->ws.out.p);
+    o.cfd = o.mit + n;
+    ScoreScalars sc{};
+    sc.method = ISSL_METHOD_MIT_AND_CFD;
+    sc.calc_mit = 1;
+    sc.calc_cfd = 1;
+    sc.use_cut = 0;
+    sc.threshold = 0;
+    hipEvent_t e0, e1;
+    ISSL_HIP(hipEventCreate(&e0));
+    ISSL_HIP(hipEventCreate(&e1));
+    const int chunks[] = {256, 512, 1024, 2048, 4096};
+    const int depths[] = {2, 4, 8, 16};
+    const int grids[] = {4, 8, 16};
+    issl_tuning top{};
+    top.us_per_guide = 1e300;
+    for (int c : chunks)
+        for (int d : depths) {
+            if (64 * d > c) continue;
+            for (int gpc : grids) {
+                issl_tuning t{c, 256, d, gpc, 0};
+                // warm-up, then the best of three timed launches
+                if ((rc = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st)) !=
+                    ISSL_OK)
+                    return rc;
+                float bestms = 1e30f;
+                for (int r = 0; r < 3; ++r) {
+                    ISSL_HIP(hipEventRecord(e0, st));
+                    if ((rc = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws,
+                                             st)) != ISSL_OK)
+                        return rc;
+                    ISSL_HIP(hipEventRecord(e1, st));
+                    ISSL_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    ISSL_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < bestms) bestms = ms;
+                }
+                t.us_per_guide = 1000.0 * bestms / static_cast<double>(n);
+                if (t.us_per_guide < top.us_per_guide) top = t;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ix->tuning = top;
+    ix->tuned = true;
+    *best = top;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "a");
+        if (f) {
+            fprintf(f, "%s %d %d %d %d %.6g\n", key.c_str(), top.chunk, top.block, top.depth,
+                    top.grid_per_cu, top.us_per_guide);
+            fclose(f);
+        }
+    }
+    return ok();
+}

@@ -1,0 +1,210 @@
+/*
+ * issl_gpu.h -- C ABI of the MI355X ISSL off-target scorer (libissl_gpu.so).
+ *
+ * The reference has no C ABI: its scorer lives inside
+ * ISSLScoringModule::run (lib/ISSLScoringModule.cpp:52-466) and the
+ * standalone ISSLScoreOfftargets main (ISSLScoreOfftargets/ISSLScoreOfftargets.cpp:36-647).
+ * Each entry point below names the part of those it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status: ISSL_OK (0) or a negative
+ *     ISSL_ERR_* code; issl_last_error() then holds a message for the calling
+ *     thread.  Nothing throws across the ABI.
+ *   - Plain pointers and sizes only; buffers are caller-owned host memory
+ *     unless a name says "_device".
+ *   - Signatures are 2-bit packed 20-mers, position j in bits 2j..2j+1,
+ *     A=0 C=1 G=2 T=3 (lib/ISSLScoringModule.cpp:10,469-476).
+ *   - Method numbering is the pipeline enum of include/util.hpp:20.
+ */
+#ifndef ISSL_GPU_H
+#define ISSL_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define ISSL_OK              0
+#define ISSL_ERR_ARG        -1  /* invalid argument                        */
+#define ISSL_ERR_IO         -2  /* file could not be opened / read / written */
+#define ISSL_ERR_FORMAT     -3  /* index file or arrays are inconsistent     */
+#define ISSL_ERR_DEVICE     -4  /* HIP error, or no usable GPU               */
+#define ISSL_ERR_NOMEM      -5  /* device or host allocation failed          */
+#define ISSL_ERR_CAPACITY   -6  /* output capacity or admission limit exceeded */
+#define ISSL_ERR_TIMEOUT    -7  /* a wait timed out                           */
+#define ISSL_ERR_STATE      -8  /* call not valid in the object's state        */
+
+/* --------------------------------------------------------------- methods */
+/* include/util.hpp:20 */
+#define ISSL_METHOD_MIT          0
+#define ISSL_METHOD_CFD          1
+#define ISSL_METHOD_MIT_AND_CFD  2
+#define ISSL_METHOD_MIT_OR_CFD   3
+#define ISSL_METHOD_AVG_MIT_CFD  4
+
+/* Scoring forms.
+ *   PLAIN   : one guide per workgroup, FP64, hits accumulated in the
+ *             reference's slice-major posting order (bit-exact sums and the
+ *             reference's early-exit point).
+ *   BATCHED : persistent work-stealing kernel over (guide, chunk) tiles of
+ *             the guide's slice-major posting sequence, sites streamed from
+ *             a signature-inline device layout; precision selectable. */
+#define ISSL_FORM_PLAIN    0
+#define ISSL_FORM_BATCHED  1
+
+/* Precision of the BATCHED form (PLAIN is always FP64).
+ *   FP64  : FP64 per-site terms and sums.
+ *   FP32C : FP32 per-site products, compensated two-float FP32 sums, FP64
+ *           only for the final total, the transform and the threshold test. */
+#define ISSL_PREC_FP64   0
+#define ISSL_PREC_FP32C  1
+
+/* Flags */
+#define ISSL_FLAG_DISTINCT  1u  /* also set the index's distinct-site bitmap
+                                   (ISSLScoreOfftargets.cpp:202-208,375-377) */
+
+typedef struct issl_index issl_index;
+
+typedef struct {
+    int32_t  max_dist;   /* must be 4: the reference tests dist <= 4
+                            (ISSLScoringModule.cpp:292, ISSLScoreOfftargets.cpp:369) */
+    int32_t  method;     /* ISSL_METHOD_* (util.hpp:20) */
+    double   threshold;  /* score threshold; 0 disables the early exit
+                            (maximum_sum = (10000 - 100 t)/t, ISSLScoringModule.cpp:225) */
+    int32_t  form;       /* ISSL_FORM_* */
+    int32_t  precision;  /* ISSL_PREC_* (BATCHED only) */
+    uint32_t flags;      /* ISSL_FLAG_* */
+    int32_t  chunk;      /* BATCHED tile length in postings; 0 = tuned/default */
+} issl_params;
+
+/* Per-guide results (each pointer nullable, n entries each). */
+typedef struct {
+    double   *mit;        /* 10000/(100+S_mit), -1 if MIT is not computed
+                             (ISSLScoringModule.cpp:404-455) */
+    double   *cfd;        /* 10000/(100+S_cfd), -1 if CFD is not computed */
+    uint64_t *occ_count;  /* sum of occurrences of scored sites up to the exit
+                             (ISSLScoreOfftargets.cpp:371, "Offtarget_Count") */
+    uint64_t *site_count; /* number of distinct sites scored up to the exit */
+    char     *pass;       /* '1' accepted, '0' rejected (lib/util.cpp:21-22) */
+    double   *mit_sum;    /* raw S_mit before the transform */
+    double   *cfd_sum;    /* raw S_cfd before the transform */
+} issl_results;
+
+typedef struct {
+    uint64_t n_sites;      /* distinct sites N (header word 0) */
+    uint64_t seq_len;      /* header word 1 */
+    uint64_t n_slices;     /* header word 2 */
+    uint64_t n_lists;      /* sum over slices of 4^w */
+    uint64_t device_bytes; /* HBM held by the resident index */
+    int32_t  device;       /* HIP device ordinal */
+    int32_t  max_width;    /* largest slice width w */
+} issl_index_info;
+
+/* ------------------------------------------------------------------ index */
+
+/* Load a .issl file and keep it resident on `device`.
+ * Replaces the fread loader of ISSLScoringModule.cpp:72-150 and the list
+ * pointer table of :181-198 (ISSLScoreOfftargets.cpp:103-243). */
+int issl_index_open(const char *path, int device, issl_index **out);
+
+/* Same, from arrays already in host memory in the .issl layout:
+ * sigs[n_sites], slice_masks[n_slices], list_sizes[sum 4^w],
+ * entries[n_slices*n_sites] (occ<<32 | id). */
+int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+                           const uint64_t *sigs, const uint64_t *slice_masks,
+                           const uint64_t *list_sizes, const uint64_t *entries,
+                           int device, issl_index **out);
+
+/* Release the resident index (no counterpart: the reference frees its
+ * function-local vectors at the end of run). */
+int issl_index_close(issl_index *ix);
+
+int issl_index_get_info(const issl_index *ix, issl_index_info *out);
+
+/* Check on the device that every posting entry sits in the list of its own
+ * key and that each slice lists every site once; *bad_entries receives the
+ * number of violations.  (The reference trusts the file.) */
+int issl_index_validate(issl_index *ix, uint64_t *bad_entries);
+
+/* Distinct-site statistic of ISSLScoreOfftargets.cpp:549-555: sites scored
+ * by any guide since the last reset (calls made with ISSL_FLAG_DISTINCT). */
+int issl_distinct_count(issl_index *ix, uint64_t *out);
+int issl_distinct_reset(issl_index *ix);
+
+/* ---------------------------------------------------------------- scoring */
+
+/* 20-character guides -> signatures.  text holds n guides, guide i at
+ * text + i*stride (stride >= 20).  ISSLScoringModule.cpp:469-476,
+ * ISSLScoreOfftargets.cpp:279-288.  Characters other than C, G, T encode as
+ * A, as in the reference's table for upper-case input. */
+int issl_encode(const char *text, uint64_t n, uint64_t stride, uint64_t *out);
+
+/* Score n guides against a resident index; host in/out buffers, blocking.
+ * Replaces the scoring loop of ISSLScoringModule.cpp:204-462 and
+ * ISSLScoreOfftargets.cpp:296-505.  n == 0 is valid and does nothing. */
+int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
+               const issl_params *params, issl_results *out);
+
+/* Same with device buffers on a caller stream (hipStream_t passed as void*,
+ * NULL = the index's own stream); asynchronous.  Each results pointer is a
+ * device pointer (nullable).  Inputs must already be resident in HBM. */
+int issl_score_device(issl_index *ix, const uint64_t *guides_device, uint64_t n,
+                      const issl_params *params, issl_results *out_device,
+                      void *stream);
+
+/* Candidate sets: for every guide the ids of the sites it scores with the
+ * threshold disabled (dist <= 4, each site once, in the reference's
+ * slice-major posting order), CSR form.  offsets[n+1]; ids[capacity].
+ * Replaces the candidate stage of the reference's CUDA scorer
+ * (ISSLScoreOfftargets/issl_cuda.cuh:46-63: distance scan + dedup by query).
+ * Returns ISSL_ERR_CAPACITY (offsets still filled) if capacity is short. */
+int issl_candidates(issl_index *ix, const uint64_t *guides, uint64_t n,
+                    uint64_t *offsets, uint32_t *ids, uint64_t capacity);
+
+/* ------------------------------------------------------------- autotuner */
+
+typedef struct {
+    int32_t chunk;        /* postings per tile */
+    int32_t block;        /* threads per workgroup (64..1024, multiple of 64) */
+    int32_t depth;        /* postings in flight per lane */
+    int32_t grid_per_cu;  /* persistent workgroups per CU */
+    double  us_per_guide; /* measured */
+} issl_tuning;
+
+/* Time the BATCHED kernel's launch shapes on a sample of guides for this
+ * index and keep the fastest (also written to the cache file when path is
+ * non-NULL).  No reference counterpart. */
+int issl_autotune(issl_index *ix, const uint64_t *sample_guides, uint64_t n,
+                  const char *cache_path, issl_tuning *best);
+int issl_get_tuning(issl_index *ix, issl_tuning *out);
+int issl_set_tuning(issl_index *ix, const issl_tuning *in);
+
+/* ------------------------------------------------------------ measurement */
+
+/* Kernel timing (no reference counterpart; the reference prints wall-clock
+ * phase timers, ISSLScoreOfftargets.cpp:185-188,296,507-510).  While enabled,
+ * HIP events bracket the dominant kernel of every scoring launch on this
+ * index (k_plain for PLAIN, k_tiles for BATCHED) on the launch's stream.
+ * Enabling clears earlier marks; read synchronizes and returns the summed
+ * kernel time and the number of bracketed launches. */
+int issl_timing_enable(issl_index *ix, int enable);
+int issl_timing_read(issl_index *ix, double *kernel_ms, uint64_t *launches);
+
+/* ------------------------------------------------------------- utilities */
+
+/* Thread-local message for the last failing call on this thread. */
+const char *issl_last_error(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int issl_device_count(void);
+
+/* Library version string. */
+const char *issl_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISSL_GPU_H */

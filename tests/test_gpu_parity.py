@@ -1,0 +1,202 @@
+"""HIP scorer vs the reference's golden vectors and the CPU oracle (MI355X).
+
+Tolerances:
+  * PLAIN (FP64, in-order accumulation): bit-exact scores, sums, counts and
+    calls for every method, threshold on and off.
+  * BATCHED FP64: scores within 1e-12 relative (summation order differs);
+    counts and candidate sets exact.
+  * BATCHED FP32C (FP32 products, compensated FP32 sums): scores within
+    1e-6 relative with the threshold disabled; calls identical with the
+    threshold on except guides whose FP64 sum lies within 1e-6 relative of
+    the cutoff.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import cracklingplusplus_gpu_amd as issl
+import oracle_lib as orc
+from conftest import GOLDEN_CASES, golden_index, load_golden
+
+pytestmark = pytest.mark.gpu
+
+METHODS = range(5)
+THRESHOLDS = (0, 75)
+
+
+@pytest.fixture(scope="module")
+def cases(gpu_available):
+    out = {}
+    for name in GOLDEN_CASES:
+        print(f"[gpu-parity] opening {name}", flush=True)
+        hx = golden_index(name)
+        out[name] = (hx, issl.Index.from_arrays(hx), load_golden(name))
+    yield out
+    for _, ix, _ in out.values():
+        ix.close()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def cut_of(t):
+    return (10000.0 - t * 100) / t
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_plain_bit_exact_vs_reference(cases, name):
+    hx, ix, gd = cases[name]
+    g = gd["guides"]
+    for m, t in itertools.product(METHODS, THRESHOLDS):
+        print(f"[gpu-parity] plain {name} m{m} t{t}", flush=True)
+        r = ix.score(g, method=m, threshold=float(t), form=issl.FORM_PLAIN)
+        key = f"m{m}_t{t}"
+        assert (bits(r.mit) == bits(gd[key + "_mit"])).all(), key
+        assert (bits(r.cfd) == bits(gd[key + "_cfd"])).all(), key
+        assert (r.passed == gd[key + "_pass"]).all(), key
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_plain_sums_and_counts_vs_oracle(cases, name):
+    hx, ix, gd = cases[name]
+    g = gd["guides"]
+    for m, t in itertools.product(METHODS, THRESHOLDS):
+        o = orc.score(hx, g, m, float(t))
+        r = ix.score(g, method=m, threshold=float(t), form=issl.FORM_PLAIN)
+        assert (bits(r.mit_sum) == bits(o["mit_sum"])).all()
+        assert (bits(r.cfd_sum) == bits(o["cfd_sum"])).all()
+        assert (r.occ_count == o["occ_count"]).all()
+        assert (r.site_count == o["site_count"]).all()
+
+
+@pytest.mark.parametrize("precision,rtol", [(issl.PREC_FP64, 1e-12), (issl.PREC_FP32C, 1e-6)])
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_batched_vs_reference(cases, name, precision, rtol):
+    hx, ix, gd = cases[name]
+    g = gd["guides"]
+    for m in METHODS:
+        print(f"[gpu-parity] batched p{precision} {name} m{m}", flush=True)
+        # threshold disabled: scores within tolerance, counts exact
+        r = ix.score(g, method=m, threshold=0.0, form=issl.FORM_BATCHED, precision=precision)
+        key = f"m{m}_t0"
+        for f in ("mit", "cfd"):
+            want = gd[key + "_" + f]
+            np.testing.assert_allclose(r.__dict__[f], want, rtol=rtol, atol=0)
+        o = orc.score(hx, g, m, 0.0)
+        assert (r.site_count == o["site_count"]).all()
+        assert (r.occ_count == o["occ_count"]).all()
+        # threshold 75: identical calls away from the cutoff
+        r75 = ix.score(g, method=m, threshold=75.0, form=issl.FORM_BATCHED, precision=precision)
+        want = gd[f"m{m}_t75_pass"]
+        full = o  # threshold-0 sums decide which guides sit at the cutoff
+        cut = cut_of(75.0)
+        near = (np.abs(full["mit_sum"] - cut) <= 1e-6 * cut) | (np.abs(full["cfd_sum"] - cut) <= 1e-6 * cut)
+        if m == issl.METHOD_AVG_MIT_CFD:
+            near |= np.abs((full["mit_sum"] + full["cfd_sum"]) / 2 - cut) <= 1e-6 * cut
+        assert (r75.passed[~near] == want[~near]).all()
+
+
+@pytest.mark.parametrize("name", ["tiny4k_5x4", "mid50k_10x10", "small200k_8x8"])
+def test_batched_exit_point_matches_reference(cases, name):
+    """Threshold on: the finalize rescan stops at the reference's exit hit."""
+    hx, ix, gd = cases[name]
+    g = gd["guides"]
+    for m in METHODS:
+        o = orc.score(hx, g, m, 75.0)
+        r = ix.score(g, method=m, threshold=75.0, form=issl.FORM_BATCHED,
+                     precision=issl.PREC_FP64)
+        same = r.site_count == o["site_count"]
+        assert same.mean() > 0.99
+        np.testing.assert_allclose(r.mit_sum[same], o["mit_sum"][same], rtol=1e-12)
+        np.testing.assert_allclose(r.cfd_sum[same], o["cfd_sum"][same], rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["legacy39_5x4", "tiny4k_5x4", "mid50k_10x10"])
+def test_candidate_sets_match_brute_force(cases, name):
+    hx, ix, gd = cases[name]
+    g = gd["guides"][:200]
+    off, ids = ix.candidates(g)
+    for i, q in enumerate(g):
+        got = ids[off[i]:off[i + 1]]
+        want = orc.candidates_brute(hx, int(q))
+        assert np.array_equal(np.sort(got), want), i
+        assert len(np.unique(got)) == len(got)
+
+
+@pytest.mark.parametrize("form", [issl.FORM_PLAIN, issl.FORM_BATCHED])
+@pytest.mark.parametrize("t", THRESHOLDS)
+def test_distinct_sites_match_oracle(cases, form, t):
+    hx, ix, gd = cases["small200k_8x8"]
+    g = gd["guides"]
+    o = orc.score(hx, g, 2, float(t), distinct=True)
+    want = sum(bin(int(w)).count("1") for w in o["distinct"])
+    ix.distinct_reset()
+    ix.score(g, method=2, threshold=float(t), form=form, flags=issl.FLAG_DISTINCT)
+    assert ix.distinct_count() == want
+
+
+def test_batch_composition_does_not_change_results(cases):
+    hx, ix, gd = cases["mid50k_10x10"]
+    g = gd["guides"]
+    for prec in (issl.PREC_FP64, issl.PREC_FP32C):
+        full = ix.score(g, form=issl.FORM_BATCHED, precision=prec)
+        for lo, hi in [(0, 1), (17, 80), (300, 500)]:
+            part = ix.score(g[lo:hi], form=issl.FORM_BATCHED, precision=prec)
+            assert (bits(part.mit) == bits(full.mit[lo:hi])).all()
+            assert (bits(part.cfd) == bits(full.cfd[lo:hi])).all()
+        shuffled = np.random.default_rng(5).permutation(len(g))
+        sh = ix.score(g[shuffled], form=issl.FORM_BATCHED, precision=prec)
+        assert (bits(sh.mit) == bits(full.mit[shuffled])).all()
+
+
+def test_validate_index(cases):
+    hx, ix, gd = cases["tiny4k_5x4"]
+    assert ix.validate() == 0
+    bad = issl.IndexArrays(hx.sigs.copy(), hx.masks.copy(), hx.list_sizes.copy(),
+                           hx.entries.copy(), hx.seq_len)
+    # move one entry's id to a different site: wrong list and a duplicate id
+    old_id = int(bad.entries[5]) & 0xFFFFFFFF
+    bad.entries[5] = (bad.entries[5] & np.uint64(0xFFFFFFFF00000000)) | np.uint64(
+        (old_id + 1) % hx.n_sites)
+    with issl.Index.from_arrays(bad) as b:
+        assert b.validate() > 0
+
+
+def test_argument_errors(cases):
+    hx, ix, gd = cases["tiny4k_5x4"]
+    g = gd["guides"][:10]
+    with pytest.raises(issl.IsslError):
+        ix.score(g, max_dist=3)
+    with pytest.raises(issl.IsslError):
+        ix.score(g, method=7)
+    with pytest.raises(issl.IsslError):
+        ix.score(g, form=issl.FORM_PLAIN, precision=issl.PREC_FP32C)
+    r = ix.score(g[:0])
+    assert r.mit.shape == (0,)
+
+
+def test_index_info_and_open_file(cases, tmp_path):
+    hx, ix, gd = cases["tiny4k_5x4"]
+    info = ix.info()
+    assert info.n_sites == hx.n_sites and info.n_slices == 5 and info.max_width == 4
+    p = str(tmp_path / "t.issl")
+    issl.write_index(p, hx)
+    with issl.Index.open(p) as f:
+        a = f.score(gd["guides"])
+        b = ix.score(gd["guides"])
+        assert (bits(a.cfd) == bits(b.cfd)).all()
+
+
+def test_autotune_keeps_results(cases, tmp_path):
+    hx, ix, gd = cases["small200k_8x8"]
+    g = gd["guides"]
+    before = ix.score(g, form=issl.FORM_BATCHED, precision=issl.PREC_FP64)
+    t = ix.autotune(g, str(tmp_path / "tune.txt"))
+    assert t.chunk % 64 == 0 and t.us_per_guide > 0
+    after = ix.score(g, form=issl.FORM_BATCHED, precision=issl.PREC_FP64)
+    np.testing.assert_allclose(after.cfd, before.cfd, rtol=1e-12)
+    assert (after.site_count == before.site_count).all()
+    t2 = ix.autotune(g, str(tmp_path / "tune.txt"))  # from the cache
+    assert (t2.chunk, t2.depth, t2.grid_per_cu) == (t.chunk, t.depth, t.grid_per_cu)

@@ -1,0 +1,119 @@
+"""The CPU oracle against the reference's golden vectors and closed forms (no GPU)."""
+import hashlib
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from conftest import GOLDEN, GOLDEN_CASES, golden_index, load_golden
+
+
+def test_encode_hand_vectors():
+    # position j -> bits 2j..2j+1, A=0 C=1 G=2 T=3 (lib/ISSLScoringModule.cpp:10,469-476)
+    assert orc.lib().or_encode(b"A" * 20) == 0
+    assert orc.lib().or_encode(b"C" + b"A" * 19) == 1
+    assert orc.lib().or_encode(b"A" * 19 + b"T") == 3 << 38
+    assert orc.lib().or_encode(b"ACGT" * 5) == sum(v << (2 * j) for j, v in
+                                                  enumerate([0, 1, 2, 3] * 5))
+    # non-ACGT upper-case letters read as A
+    assert orc.lib().or_encode(b"N" * 20) == 0
+
+
+def test_fold_worked_example():
+    # lib/ISSLScoringModule.cpp:260-285: AATTGCAT vs ATATCGAT -> 4 mismatches
+    q = orc.lib().or_encode(b"AATTGCAT" + b"A" * 12)
+    o = orc.lib().or_encode(b"ATATCGAT" + b"A" * 12)
+    x = q ^ o
+    fold = ((x >> 1) & 0x5555555555555555) | (x & 0x5555555555555555)
+    assert bin(fold).count("1") == 4
+    assert [j for j in range(20) if (fold >> (2 * j)) & 3] == [1, 2, 4, 5]
+
+
+def _legacy_mit():
+    raw = np.fromfile(os.path.join(GOLDEN, "ref_test_genome_offTargets_indexed.issl"),
+                      dtype=np.uint64)
+    ns = int(raw[5])
+    tab = raw[6:6 + 2 * ns].reshape(-1, 2)
+    return tab[:, 0].copy(), tab[:, 1].copy().view(np.float64)
+
+
+def test_mit_closed_form_matches_reference_fixture():
+    """All 6,195 MIT values of the reference's test index, bit for bit."""
+    lk, lv = _legacy_mit()
+    k, v = orc.mit_table()
+    assert k.shape == (6195,)
+    assert (k == lk).all()
+    assert (v.view(np.uint64) == lv.view(np.uint64)).all()
+
+
+def test_mit_special_values():
+    # one mismatch at position j: 100 * (1 - w[j]); w = 0 at 0,1,3,4,7
+    for j, want in [(0, 100.0), (1, 100.0), (2, 98.6), (5, 60.5)]:
+        assert orc.lib().or_mit_fold(1 << (2 * j)) == pytest.approx(want, rel=1e-15)
+    assert orc.lib().or_mit_fold(0) == 0.0
+    # a plausible slip (1/k instead of 1/k^2) fails this: two adjacent PAM-distal
+    # mismatches at 0,1: T1 = 1, d = 1, T2 = 1/((18/19)*4+1), T3 = 1/4
+    want = 1.0 * (1.0 / ((19.0 - 1.0) / 19.0 * 4.0 + 1)) * (1.0 / 4) * 100
+    assert orc.lib().or_mit_fold(0b101) == want
+
+
+def test_cfd_tables_checksum():
+    """The 336 CFD constants, as the reference's compiled table bytes."""
+    p, a = orc.cfd_tables()
+    digest = hashlib.sha256(p.tobytes() + a.tobytes()).hexdigest()
+    assert digest == CFD_SHA256
+    # structural facts of the Doench table: matching pairs (rc of the guide
+    # base) are 1, NGG PAM is 1, other PAMs < 1
+    for pos in range(20):
+        for g in range(4):
+            assert p[(pos << 4) | (g << 2) | (g ^ 3)] == 1.0
+    assert a[0b1010] == 1.0 and (a[np.arange(16) != 10] < 1).all()
+
+
+# SHA-256 of cfdPosPenalties[320] || cfdPamPenalties[16] as doubles
+# (lib/otScorePenalties.cpp:6201-6546), taken from the compiled reference.
+CFD_SHA256 = "5d37e48d643fc7921ac514217b57bf3035ade136181b51b18d68cde7af209b33"
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_matches_reference_goldens(name):
+    """Bit-exact FP64 scores and identical calls for every method, threshold on and off."""
+    gd = load_golden(name)
+    ix = golden_index(name)
+    g = gd["guides"]
+    for m, t in itertools.product(range(5), (0, 75)):
+        r = orc.score(ix, g, m, float(t))
+        key = f"m{m}_t{t}"
+        assert (r["mit"].view(np.uint64) == gd[key + "_mit"].view(np.uint64)).all(), key
+        assert (r["cfd"].view(np.uint64) == gd[key + "_cfd"].view(np.uint64)).all(), key
+        assert (r["pass"] == gd[key + "_pass"]).all(), key
+
+
+@pytest.mark.parametrize("name", ["legacy39_5x4", "tiny4k_5x4", "mid50k_10x10"])
+def test_oracle_candidate_sets_match_brute_force(name):
+    """Threshold disabled: sites scored per guide == brute force over all sites."""
+    gd = load_golden(name)
+    ix = golden_index(name)
+    g = gd["guides"][:200]
+    r = orc.score(ix, g, 2, 0.0, distinct=True)
+    occ = ix.occurrences()
+    union = set()
+    for i, q in enumerate(g):
+        ids = orc.candidates_brute(ix, int(q))
+        assert r["site_count"][i] == len(ids)
+        assert r["occ_count"][i] == int(occ[ids].astype(np.uint64).sum())
+        union.update(ids.tolist())
+    bm = r["distinct"]
+    assert sum(bin(int(w)).count("1") for w in bm) == len(union)
+
+
+def test_threshold_zero_disables_exit():
+    """t = 0: maximum_sum = +inf, never exits; pass never rejects (score < 0 is false)."""
+    gd = load_golden("tiny4k_5x4")
+    ix = golden_index("tiny4k_5x4")
+    r = orc.score(ix, gd["guides"], 2, 0.0)
+    assert (r["pass"] == ord("1")).all()
+    # the sums are full: equal to the brute-force occurrence-weighted CFD count floor
+    assert (r["cfd_sum"] >= 0).all()
