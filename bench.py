@@ -119,7 +119,7 @@ def main() -> int:
     os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
     tn = ix.autotune(guides, args.tune_cache)
     log(f"tuning: chunk {tn.chunk} depth {tn.depth} grid/CU {tn.grid_per_cu} "
-        f"({tn.us_per_guide * 1e3:.2f} ns/guide)")
+        f"sched {tn.sched} steal {tn.steal} ({tn.us_per_guide * 1e3:.2f} ns/guide)")
     setup_s = time.time() - t_setup
 
     kw = dict(method=issl.METHOD_MIT_AND_CFD, threshold=0.0, form=issl.FORM_BATCHED,
@@ -230,7 +230,8 @@ def main() -> int:
                    "max_dist": 4, "form": "batched", "precision": "fp32c",
                    "postings_per_step": p_total,
                    "tuning": {"chunk": tn.chunk, "depth": tn.depth,
-                              "grid_per_cu": tn.grid_per_cu}},
+                              "grid_per_cu": tn.grid_per_cu, "sched": tn.sched,
+                              "steal": tn.steal}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_tiles",

@@ -327,6 +327,7 @@ class Index:
         p = self.params(method, threshold, form, precision, flags, chunk)
         check(lib().issl_score_device(self._h, d_guides.ptr, n, byref(p), byref(res), stream))
 
-    def set_tuning(self, chunk: int, depth: int, grid_per_cu: int) -> None:
-        t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, 0.0)
+    def set_tuning(self, chunk: int, depth: int, grid_per_cu: int, sched: int = 2,
+                   steal: int = 8) -> None:
+        t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, sched, steal, 0.0)
         check(lib().issl_set_tuning(self._h, byref(t)))

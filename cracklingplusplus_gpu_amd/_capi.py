@@ -50,7 +50,8 @@ class issl_index_info(Structure):
 
 class issl_tuning(Structure):
     _fields_ = [("chunk", c_int32), ("block", c_int32), ("depth", c_int32),
-                ("grid_per_cu", c_int32), ("us_per_guide", c_double)]
+                ("grid_per_cu", c_int32), ("sched", c_int32), ("steal", c_int32),
+                ("us_per_guide", c_double)]
 
 
 class issl_synth_cfg(Structure):

@@ -1,0 +1,46 @@
+// ISSLScoringModule.hpp -- drop-in replacement of the reference's ISSL
+// pipeline stage (include/ISSLScoringModule.hpp:15-29): same constructor,
+// run(std::vector<guideResults>&) and processGuide gating, scoring on the
+// GPU through libissl_gpu.so.
+//
+// Differences from the reference, all on the resource side:
+//   * the index is opened once and stays resident in HBM across run() calls
+//     (the reference re-reads the file inside every run,
+//     lib/ISSLScoringModule.cpp:72-150);
+//   * config.offTarget.threads does not apply (no OpenMP team);
+//   * ISSL_FORM=batched in the environment selects the BATCHED form
+//     (FP64), otherwise the PLAIN form gives the reference's FP64 sums bit
+//     for bit.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "crackling_types.hpp"
+
+struct issl_index;
+
+class ISSLScoringModule {
+public:
+    explicit ISSLScoringModule(const cracklingConfig &config);
+    ~ISSLScoringModule();
+    ISSLScoringModule(const ISSLScoringModule &) = delete;
+    ISSLScoringModule &operator=(const ISSLScoringModule &) = delete;
+
+    void run(std::vector<guideResults> &candidateGuides);
+
+    // Signature helpers of lib/ISSLScoringModule.cpp:469-485.
+    static uint64_t sequenceToSignature(const std::string &seq, uint64_t seqLen);
+    static std::string signatureToSequence(uint64_t sig, uint64_t seqLen);
+
+private:
+    bool processGuide(const guideResults &guide) const;
+
+    offTargetConfig config;
+    std::filesystem::path ISSLIndex;
+    bool toolIsSelected;
+    optimisationLevel optimsationLevel;
+    uint8_t consensusN;
+    int device = 0;
+    issl_index *index = nullptr;
+};

@@ -51,16 +51,35 @@ struct TilePart {
     uint64_t sites;
 };
 
+// Queue counters sit 128 bytes apart so concurrent atomics on different
+// counters never share a line.
+constexpr uint32_t kCounterStride = 32;
+constexpr uint32_t kMaxWaves = 256 * 32 * kTileWaves;  // CUs x grid/CU x waves
+
 struct BatchWs {
     uint64_t *pre;       // [n][S+1]
     uint64_t *delta;     // [n][S]   posting index = position + delta
     uint32_t *gchunk;    // [n]
     uint32_t *gtiles;    // [n]
     uint32_t *tile_base; // [n+1]
-    uint32_t *ctrl;      // [0] queue counter, [1] total tiles
+    uint32_t *ctrl;      // [0] global queue counter, [1] total tiles
+    uint32_t *cnt;       // [waves * kCounterStride] per-wave range counters
     uint64_t *tile_map;  // [n*kMaxTiles] guide<<32 | k
     TilePart *part;      // [n*kMaxTiles]
 };
+
+// Tile scheduling of k_tiles.
+//   kSchedQueue : every wave pops the next tile from one global counter
+//   kSchedStatic: wave w takes tiles w, w+W, w+2W, ... (no atomics)
+//   kSchedSteal : wave w owns tiles [wT/W, (w+1)T/W) behind its own counter,
+//                 then steals from up to `steal` other waves' ranges; each
+//                 owner drains its own range, so every tile is processed
+enum { kSchedQueue = 0, kSchedStatic = 1, kSchedSteal = 2 };
+
+// Candidate ring entries: position within the tile (tiles are shorter than
+// 2^26 postings for any guide with P < 2^32) and the slice index (< 64).
+constexpr int kRelBits = 26;
+
 
 // ------------------------------------------------------------ FP helpers
 
@@ -153,36 +172,51 @@ struct PlanArgs {
     BatchWs ws;
 };
 
-__global__ void k_plan(PlanArgs a) {
-    const uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (g >= a.n) return;
+// One wave per guide, one lane per slice: the slice keys, list ranges and
+// their prefix are formed in parallel (two dependent loads per guide).
+__global__ __launch_bounds__(256) void k_plan(PlanArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t g = blockIdx.x * 4ULL + (threadIdx.x >> 6);
+    if (g >= a.n) return;  // uniform per wave
     const uint64_t q = a.guides[g] & kSigBits;
-    uint64_t run = 0;
-    uint64_t *pre = a.ws.pre + g * (a.S + 1);
-    uint64_t *delta = a.ws.delta + g * a.S;
-    for (int s = 0; s < a.S; ++s) {
-        const SliceMeta m = a.meta[s];
+    uint32_t b = 0, len = 0;
+    if (lane < a.S) {
+        const SliceMeta m = a.meta[lane];
         uint32_t key = 0;
         for (uint32_t j = 0; j < m.width; ++j)
             key |= static_cast<uint32_t>((q >> (2 * m.pos[j])) & 3ULL) << (2 * j);
-        const uint32_t b = a.lstart[m.lbase + key];
-        const uint32_t e = a.lstart[m.lbase + key + 1];
-        pre[s] = run;
-        // posting index of position p in slice s: s*N + b + (p - run)
-        delta[s] = static_cast<uint64_t>(s) * a.n_sites + b - run;
-        run += e - b;
+        b = a.lstart[m.lbase + key];
+        len = a.lstart[m.lbase + key + 1] - b;
     }
-    pre[a.S] = run;
-    uint64_t c = (run + kMaxTiles - 1) / kMaxTiles;
-    c = (c + 63) / 64 * 64;
-    if (c < a.chunk) c = a.chunk;
-    a.ws.gchunk[g] = static_cast<uint32_t>(c);
-    a.ws.gtiles[g] = static_cast<uint32_t>((run + c - 1) / c);
+    uint64_t incl = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t v = shfl64(incl, lane >= off ? lane - off : lane);
+        if (lane >= off) incl += v;
+    }
+    const uint64_t excl = incl - len;
+    const uint64_t run = a.S > 0 ? shfl64(incl, a.S - 1) : 0;
+    if (lane < a.S) {
+        a.ws.pre[g * (a.S + 1) + lane] = excl;
+        // posting index of position p in slice s: s*N + b + (p - pre[s])
+        a.ws.delta[g * a.S + lane] = static_cast<uint64_t>(lane) * a.n_sites + b - excl;
+    }
+    if (lane == 0) {
+        a.ws.pre[g * (a.S + 1) + a.S] = run;
+        uint64_t c = (run + kMaxTiles - 1) / kMaxTiles;
+        c = (c + 63) / 64 * 64;
+        if (c < a.chunk) c = a.chunk;
+        a.ws.gchunk[g] = static_cast<uint32_t>(c);
+        a.ws.gtiles[g] = static_cast<uint32_t>((run + c - 1) / c);
+    }
 }
 
 constexpr int kScanBlock = 1024;
 
-__global__ __launch_bounds__(kScanBlock) void k_scan(BatchWs ws, uint64_t n) {
+// Exclusive scan of the per-guide tile counts (one workgroup), total tile
+// count, and reset of the queue counters for the next k_tiles launch.
+__global__ __launch_bounds__(kScanBlock) void k_scan(BatchWs ws, uint64_t n, uint32_t n_counters) {
+    for (uint32_t i = threadIdx.x; i < n_counters; i += kScanBlock) ws.cnt[i * kCounterStride] = 0;
     __shared__ uint32_t s_part[kScanBlock];
     const int t = threadIdx.x;
     const uint64_t per = (n + kScanBlock - 1) / kScanBlock;
@@ -226,8 +260,10 @@ struct TileArgs {
     const void *mit;        // float or double by precision
     const void *cfd;
     int S;
-    int use_cut_distinct;   // mark distinct sites here (no early exit armed)
+    int mark_distinct;      // mark distinct sites here (no early exit armed)
     int calc_mit, calc_cfd;
+    int sched;              // kSched*
+    int steal;              // victims tried after the own range (kSchedSteal)
     uint64_t *distinct;
     BatchWs ws;
 };
@@ -240,6 +276,9 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __shared__ uint64_t s_fold[kMaxSlices];
     __shared__ uint64_t s_pre[kTileWaves][kMaxSlices + 1];
     __shared__ uint64_t s_delta[kTileWaves][kMaxSlices];
+    // Candidate ring: holds < 64 pending plus one round's 64 * DEPTH.
+    constexpr uint32_t kRingD = 64 * (DEPTH + 1);
+    __shared__ uint32_t s_ring[kTileWaves][kRingD];   // tile-relative position | slice << 26
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Real *cfdg = static_cast<const Real *>(a.cfd);
     const Real *mitg = static_cast<const Real *>(a.mit);
@@ -248,11 +287,37 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __syncthreads();
     const int S = a.S;
     const uint32_t total = a.ws.ctrl[1];
+    const uint32_t W = gridDim.x * kTileWaves;
+    const uint32_t w = blockIdx.x * kTileWaves + wave;
+    uint32_t iter = 0;   // kSchedStatic
+    int tries = 0;       // kSchedSteal: ranges visited after the own one
+    uint32_t v = w;      // kSchedSteal: range being drained
 
     for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(a.ws.ctrl, 1u);
-        t = static_cast<uint32_t>(__shfl(static_cast<int>(t), 0));
+        uint32_t t = total;
+        if (a.sched == kSchedStatic) {
+            const uint64_t tt = w + static_cast<uint64_t>(iter++) * W;
+            t = tt < total ? static_cast<uint32_t>(tt) : total;
+        } else if (a.sched == kSchedSteal) {
+            for (;;) {
+                const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(v) * total / W);
+                const uint32_t hi = static_cast<uint32_t>((static_cast<uint64_t>(v) + 1) * total / W);
+                uint32_t k = 0xFFFFFFFFu;
+                if (lane == 0 && lo < hi) k = atomicAdd(a.ws.cnt + static_cast<size_t>(v) * kCounterStride, 1u);
+                k = static_cast<uint32_t>(__shfl(static_cast<int>(k), 0));
+                if (lo < hi && k < hi - lo) {
+                    t = lo + k;
+                    break;
+                }
+                if (tries >= a.steal) break;
+                ++tries;
+                v = static_cast<uint32_t>((w + static_cast<uint64_t>(tries) * 61) % W);
+            }
+        } else {
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(a.ws.ctrl, 1u);
+            t = static_cast<uint32_t>(__shfl(static_cast<int>(k), 0));
+        }
         if (t >= total) break;
         const uint64_t gm = a.ws.tile_map[t];
         const uint64_t g = gm >> 32;
@@ -269,65 +334,107 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         Real am = 0, ac = 0;
         F2 fm{0.f, 0.f}, fc{0.f, 0.f};
         uint64_t occ_sum = 0, sites = 0;
+
+        // Candidates (dist <= 4) are queued in a per-wave LDS ring in position
+        // order and scored 64 at a time, one per lane, so the rare scoring
+        // path runs with every lane busy instead of once per round.  An entry
+        // is the tile-relative position and the slice; scoring reloads the
+        // site word (a cache hit).
+        uint32_t head = 0, tail = 0;  // wave-uniform ring cursors
+        auto drain = [&](uint32_t n) {
+            __builtin_amdgcn_wave_barrier();
+            if (static_cast<uint32_t>(lane) < n) {
+                const uint32_t ent = s_ring[wave][(head + lane) % kRingD];
+                const int sl = static_cast<int>(ent >> kRelBits);
+                const uint64_t addr = a0 + (ent & ((1u << kRelBits) - 1)) + s_delta[wave][sl];
+                const uint64_t wv = a.site[addr];
+                const uint64_t sg = wv & kSigBits;
+                const uint64_t f = bfold(q ^ sg);
+                const int dist = __popcll(f);
+                bool first = true;
+                for (int s2 = 0; s2 < sl; ++s2)
+                    if ((f & s_fold[s2]) == 0) {
+                        first = false;
+                        break;
+                    }
+                if (first) {
+                    uint32_t occ = static_cast<uint32_t>(wv >> 40);
+                    if (occ == kOccEscape || a.mark_distinct) {
+                        const uint64_t pe = a.post[addr];
+                        occ = static_cast<uint32_t>(pe >> 32);
+                        if (a.mark_distinct)
+                            atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
+                                         (static_cast<uint32_t>(pe) >> 6),
+                                     1ULL << (static_cast<uint32_t>(pe) & 63));
+                    }
+                    occ_sum += occ;
+                    sites += 1;
+                    const Real ro = static_cast<Real>(occ);
+                    Real tm = 0, tc = 0;
+                    if (a.calc_mit && dist > 0) tm = mitg[brank(f, dist)] * ro;
+                    if (a.calc_cfd) tc = cfd_product<Real>(q, sg, f, dist, s_cfd) * ro;
+                    if constexpr (PREC == ISSL_PREC_FP32C) {
+                        fm = f2_add(fm, tm);
+                        fc = f2_add(fc, tc);
+                    } else {
+                        am += tm;
+                        ac += tc;
+                    }
+                }
+            }
+            head += n;
+            __builtin_amdgcn_wave_barrier();
+        };
+
+        // Per-lane slice cursor: the next slice boundary and this slice's
+        // posting offset stay in registers; LDS is read only at boundaries.
         int cur = 0;
+        uint64_t nb, dl;
         {
             const uint64_t p0 = a0 + lane;
             while (cur < S - 1 && p0 >= s_pre[wave][cur + 1]) ++cur;
+            nb = s_pre[wave][cur + 1];
+            dl = s_delta[wave][cur];
         }
         for (uint64_t base = a0; base < a1; base += 64 * DEPTH) {
-            uint64_t w[DEPTH], addr[DEPTH];
+            uint64_t wv[DEPTH];
             int sl[DEPTH];
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
                 const uint64_t pos = base + 64 * d + lane;
                 sl[d] = -1;
-                addr[d] = 0;
-                w[d] = 0;
+                wv[d] = 0;
                 if (pos < a1) {
-                    while (pos >= s_pre[wave][cur + 1]) ++cur;
+                    if (pos >= nb) {
+                        do {
+                            ++cur;
+                            nb = s_pre[wave][cur + 1];
+                        } while (pos >= nb);
+                        dl = s_delta[wave][cur];
+                    }
                     sl[d] = cur;
-                    addr[d] = pos + s_delta[wave][cur];
-                    w[d] = a.site[addr[d]];
+                    wv[d] = a.site[pos + dl];
                 }
             }
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
-                if (sl[d] < 0) continue;
-                const uint64_t sg = w[d] & kSigBits;
-                const uint64_t f = bfold(q ^ sg);
-                const int dist = __popcll(f);
-                if (dist > 4) continue;
-                bool first = true;
-                for (int s2 = 0; s2 < sl[d]; ++s2)
-                    if ((f & s_fold[s2]) == 0) {
-                        first = false;
-                        break;
+                bool cand = false;
+                if (sl[d] >= 0) cand = __popcll(bfold(q ^ (wv[d] & kSigBits))) <= 4;
+                const uint64_t b = __ballot(cand);
+                if (b) {
+                    const uint32_t r = __builtin_amdgcn_mbcnt_hi(
+                        static_cast<uint32_t>(b >> 32),
+                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+                    if (cand) {
+                        const uint32_t rel = static_cast<uint32_t>(base - a0) + 64 * d + lane;
+                        s_ring[wave][(tail + r) % kRingD] = rel | (static_cast<uint32_t>(sl[d]) << kRelBits);
                     }
-                if (!first) continue;
-                uint32_t occ = static_cast<uint32_t>(w[d] >> 40);
-                if (occ == kOccEscape || a.use_cut_distinct) {
-                    const uint64_t e = a.post[addr[d]];
-                    occ = static_cast<uint32_t>(e >> 32);
-                    if (a.use_cut_distinct)
-                        atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
-                                     (static_cast<uint32_t>(e) >> 6),
-                                 1ULL << (static_cast<uint32_t>(e) & 63));
-                }
-                occ_sum += occ;
-                sites += 1;
-                const Real ro = static_cast<Real>(occ);
-                Real tm = 0, tc = 0;
-                if (a.calc_mit && dist > 0) tm = mitg[brank(f, dist)] * ro;
-                if (a.calc_cfd) tc = cfd_product<Real>(q, sg, f, dist, s_cfd) * ro;
-                if constexpr (PREC == ISSL_PREC_FP32C) {
-                    fm = f2_add(fm, tm);
-                    fc = f2_add(fc, tc);
-                } else {
-                    am += tm;
-                    ac += tc;
+                    tail += static_cast<uint32_t>(__popcll(b));
                 }
             }
+            while (tail - head >= 64) drain(64);
         }
+        while (tail != head) drain(std::min<uint32_t>(64, tail - head));
         // Fixed butterfly: every lane ends with the same, order-determined value.
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
@@ -562,9 +669,11 @@ int carve(Workspace &ws, uint64_t n, int S, BatchWs *b) {
     const size_t sz_u32 = align256(n * 4);
     const size_t sz_base = align256((n + 1) * 4);
     const size_t sz_ctrl = 256;
+    const size_t sz_cnt = align256(static_cast<size_t>(kMaxWaves) * kCounterStride * 4);
     const size_t sz_map = align256(n * kMaxTiles * 8);
     const size_t sz_part = align256(n * kMaxTiles * sizeof(TilePart));
-    const size_t total = sz_pre + sz_delta + 2 * sz_u32 + sz_base + sz_ctrl + sz_map + sz_part;
+    const size_t total =
+        sz_pre + sz_delta + 2 * sz_u32 + sz_base + sz_ctrl + sz_cnt + sz_map + sz_part;
     int rc = ws.tiles.reserve(total);
     if (rc != ISSL_OK) return rc;
     char *p = static_cast<char *>(ws.tiles.p);
@@ -580,6 +689,8 @@ int carve(Workspace &ws, uint64_t n, int S, BatchWs *b) {
     p += sz_base;
     b->ctrl = reinterpret_cast<uint32_t *>(p);
     p += sz_ctrl;
+    b->cnt = reinterpret_cast<uint32_t *>(p);
+    p += sz_cnt;
     b->tile_map = reinterpret_cast<uint64_t *>(p);
     p += sz_map;
     b->part = reinterpret_cast<TilePart *>(p);
@@ -604,6 +715,21 @@ int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st) 
     return ISSL_OK;
 }
 
+}  // namespace
+
+issl_tuning default_tuning() {
+    issl_tuning t{};
+    t.chunk = 2048;
+    t.block = kTileBlock;
+    t.depth = 8;
+    t.grid_per_cu = 4;
+    t.sched = kSchedSteal;
+    t.steal = 8;
+    return t;
+}
+
+namespace {
+
 int cu_count(int device) {
     static int cached[64] = {0};
     if (device >= 0 && device < 64 && cached[device]) return cached[device];
@@ -623,9 +749,8 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     if (n == 0) return ISSL_OK;
     if (n > (0xFFFFFFFFULL / kMaxTiles)) return fail(ISSL_ERR_ARG, "too many guides for one launch");
     issl_tuning tn = tn_in;
-    if (tn.chunk <= 0) tn.chunk = 512;
-    if (tn.depth <= 0) tn.depth = 4;
-    if (tn.grid_per_cu <= 0) tn.grid_per_cu = 8;
+    if (tn.chunk <= 0) tn = default_tuning();
+    if (tn.sched < 0 || tn.sched > kSchedSteal) return fail(ISSL_ERR_ARG, "unknown tile schedule");
     if (chunk > 0) tn.chunk = chunk;
     if (tn.chunk % 64 != 0 || tn.chunk > (1 << 20))
         return fail(ISSL_ERR_ARG, "chunk must be a multiple of 64 up to 2^20");
@@ -634,10 +759,18 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     int rc = carve(ws, n, S, &b);
     if (rc != ISSL_OK) return rc;
 
+    // Persistent grid: grid_per_cu workgroups per CU, no more waves than tiles can need.
+    const uint64_t max_tiles = n * kMaxTiles;
+    uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
+    grid = std::min<uint64_t>(grid, kMaxWaves / kTileWaves);
+    const uint64_t waves_needed = std::max<uint64_t>(1, std::min<uint64_t>(max_tiles, grid * kTileWaves));
+    grid = std::min<uint64_t>(grid, (waves_needed + kTileWaves - 1) / kTileWaves);
+    const uint32_t n_waves = static_cast<uint32_t>(grid * kTileWaves);
+
     PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk), b};
     const uint32_t gb = static_cast<uint32_t>((n + 255) / 256);
-    hipLaunchKernelGGL(k_plan, dim3(gb), dim3(256), 0, st, pa);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, st, b, n);
+    hipLaunchKernelGGL(k_plan, dim3(static_cast<uint32_t>((n + 3) / 4)), dim3(256), 0, st, pa);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, st, b, n, n_waves);
     hipLaunchKernelGGL(k_fillmap, dim3(gb), dim3(256), 0, st, b, n);
 
     const bool fp32 = precision == ISSL_PREC_FP32C;
@@ -649,15 +782,13 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     ta.mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
     ta.cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
     ta.S = S;
-    ta.use_cut_distinct = (distinct && !sc.use_cut) ? 1 : 0;
+    ta.mark_distinct = (distinct && !sc.use_cut) ? 1 : 0;
     ta.calc_mit = sc.calc_mit;
     ta.calc_cfd = sc.calc_cfd;
+    ta.sched = tn.sched;
+    ta.steal = tn.steal;
     ta.distinct = distinct;
     ta.ws = b;
-    const uint64_t max_tiles = n * kMaxTiles;
-    uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
-    const uint64_t waves_needed = std::max<uint64_t>(1, std::min<uint64_t>(max_tiles, grid * kTileWaves));
-    grid = std::min<uint64_t>(grid, (waves_needed + kTileWaves - 1) / kTileWaves);
     timing_mark(ix, st);
     rc = fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st)
               : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st);

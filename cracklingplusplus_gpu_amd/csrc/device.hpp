@@ -115,6 +115,8 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
                    int precision, int chunk, const issl_tuning &tn, const DevResults &out,
                    uint64_t *distinct, Workspace &ws, hipStream_t stream);
 int build_inline_layout(issl_index *ix);
+// Launch shape used before any tuning.
+issl_tuning default_tuning();
 // Record a timing event on `stream` when timing is enabled (no-op otherwise).
 void timing_mark(issl_index *ix, hipStream_t stream);
 

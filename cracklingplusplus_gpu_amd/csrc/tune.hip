@@ -45,7 +45,9 @@ bool cache_lookup(const char *path, const std::string &key, issl_tuning *t) {
         std::istringstream ls(line);
         std::string k;
         issl_tuning v{};
-        if (ls >> k >> v.chunk >> v.block >> v.depth >> v.grid_per_cu >> v.us_per_guide && k == key) {
+        if (ls >> k >> v.chunk >> v.block >> v.depth >> v.grid_per_cu >> v.sched >> v.steal >>
+                v.us_per_guide &&
+            k == key) {
             *t = v;
             found = true;  // the last entry for a key wins
         }
@@ -57,11 +59,7 @@ bool cache_lookup(const char *path, const std::string &key, issl_tuning *t) {
 
 extern "C" int issl_get_tuning(issl_index *ix, issl_tuning *out) {
     if (!ix || !out) return fail(ISSL_ERR_ARG, "null argument");
-    *out = ix->tuning;
-    if (out->chunk <= 0) out->chunk = 512;
-    if (out->depth <= 0) out->depth = 4;
-    if (out->grid_per_cu <= 0) out->grid_per_cu = 8;
-    out->block = 256;
+    *out = ix->tuning.chunk > 0 ? ix->tuning : default_tuning();
     return ok();
 }
 
@@ -70,7 +68,8 @@ extern "C" int issl_set_tuning(issl_index *ix, const issl_tuning *in) {
     const int d = in->depth;
     if (in->chunk < 64 || in->chunk % 64 != 0 || in->chunk > (1 << 20) ||
         !(d == 1 || d == 2 || d == 4 || d == 8 || d == 16) || in->grid_per_cu < 1 ||
-        in->grid_per_cu > 64)
+        in->grid_per_cu > 32 || in->sched < 0 || in->sched > 2 || in->steal < 0 ||
+        in->steal > 1024)
         return fail(ISSL_ERR_ARG, "tuning out of range");
     std::lock_guard<std::mutex> lk(ix->mu);
     ix->tuning = *in;
@@ -112,16 +111,18 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     hipEvent_t e0, e1;
     ISSL_HIP(hipEventCreate(&e0));
     ISSL_HIP(hipEventCreate(&e1));
-    const int chunks[] = {256, 512, 1024, 2048, 4096};
-    const int depths[] = {2, 4, 8, 16};
-    const int grids[] = {4, 8, 16};
+    const int chunks[] = {512, 1024, 2048, 4096, 8192};
+    const int depths[] = {4, 8, 16};
+    const int grids[] = {2, 4, 8};
+    const int scheds[][2] = {{2, 8}, {1, 0}, {0, 0}};
     issl_tuning top{};
     top.us_per_guide = 1e300;
     for (int c : chunks)
-        for (int d : depths) {
+        for (int d : depths)
+            for (const auto &sch : scheds) {
             if (64 * d > c) continue;
             for (int gpc : grids) {
-                issl_tuning t{c, 256, d, gpc, 0};
+                issl_tuning t{c, 256, d, gpc, sch[0], sch[1], 0};
                 // warm-up, then the best of three timed launches
                 if ((rc = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st)) !=
                     ISSL_OK)
@@ -150,8 +151,8 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%s %d %d %d %d %.6g\n", key.c_str(), top.chunk, top.block, top.depth,
-                    top.grid_per_cu, top.us_per_guide);
+            fprintf(f, "%s %d %d %d %d %d %d %.6g\n", key.c_str(), top.chunk, top.block, top.depth,
+                    top.grid_per_cu, top.sched, top.steal, top.us_per_guide);
             fclose(f);
         }
     }

@@ -167,10 +167,14 @@ int issl_candidates(issl_index *ix, const uint64_t *guides, uint64_t n,
 /* ------------------------------------------------------------- autotuner */
 
 typedef struct {
-    int32_t chunk;        /* postings per tile */
-    int32_t block;        /* threads per workgroup (64..1024, multiple of 64) */
-    int32_t depth;        /* postings in flight per lane */
-    int32_t grid_per_cu;  /* persistent workgroups per CU */
+    int32_t chunk;        /* postings per tile (minimum; heavy guides get longer
+                             tiles so no guide has more than 64) */
+    int32_t block;        /* threads per workgroup (fixed at 256) */
+    int32_t depth;        /* postings in flight per lane: 1, 2, 4, 8 or 16 */
+    int32_t grid_per_cu;  /* persistent workgroups per CU: 1..32 */
+    int32_t sched;        /* tile schedule: 0 one global queue, 1 static
+                             round-robin, 2 per-wave ranges + stealing */
+    int32_t steal;        /* ranges of other waves tried after the own one */
     double  us_per_guide; /* measured */
 } issl_tuning;
 

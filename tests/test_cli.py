@@ -1,0 +1,124 @@
+"""Drop-in CLI and pipeline module: argument handling (CPU) and outputs (GPU).
+
+The CLI mirrors ISSLScoreOfftargets/ISSLScoreOfftargets.cpp:36-647 and the
+module lib/ISSLScoringModule.cpp:13-513; expected values come from the
+reference's golden vectors and the oracle.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cracklingplusplus_gpu_amd as issl
+import oracle_lib as orc
+from conftest import GOLDEN, ROOT, golden_index, load_golden
+
+BIN = os.path.join(ROOT, "cracklingplusplus_gpu_amd", "bin")
+CLI = os.path.join(BIN, "ISSLScoreOfftargets")
+DEMO = os.path.join(BIN, "issl_module_demo")
+TINY = os.path.join(GOLDEN, "tiny4k_5x4.issl")
+CLI_METHOD = {"and": 2, "or": 3, "avg": 4, "mit": 0, "cfd": 1}
+
+
+def run(args, **kw):
+    return subprocess.run(args, capture_output=True, text=True, timeout=120, **kw)
+
+
+def test_cli_usage_and_bad_method():
+    r = run([CLI])
+    assert r.returncode == 1 and "Usage:" in r.stderr
+    r = run([CLI, TINY, "q.txt", "4", "75", "best"])
+    assert r.returncode == 1 and "Invalid scoring method" in r.stderr
+
+
+def test_module_rejects_bad_method(tmp_path):
+    g = tmp_path / "g.txt"
+    g.write_text("ACGTACGTACGTACGTACGTAGG 1 0 ?\n")
+    r = run([DEMO, TINY, str(g), "best", "75", "ultralow", "1", "0"])
+    assert r.returncode == 1 and "Scoring method is not valid." in r.stderr
+
+
+def test_module_disabled_skips(tmp_path):
+    g = tmp_path / "g.txt"
+    g.write_text("ACGTACGTACGTACGTACGTAGG 1 0 ?\n")
+    r = run([DEMO, TINY, str(g), "mit", "75", "ultralow", "0", "0"])
+    assert r.returncode == 0
+    assert "Skipping Off-target scoring" in r.stdout
+    line = [l for l in r.stdout.splitlines() if l.startswith("RESULT")][0].split("\t")
+    assert line[4] == "?" and float(line[2]) == -1.7976931348623157e308
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["and", "or", "avg", "mit", "cfd"])
+def test_cli_output_matches_reference(tmp_path, name, gpu_available):
+    gd = load_golden("tiny4k_5x4")
+    hx = golden_index("tiny4k_5x4")
+    g = gd["guides"]
+    seqs = issl.decode(g)
+    q = tmp_path / "queries.txt"
+    q.write_text("".join(s + "\n" for s in seqs))
+    out = tmp_path / "res.txt"
+    r = run([CLI, TINY, str(q), "4", "75", name, str(out)])
+    assert r.returncode == 0, r.stderr
+    m = CLI_METHOD[name]
+    key = f"m{m}_t75"
+    mit, cfd = gd[key + "_mit"], gd[key + "_cfd"]
+    o = orc.score(hx, g, m, 75.0, distinct=True)
+    lines = [l for l in r.stdout.splitlines() if l.count("\t") == 2]
+    assert len(lines) == len(seqs)
+    for i, l in enumerate(lines):
+        ms = "%f" % mit[i] if m != 1 else "-1"
+        cs = "%f" % cfd[i] if m != 0 else "-1"
+        assert l == f"{seqs[i]}\t{ms}\t{cs}"
+    file_lines = out.read_text().splitlines()
+    assert file_lines[0] == "# Query\tMIT_Score\tCFD_Score\tOfftarget_Count"
+    for i, l in enumerate(file_lines[1:]):
+        parts = l.split("\t")
+        assert parts[0] == seqs[i] and int(parts[3]) == int(o["occ_count"][i])
+    want_distinct = sum(bin(int(w)).count("1") for w in o["distinct"])
+    assert f"Distinct off-targets: {want_distinct}" in r.stderr
+    assert f"Total off-targets scored: {int(o['occ_count'].sum())}" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_rejects_ragged_query_file(tmp_path, gpu_available):
+    q = tmp_path / "q.txt"
+    q.write_text("ACGTACGTACGTACGTACGT\nACG\n")
+    r = run([CLI, TINY, str(q), "4", "75", "and", str(tmp_path / "o.txt")])
+    assert r.returncode == 1 and "not a multiple of the expected line length (21)" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,level", [("mitAndCfd", "ultralow"), ("avgMitCfd", "medium"),
+                                          ("mit", "low"), ("cfd", "high"),
+                                          ("mitOrCfd", "medium")])
+def test_module_matches_reference(tmp_path, method, level, gpu_available):
+    gd = load_golden("tiny4k_5x4")
+    g = gd["guides"]
+    seqs = issl.decode(g)
+    rng = np.random.default_rng(3)
+    uniq = rng.random(len(g)) < 0.8
+    cons = rng.integers(0, 4, len(g))
+    bt = rng.choice(["1", "0", "?"], len(g))
+    f = tmp_path / "g.txt"
+    f.write_text("".join(f"{s}AGG {int(u)} {c} {b}\n" for s, u, c, b in zip(seqs, uniq, cons, bt)))
+    r = run([DEMO, TINY, str(f), method, "75", level, "1", "2"])
+    assert r.returncode == 0, r.stderr
+    m = issl.METHOD_NAMES[method]
+    key = f"m{m}_t75"
+    res = [l.split("\t") for l in r.stdout.splitlines() if l.startswith("RESULT")]
+    assert len(res) == len(g)
+    tested = 0
+    for i, row in enumerate(res):
+        keep = level == "ultralow" or (
+            uniq[i] and (level == "low" or (cons[i] >= 2 and bt[i] != "0")))
+        if not keep:
+            assert row[4] == "?"
+            continue
+        tested += 1
+        assert float(row[2]) == gd[key + "_mit"][i]
+        assert float(row[3]) == gd[key + "_cfd"][i]
+        assert row[4] == chr(gd[key + "_pass"][i])
+    failed = sum(1 for i, row in enumerate(res) if row[4] == "0")
+    assert f"{failed:,} of {tested:,} failed here." in r.stdout
