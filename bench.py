@@ -119,7 +119,8 @@ def main() -> int:
     os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
     tn = ix.autotune(guides, args.tune_cache)
     log(f"tuning: chunk {tn.chunk} depth {tn.depth} grid/CU {tn.grid_per_cu} "
-        f"sched {tn.sched} steal {tn.steal} ({tn.us_per_guide * 1e3:.2f} ns/guide)")
+        f"sched {tn.sched} steal {tn.steal} stream depth {tn.direct_depth} "
+        f"heavy {tn.heavy_kpost} ({tn.us_per_guide * 1e3:.2f} ns/guide)")
     setup_s = time.time() - t_setup
 
     kw = dict(method=issl.METHOD_MIT_AND_CFD, threshold=0.0, form=issl.FORM_BATCHED,
@@ -231,7 +232,8 @@ def main() -> int:
                    "postings_per_step": p_total,
                    "tuning": {"chunk": tn.chunk, "depth": tn.depth,
                               "grid_per_cu": tn.grid_per_cu, "sched": tn.sched,
-                              "steal": tn.steal}},
+                              "steal": tn.steal, "stream_depth": tn.direct_depth,
+                              "heavy_kpost": tn.heavy_kpost}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_tiles",

@@ -181,12 +181,10 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a) {
     const uint64_t q = a.guides[g] & kSigBits;
     uint32_t b = 0, len = 0;
     if (lane < a.S) {
-        const SliceMeta m = a.meta[lane];
-        uint32_t key = 0;
-        for (uint32_t j = 0; j < m.width; ++j)
-            key |= static_cast<uint32_t>((q >> (2 * m.pos[j])) & 3ULL) << (2 * j);
-        b = a.lstart[m.lbase + key];
-        len = a.lstart[m.lbase + key + 1] - b;
+        const uint32_t lb = a.meta[lane].lbase;
+        const uint32_t key = slice_key(q, a.meta[lane].fold);
+        b = a.lstart[lb + key];
+        len = a.lstart[lb + key + 1] - b;
     }
     uint64_t incl = len;
 #pragma unroll
@@ -213,41 +211,50 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a) {
 
 constexpr int kScanBlock = 1024;
 
-// Exclusive scan of the per-guide tile counts (one workgroup), total tile
-// count, and reset of the queue counters for the next k_tiles launch.
+// One workgroup: exclusive scan of the per-guide tile counts (tile_base,
+// total), the tile -> (guide, k) map, and reset of the per-wave range
+// counters (n_counters > 0 only when k_tiles steals).  Each thread owns a
+// contiguous run of guides; the run totals are scanned with wave shuffles
+// and one LDS pass over the 16 wave totals.
 __global__ __launch_bounds__(kScanBlock) void k_scan(BatchWs ws, uint64_t n, uint32_t n_counters) {
-    for (uint32_t i = threadIdx.x; i < n_counters; i += kScanBlock) ws.cnt[i * kCounterStride] = 0;
-    __shared__ uint32_t s_part[kScanBlock];
-    const int t = threadIdx.x;
+    __shared__ uint32_t s_wave[kScanBlock / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (uint32_t i = t; i < n_counters; i += kScanBlock) ws.cnt[i * kCounterStride] = 0;
     const uint64_t per = (n + kScanBlock - 1) / kScanBlock;
-    const uint64_t lo = per * t, hi = std::min<uint64_t>(n, lo + per);
+    const uint64_t lo = std::min<uint64_t>(n, per * t), hi = std::min<uint64_t>(n, lo + per);
     uint32_t sum = 0;
     for (uint64_t i = lo; i < hi; ++i) sum += ws.gtiles[i];
-    s_part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < kScanBlock; off <<= 1) {
-        const uint32_t v = t >= off ? s_part[t - off] : 0u;
-        __syncthreads();
-        s_part[t] += v;
-        __syncthreads();
+    uint32_t incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(incl), lane >= off ? lane - off : lane));
+        if (lane >= off) incl += v;
     }
-    uint32_t run = s_part[t] - sum;
-    for (uint64_t i = lo; i < hi; ++i) {
-        ws.tile_base[i] = run;
-        run += ws.gtiles[i];
+    if (lane == 63) s_wave[wv] = incl;
+    __syncthreads();
+    if (wv == 0) {
+        uint32_t x = lane < kScanBlock / 64 ? s_wave[lane] : 0u;
+#pragma unroll
+        for (int off = 1; off < kScanBlock / 64; off <<= 1) {
+            const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(x), lane >= off ? lane - off : lane));
+            if (lane >= off) x += v;
+        }
+        if (lane < kScanBlock / 64) s_wave[lane] = x;  // inclusive wave totals
+    }
+    __syncthreads();
+    uint32_t run = (wv > 0 ? s_wave[wv - 1] : 0u) + incl - sum;
+    for (uint64_t g = lo; g < hi; ++g) {
+        const uint32_t k = ws.gtiles[g];
+        ws.tile_base[g] = run;
+        for (uint32_t i = 0; i < k; ++i) ws.tile_map[run + i] = (g << 32) | i;
+        run += k;
     }
     if (t == kScanBlock - 1) {
-        ws.tile_base[n] = s_part[t];
-        ws.ctrl[0] = 0;            // queue counter
-        ws.ctrl[1] = s_part[t];    // total tiles
+        const uint32_t total = s_wave[kScanBlock / 64 - 1];
+        ws.tile_base[n] = total;
+        ws.ctrl[0] = 0;      // global queue counter
+        ws.ctrl[1] = total;  // total tiles
     }
-}
-
-__global__ void k_fillmap(BatchWs ws, uint64_t n) {
-    const uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (g >= n) return;
-    const uint32_t b = ws.tile_base[g], k = ws.gtiles[g];
-    for (uint32_t i = 0; i < k; ++i) ws.tile_map[b + i] = (g << 32) | i;
 }
 
 // ------------------------------------------------------------ tiles
@@ -324,7 +331,8 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         const uint32_t k = static_cast<uint32_t>(gm);
         const uint64_t q = a.guides[g] & kSigBits;
         const uint64_t C = a.ws.gchunk[g];
-        if (lane <= S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
+        if (lane < S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
+        if (lane == 0) s_pre[wave][S] = a.ws.pre[g * (S + 1) + S];  // S may be 64
         if (lane < S) s_delta[wave][lane] = a.ws.delta[g * S + lane];
         __builtin_amdgcn_wave_barrier();
         const uint64_t P = s_pre[wave][S];
@@ -483,6 +491,7 @@ struct FinArgs {
     ScoreScalars sc;
     DevResults out;
     uint64_t *distinct;     // mark sites up to the exit (early exit armed)
+    int skip_untiled;       // guides without tiles were finished by k_stream
     BatchWs ws;
 };
 
@@ -501,6 +510,7 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     const int S = a.S;
     const uint32_t base = a.ws.tile_base[g];
     const uint32_t K = a.ws.tile_base[g + 1] - base;
+    if (K == 0 && a.skip_untiled) return;
     TilePart mine{};
     if (lane < static_cast<int>(K)) mine = a.ws.part[base + lane];
 
@@ -537,7 +547,8 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
 
     const uint64_t q = a.guides[g] & kSigBits;
     if (exit_tile >= 0 || a.distinct) {
-        if (lane <= S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
+        if (lane < S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
+        if (lane == 0) s_pre[wave][S] = a.ws.pre[g * (S + 1) + S];  // S may be 64
         if (lane < S) s_delta[wave][lane] = a.ws.delta[g * S + lane];
         __builtin_amdgcn_wave_barrier();
     }
@@ -657,6 +668,293 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     if (o.pass) o.pass[g] = reject ? '0' : '1';
 }
 
+// ------------------------------------------------------------ direct
+
+struct StreamArgs {
+    const uint64_t *guides;
+    uint64_t n;
+    const uint64_t *site;   // signature-inline words
+    const uint64_t *post;   // postings (escape occ, ids)
+    const uint32_t *lstart;
+    const SliceMeta *meta;
+    const void *mit;
+    const void *cfd;
+    int S;
+    uint64_t n_sites;
+    ScoreScalars sc;
+    DevResults out;
+    uint64_t *distinct;     // mark scored sites up to the exit (nullable)
+    uint64_t heavy;         // guides with more postings go to the tile kernel (0: none)
+    uint32_t chunk;         // tile length for those guides
+    BatchWs ws;
+};
+
+constexpr uint64_t kHanded = ~0ULL;  // plan handed to the tile kernel
+
+// Persistent guide streams.  Wave w scores guides w, w+W, w+2W, ... one after
+// another.  For each guide it streams the slice-major site words, DEPTH loads
+// per lane in flight, queues the candidates (dist <= 4) in position order in
+// an LDS ring and scores them 64 at a time: terms in parallel, then added in
+// position order with the exit test after each, as the reference's loop does
+// (lib/ISSLScoringModule.cpp:231-401).  In FP64 the sums are therefore the
+// reference's, bit for bit.  The next guide is planned while the current one
+// streams: its signature is loaded two guides ahead and its list bounds are
+// requested right behind the current guide's first round of loads.  Guides
+// longer than `heavy` postings leave their plan for the persistent tile
+// kernel instead.
+template <int PREC, int DEPTH>
+__global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
+    using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
+    constexpr int WAVES = 4;
+    constexpr uint32_t RING = 64 * (DEPTH + 1);
+    __shared__ Real s_cfd[336];
+    __shared__ uint64_t s_fold[kMaxSlices];
+    __shared__ uint32_t s_lbase[kMaxSlices];
+    __shared__ uint64_t s_pre[WAVES][2][kMaxSlices + 1];
+    __shared__ uint64_t s_delta[WAVES][2][kMaxSlices];
+    __shared__ uint32_t s_rpos[WAVES][RING];
+    __shared__ uint8_t s_rsl[WAVES][RING];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const Real *cfdg = static_cast<const Real *>(a.cfd);
+    const Real *mitg = static_cast<const Real *>(a.mit);
+    for (int i = tid; i < 336; i += 256) s_cfd[i] = cfdg[i];
+    if (tid < a.S) {
+        s_fold[tid] = a.meta[tid].fold;
+        s_lbase[tid] = a.meta[tid].lbase;
+    }
+    __syncthreads();
+    const int S = a.S;
+    const uint64_t W = gridDim.x * static_cast<uint64_t>(WAVES);
+    uint64_t g = blockIdx.x * static_cast<uint64_t>(WAVES) + wave;
+    if (g >= a.n) return;  // uniform per wave; no workgroup barrier follows
+
+    // Plan stage 1: list bounds of every slice (lane s) for signature qq.
+    auto plan_issue = [&](uint64_t qq, uint32_t &bb, uint32_t &ee) {
+        bb = 0;
+        ee = 0;
+        if (lane < S) {
+            const uint32_t key = slice_key(qq, s_fold[lane]);
+            const uint32_t lb = s_lbase[lane];
+            bb = a.lstart[lb + key];
+            ee = a.lstart[lb + key + 1];
+        }
+    };
+    // Plan stage 2: prefix of the list lengths into LDS buffer buf; returns
+    // P, or kHanded when the guide goes to the tile kernel.
+    auto plan_finish = [&](uint64_t gg, uint32_t bb, uint32_t ee, int buf) -> uint64_t {
+        const uint32_t len = ee - bb;
+        uint64_t incl = len;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t v = shfl64(incl, lane >= off ? lane - off : lane);
+            if (lane >= off) incl += v;
+        }
+        const uint64_t excl = incl - len;
+        const uint64_t P = S > 0 ? shfl64(incl, S - 1) : 0;
+        const uint64_t dl_s = static_cast<uint64_t>(lane) * a.n_sites + bb - excl;
+        if (lane < S) {
+            s_pre[wave][buf][lane] = excl;
+            s_delta[wave][buf][lane] = dl_s;
+        }
+        if (lane == 0) s_pre[wave][buf][S] = P;  // S may be 64: no lane S
+        if (a.heavy) {
+            if (P > a.heavy) {
+                if (lane < S) {
+                    a.ws.pre[gg * (S + 1) + lane] = excl;
+                    a.ws.delta[gg * S + lane] = dl_s;
+                }
+                if (lane == 0) {
+                    a.ws.pre[gg * (S + 1) + S] = P;
+                    uint64_t c = (P + kMaxTiles - 1) / kMaxTiles;
+                    c = (c + 63) / 64 * 64;
+                    if (c < a.chunk) c = a.chunk;
+                    a.ws.gchunk[gg] = static_cast<uint32_t>(c);
+                    a.ws.gtiles[gg] = static_cast<uint32_t>((P + c - 1) / c);
+                }
+                return kHanded;
+            }
+            if (lane == 0) a.ws.gtiles[gg] = 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        return P;
+    };
+
+    uint64_t qc = a.guides[g] & kSigBits;
+    uint64_t Pc;
+    {
+        uint32_t b0, e0;
+        plan_issue(qc, b0, e0);
+        Pc = plan_finish(g, b0, e0, 0);
+    }
+    int cb = 0;
+    uint64_t qn = (g + W < a.n) ? (a.guides[g + W] & kSigBits) : 0;
+
+    for (;;) {
+        const uint64_t gn = g + W;
+        const bool has_next = gn < a.n;
+        const uint64_t qnn = (gn + W < a.n) ? (a.guides[gn + W] & kSigBits) : 0;
+        uint32_t bn = 0, en = 0;
+        bool issued = !has_next;
+
+        if (Pc != kHanded) {
+            F2 fm{0.f, 0.f}, fc{0.f, 0.f};
+            double dm = 0.0, dc = 0.0;
+            uint64_t occ_sum = 0, sites = 0;
+            bool done = false;
+            uint32_t head = 0, tail = 0;  // wave-uniform ring cursors
+            const uint64_t q = qc;
+
+            // Score ring entries [head, head + n): terms per lane, then the
+            // running sums in position order with the exit test after each hit.
+            auto drain = [&](uint32_t nn) {
+                __builtin_amdgcn_wave_barrier();
+                bool hit = false;
+                Real tm = 0, tc = 0;
+                uint32_t occ = 0;
+                uint64_t addr = 0;
+                if (static_cast<uint32_t>(lane) < nn) {
+                    const uint32_t e = (head + lane) % RING;
+                    const int sl = s_rsl[wave][e];
+                    addr = static_cast<uint64_t>(s_rpos[wave][e]) + s_delta[wave][cb][sl];
+                    const uint64_t wv = a.site[addr];
+                    const uint64_t sg = wv & kSigBits;
+                    const uint64_t f = bfold(q ^ sg);
+                    const int dist = __popcll(f);
+                    hit = true;
+                    for (int s2 = 0; s2 < sl; ++s2)
+                        if ((f & s_fold[s2]) == 0) {
+                            hit = false;
+                            break;
+                        }
+                    if (hit) {
+                        occ = static_cast<uint32_t>(wv >> 40);
+                        if (occ == kOccEscape) occ = static_cast<uint32_t>(a.post[addr] >> 32);
+                        const Real ro = static_cast<Real>(occ);
+                        if (a.sc.calc_mit && dist > 0) tm = mitg[brank(f, dist)] * ro;
+                        if (a.sc.calc_cfd) tc = cfd_product<Real>(q, sg, f, dist, s_cfd) * ro;
+                    }
+                }
+                uint64_t hb = __ballot(hit);
+                uint32_t accepted = nn;  // positions past the exit are not scored
+                while (hb) {
+                    const int h = __builtin_ctzll(hb);
+                    hb &= hb - 1;
+                    const Real hm = __shfl(tm, h), hc = __shfl(tc, h);
+                    occ_sum += static_cast<uint32_t>(__shfl(static_cast<int>(occ), h));
+                    sites += 1;
+                    double vm, vc;
+                    if constexpr (PREC == ISSL_PREC_FP32C) {
+                        fm = f2_add(fm, hm);
+                        fc = f2_add(fc, hc);
+                        vm = f2_value(fm);
+                        vc = f2_value(fc);
+                    } else {
+                        dm += hm;
+                        dc += hc;
+                        vm = dm;
+                        vc = dc;
+                    }
+                    if (a.sc.use_cut && bexit(a.sc.method, vm, vc, a.sc.cut)) {
+                        done = true;
+                        accepted = static_cast<uint32_t>(h) + 1;
+                        break;
+                    }
+                }
+                if (a.distinct && hit && static_cast<uint32_t>(lane) < accepted) {
+                    const uint32_t id = static_cast<uint32_t>(a.post[addr]);
+                    atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (id >> 6),
+                             1ULL << (id & 63));
+                }
+                head += nn;
+                __builtin_amdgcn_wave_barrier();
+            };
+
+            int cur = 0;
+            uint64_t nb = 0, dl = 0;
+            if (S > 0) {
+                while (cur < S - 1 && static_cast<uint64_t>(lane) >= s_pre[wave][cb][cur + 1]) ++cur;
+                nb = s_pre[wave][cb][cur + 1];
+                dl = s_delta[wave][cb][cur];
+            }
+            for (uint64_t base = 0; base < Pc && !done; base += 64 * DEPTH) {
+                uint64_t wv[DEPTH];
+                int sl[DEPTH];
+#pragma unroll
+                for (int d = 0; d < DEPTH; ++d) {
+                    const uint64_t pos = base + 64 * d + lane;
+                    sl[d] = -1;
+                    wv[d] = 0;
+                    if (pos < Pc) {
+                        if (pos >= nb) {
+                            do {
+                                ++cur;
+                                nb = s_pre[wave][cb][cur + 1];
+                            } while (pos >= nb);
+                            dl = s_delta[wave][cb][cur];
+                        }
+                        sl[d] = cur;
+                        wv[d] = a.site[pos + dl];
+                    }
+                }
+                if (!issued) {  // next guide's list bounds, behind this round's loads
+                    plan_issue(qn, bn, en);
+                    issued = true;
+                }
+#pragma unroll
+                for (int d = 0; d < DEPTH; ++d) {
+                    bool cand = false;
+                    if (sl[d] >= 0) cand = __popcll(bfold(q ^ (wv[d] & kSigBits))) <= 4;
+                    const uint64_t bb = __ballot(cand);
+                    if (bb) {
+                        const uint32_t r = __builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(bb >> 32),
+                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bb), 0u));
+                        if (cand) {
+                            const uint32_t e = (tail + r) % RING;
+                            s_rpos[wave][e] = static_cast<uint32_t>(base + 64 * d + lane);
+                            s_rsl[wave][e] = static_cast<uint8_t>(sl[d]);
+                        }
+                        tail += static_cast<uint32_t>(__popcll(bb));
+                    }
+                }
+                while (tail - head >= 64 && !done) drain(64);
+            }
+            while (tail != head && !done) drain(std::min<uint32_t>(64, tail - head));
+
+            if (lane == 0) {
+                const double smit = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
+                const double scfd = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
+                const double vm = 10000.0 / (100.0 + smit);
+                const double vc = 10000.0 / (100.0 + scfd);
+                const double th = a.sc.threshold;
+                bool reject;
+                switch (a.sc.method) {
+                case ISSL_METHOD_MIT_AND_CFD: reject = vm < th && vc < th; break;
+                case ISSL_METHOD_MIT_OR_CFD: reject = vm < th || vc < th; break;
+                case ISSL_METHOD_AVG_MIT_CFD: reject = (vm + vc) / 2 < th; break;
+                case ISSL_METHOD_MIT: reject = vm < th; break;
+                default: reject = vc < th; break;
+                }
+                const DevResults &o = a.out;
+                if (o.mit) o.mit[g] = a.sc.calc_mit ? vm : -1.0;
+                if (o.cfd) o.cfd[g] = a.sc.calc_cfd ? vc : -1.0;
+                if (o.mit_sum) o.mit_sum[g] = smit;
+                if (o.cfd_sum) o.cfd_sum[g] = scfd;
+                if (o.occ) o.occ[g] = occ_sum;
+                if (o.sites) o.sites[g] = sites;
+                if (o.pass) o.pass[g] = reject ? '0' : '1';
+            }
+        }
+        if (!has_next) break;
+        if (!issued) plan_issue(qn, bn, en);
+        Pc = plan_finish(gn, bn, en, cb ^ 1);
+        cb ^= 1;
+        g = gn;
+        qc = qn;
+        qn = qnn;
+    }
+}
+
 // ------------------------------------------------------------ launcher
 
 namespace {
@@ -723,12 +1021,25 @@ issl_tuning default_tuning() {
     t.block = kTileBlock;
     t.depth = 8;
     t.grid_per_cu = 4;
-    t.sched = kSchedSteal;
+    t.sched = kSchedStatic;
     t.steal = 8;
+    t.direct_depth = 8;
+    t.heavy_kpost = kHeavyNever;
     return t;
 }
 
 namespace {
+
+template <int PREC>
+int launch_direct_depth(const StreamArgs &da, int depth, uint32_t grid, hipStream_t st) {
+    switch (depth) {
+    case 4: hipLaunchKernelGGL((k_stream<PREC, 4>), dim3(grid), dim3(256), 0, st, da); break;
+    case 8: hipLaunchKernelGGL((k_stream<PREC, 8>), dim3(grid), dim3(256), 0, st, da); break;
+    case 16: hipLaunchKernelGGL((k_stream<PREC, 16>), dim3(grid), dim3(256), 0, st, da); break;
+    default: return fail(ISSL_ERR_ARG, "direct depth must be 4, 8 or 16");
+    }
+    return ISSL_OK;
+}
 
 int cu_count(int device) {
     static int cached[64] = {0};
@@ -758,6 +1069,11 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     BatchWs b;
     int rc = carve(ws, n, S, &b);
     if (rc != ISSL_OK) return rc;
+    const bool fp32 = precision == ISSL_PREC_FP32C;
+    const void *mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
+    const void *cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
+    const bool direct = tn.heavy_kpost != 0;
+    const bool tiles = !direct || tn.heavy_kpost != kHeavyNever;
 
     // Persistent grid: grid_per_cu workgroups per CU, no more waves than tiles can need.
     const uint64_t max_tiles = n * kMaxTiles;
@@ -766,21 +1082,52 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     const uint64_t waves_needed = std::max<uint64_t>(1, std::min<uint64_t>(max_tiles, grid * kTileWaves));
     grid = std::min<uint64_t>(grid, (waves_needed + kTileWaves - 1) / kTileWaves);
     const uint32_t n_waves = static_cast<uint32_t>(grid * kTileWaves);
+    const uint32_t gw = static_cast<uint32_t>((n + 3) / 4);
 
-    PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk), b};
-    const uint32_t gb = static_cast<uint32_t>((n + 255) / 256);
-    hipLaunchKernelGGL(k_plan, dim3(static_cast<uint32_t>((n + 3) / 4)), dim3(256), 0, st, pa);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, st, b, n, n_waves);
-    hipLaunchKernelGGL(k_fillmap, dim3(gb), dim3(256), 0, st, b, n);
+    if (direct) {
+        StreamArgs da;
+        da.guides = d_guides;
+        da.n = n;
+        da.site = ix->d_inline;
+        da.post = ix->d_post;
+        da.lstart = ix->d_lstart;
+        da.meta = ix->d_meta;
+        da.mit = mit;
+        da.cfd = cfd;
+        da.S = S;
+        da.n_sites = ix->n;
+        da.sc = sc;
+        da.out = out;
+        da.distinct = distinct;
+        da.heavy = tiles ? static_cast<uint64_t>(tn.heavy_kpost) * 1024 : 0;
+        da.chunk = static_cast<uint32_t>(tn.chunk);
+        da.ws = b;
+        timing_mark(ix, st);
+        // Persistent guide streams: grid_per_cu workgroups per CU, at most one wave per guide.
+        const uint64_t sgrid = std::max<uint64_t>(
+            1, std::min<uint64_t>(gw, static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu));
+        rc = fp32 ? launch_direct_depth<ISSL_PREC_FP32C>(da, tn.direct_depth, static_cast<uint32_t>(sgrid), st)
+                  : launch_direct_depth<ISSL_PREC_FP64>(da, tn.direct_depth, static_cast<uint32_t>(sgrid), st);
+        if (rc != ISSL_OK) return rc;
+        timing_mark(ix, st);
+        if (!tiles) {
+            ISSL_HIP(hipGetLastError());
+            return ISSL_OK;
+        }
+    } else {
+        PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk), b};
+        hipLaunchKernelGGL(k_plan, dim3(gw), dim3(256), 0, st, pa);
+    }
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, st, b, n,
+                       tn.sched == kSchedSteal ? n_waves : 0u);
 
-    const bool fp32 = precision == ISSL_PREC_FP32C;
     TileArgs ta;
     ta.guides = d_guides;
     ta.site = ix->d_inline;
     ta.post = ix->d_post;
     ta.meta = ix->d_meta;
-    ta.mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
-    ta.cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
+    ta.mit = mit;
+    ta.cfd = cfd;
     ta.S = S;
     ta.mark_distinct = (distinct && !sc.use_cut) ? 1 : 0;
     ta.calc_mit = sc.calc_mit;
@@ -789,30 +1136,30 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     ta.steal = tn.steal;
     ta.distinct = distinct;
     ta.ws = b;
-    timing_mark(ix, st);
+    if (!direct) timing_mark(ix, st);
     rc = fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st)
               : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st);
     if (rc != ISSL_OK) return rc;
-    timing_mark(ix, st);
+    if (!direct) timing_mark(ix, st);
 
     FinArgs fa;
     fa.guides = d_guides;
     fa.site = ix->d_inline;
     fa.post = ix->d_post;
     fa.meta = ix->d_meta;
-    fa.mit = ta.mit;
-    fa.cfd = ta.cfd;
+    fa.mit = mit;
+    fa.cfd = cfd;
     fa.S = S;
     fa.n = n;
     fa.sc = sc;
     fa.out = out;
     fa.distinct = (distinct && sc.use_cut) ? distinct : nullptr;
+    fa.skip_untiled = direct ? 1 : 0;
     fa.ws = b;
-    const uint32_t fb = static_cast<uint32_t>((n + 3) / 4);
     if (fp32)
-        hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP32C>), dim3(fb), dim3(256), 0, st, fa);
+        hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP32C>), dim3(gw), dim3(256), 0, st, fa);
     else
-        hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP64>), dim3(fb), dim3(256), 0, st, fa);
+        hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP64>), dim3(gw), dim3(256), 0, st, fa);
     ISSL_HIP(hipGetLastError());
     return ISSL_OK;
 }

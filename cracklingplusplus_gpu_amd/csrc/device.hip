@@ -202,15 +202,14 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
     const int S = a.n_slices;
 
     if (tid < S) {
-        const SliceMeta m = a.meta[tid];
-        uint32_t key = 0;
-        for (uint32_t j = 0; j < m.width; ++j)
-            key |= static_cast<uint32_t>((q >> (2 * m.pos[j])) & 3ULL) << (2 * j);
-        const uint32_t b = a.lstart[m.lbase + key];
-        const uint32_t e = a.lstart[m.lbase + key + 1];
+        const uint64_t mf = a.meta[tid].fold;
+        const uint32_t lb = a.meta[tid].lbase;
+        const uint32_t key = slice_key(q, mf);
+        const uint32_t b = a.lstart[lb + key];
+        const uint32_t e = a.lstart[lb + key + 1];
         s_beg[tid] = static_cast<uint64_t>(tid) * a.n_sites + b;
         s_pre[tid + 1] = e - b;
-        s_fold[tid] = m.fold;
+        s_fold[tid] = mf;
     }
     __syncthreads();
     if (tid == 0) {
@@ -403,11 +402,7 @@ __global__ void k_validate(const uint64_t *__restrict__ post, const uint64_t *__
         const uint64_t id = e & 0xFFFFFFFFULL;
         bool good = id < n;
         if (good) {
-            uint32_t key = 0;
-            const uint64_t sg = sig[id];
-            for (uint32_t j = 0; j < m.width; ++j)
-                key |= static_cast<uint32_t>((sg >> (2 * m.pos[j])) & 3ULL) << (2 * j);
-            good = key == lo;
+            good = slice_key(sig[id] & kSigBits, m.fold) == lo;
             if (atomicAdd(seen + id, 1u) != 0u) good = false;
         }
         if (!good) atomicAdd(bad, 1ULL);

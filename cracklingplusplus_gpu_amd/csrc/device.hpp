@@ -30,6 +30,22 @@ struct SliceMeta {
     uint8_t pos[16];
 };
 
+// Slice key of a signature: the 2-bit codes at the slice's positions,
+// ascending, concatenated low to high (ISSLScoringModule.cpp:235-239).  The
+// positions are read off the slice's fold mask (bit 2j for position j).
+__device__ __forceinline__ uint32_t slice_key(uint64_t sig, uint64_t fold) {
+    uint32_t key = 0;
+    for (int j = 0; fold; ++j) {
+        const int p = __builtin_ctzll(fold);
+        key |= static_cast<uint32_t>((sig >> p) & 3ULL) << (2 * j);
+        fold &= fold - 1;
+    }
+    return key;
+}
+
+// heavy_kpost value meaning "no guide goes to the tile kernel".
+constexpr int32_t kHeavyNever = ISSL_HEAVY_NEVER;
+
 // Site word of the signature-inline layout: signature in bits 0..39,
 // occurrences in bits 40..63; kOccEscape means "read occ from the posting".
 constexpr uint64_t kSigBits = (1ULL << 40) - 1;

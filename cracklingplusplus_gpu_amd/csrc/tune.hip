@@ -46,7 +46,7 @@ bool cache_lookup(const char *path, const std::string &key, issl_tuning *t) {
         std::string k;
         issl_tuning v{};
         if (ls >> k >> v.chunk >> v.block >> v.depth >> v.grid_per_cu >> v.sched >> v.steal >>
-                v.us_per_guide &&
+                v.direct_depth >> v.heavy_kpost >> v.us_per_guide &&
             k == key) {
             *t = v;
             found = true;  // the last entry for a key wins
@@ -69,7 +69,9 @@ extern "C" int issl_set_tuning(issl_index *ix, const issl_tuning *in) {
     if (in->chunk < 64 || in->chunk % 64 != 0 || in->chunk > (1 << 20) ||
         !(d == 1 || d == 2 || d == 4 || d == 8 || d == 16) || in->grid_per_cu < 1 ||
         in->grid_per_cu > 32 || in->sched < 0 || in->sched > 2 || in->steal < 0 ||
-        in->steal > 1024)
+        in->steal > 1024 ||
+        !(in->direct_depth == 4 || in->direct_depth == 8 || in->direct_depth == 16) ||
+        in->heavy_kpost < 0)
         return fail(ISSL_ERR_ARG, "tuning out of range");
     std::lock_guard<std::mutex> lk(ix->mu);
     ix->tuning = *in;
@@ -111,38 +113,61 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     hipEvent_t e0, e1;
     ISSL_HIP(hipEventCreate(&e0));
     ISSL_HIP(hipEventCreate(&e1));
-    const int chunks[] = {512, 1024, 2048, 4096, 8192};
-    const int depths[] = {4, 8, 16};
-    const int grids[] = {2, 4, 8};
-    const int scheds[][2] = {{2, 8}, {1, 0}, {0, 0}};
+    // Time one launch shape: a warm-up, then the best of three launches.
+    auto measure = [&](issl_tuning &t) -> int {
+        int r0 = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st);
+        if (r0 != ISSL_OK) return r0;
+        float bestms = 1e30f;
+        for (int r = 0; r < 3; ++r) {
+            ISSL_HIP(hipEventRecord(e0, st));
+            r0 = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st);
+            if (r0 != ISSL_OK) return r0;
+            ISSL_HIP(hipEventRecord(e1, st));
+            ISSL_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            ISSL_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < bestms) bestms = ms;
+        }
+        t.us_per_guide = 1000.0 * bestms / static_cast<double>(n);
+        return ISSL_OK;
+    };
     issl_tuning top{};
     top.us_per_guide = 1e300;
+    auto consider = [&](issl_tuning t) -> int {
+        const int r0 = measure(t);
+        if (r0 != ISSL_OK) return r0;
+        if (t.us_per_guide < top.us_per_guide) top = t;
+        return ISSL_OK;
+    };
+    // (1) every guide on the persistent tile kernel: tile shape and schedule
+    const int chunks[] = {1024, 2048, 4096};
+    const int depths[] = {4, 8};
+    const int grids[] = {4, 7, 8};
+    const int scheds[][2] = {{1, 0}, {2, 8}};
+    issl_tuning tiles_best{};
+    tiles_best.us_per_guide = 1e300;
     for (int c : chunks)
         for (int d : depths)
-            for (const auto &sch : scheds) {
-            if (64 * d > c) continue;
-            for (int gpc : grids) {
-                issl_tuning t{c, 256, d, gpc, sch[0], sch[1], 0};
-                // warm-up, then the best of three timed launches
-                if ((rc = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st)) !=
-                    ISSL_OK)
-                    return rc;
-                float bestms = 1e30f;
-                for (int r = 0; r < 3; ++r) {
-                    ISSL_HIP(hipEventRecord(e0, st));
-                    if ((rc = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws,
-                                             st)) != ISSL_OK)
-                        return rc;
-                    ISSL_HIP(hipEventRecord(e1, st));
-                    ISSL_HIP(hipEventSynchronize(e1));
-                    float ms = 0;
-                    ISSL_HIP(hipEventElapsedTime(&ms, e0, e1));
-                    if (ms < bestms) bestms = ms;
+            for (const auto &sch : scheds)
+                for (int gpc : grids) {
+                    issl_tuning t{c, 256, d, gpc, sch[0], sch[1], 8, 0, 0};
+                    if ((rc = measure(t)) != ISSL_OK) return rc;
+                    if (t.us_per_guide < tiles_best.us_per_guide) tiles_best = t;
                 }
-                t.us_per_guide = 1000.0 * bestms / static_cast<double>(n);
-                if (t.us_per_guide < top.us_per_guide) top = t;
+    if (tiles_best.us_per_guide < top.us_per_guide) top = tiles_best;
+    // (2) one wave per guide, optionally sending long guides to the tile kernel
+    const int ddepths[] = {4, 8, 16};
+    const int heavies[] = {kHeavyNever, 64, 16, 4};
+    const int sgrids[] = {2, 4, 7};
+    for (int h : heavies)
+        for (int dd : ddepths)
+            for (int gpc : sgrids) {
+                issl_tuning t = tiles_best;
+                t.direct_depth = dd;
+                t.heavy_kpost = h;
+                t.grid_per_cu = gpc;
+                if ((rc = consider(t)) != ISSL_OK) return rc;
             }
-        }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ix->tuning = top;
@@ -151,8 +176,9 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%s %d %d %d %d %d %d %.6g\n", key.c_str(), top.chunk, top.block, top.depth,
-                    top.grid_per_cu, top.sched, top.steal, top.us_per_guide);
+            fprintf(f, "%s %d %d %d %d %d %d %d %d %.6g\n", key.c_str(), top.chunk, top.block,
+                    top.depth, top.grid_per_cu, top.sched, top.steal, top.direct_depth,
+                    top.heavy_kpost, top.us_per_guide);
             fclose(f);
         }
     }

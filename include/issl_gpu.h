@@ -175,8 +175,16 @@ typedef struct {
     int32_t sched;        /* tile schedule: 0 one global queue, 1 static
                              round-robin, 2 per-wave ranges + stealing */
     int32_t steal;        /* ranges of other waves tried after the own one */
+    int32_t direct_depth; /* postings in flight per lane in the one-wave-per-
+                             guide kernel: 4, 8 or 16 */
+    int32_t heavy_kpost;  /* guides with more than heavy_kpost*1024 postings go
+                             to the persistent tile kernel, the rest are scored
+                             one wave per guide; 0 = all guides use tiles,
+                             ISSL_HEAVY_NEVER = none do */
     double  us_per_guide; /* measured */
 } issl_tuning;
+
+#define ISSL_HEAVY_NEVER 0x7FFFFFFF
 
 /* Time the BATCHED kernel's launch shapes on a sample of guides for this
  * index and keep the fastest (also written to the cache file when path is

@@ -113,6 +113,47 @@ def test_batched_exit_point_matches_reference(cases, name):
         np.testing.assert_allclose(r.cfd_sum[same], o["cfd_sum"][same], rtol=1e-12)
 
 
+BATCHED_MODES = {
+    # every guide through the persistent tile kernel
+    "tiles": dict(chunk=1024, depth=4, grid_per_cu=4, sched=2, steal=8, heavy_kpost=0),
+    # one wave per guide only
+    "direct": dict(direct_depth=8, heavy_kpost=issl._capi.HEAVY_NEVER),
+    # guides with more than 1,024 postings go to the tile kernel
+    "hybrid": dict(chunk=512, depth=4, grid_per_cu=2, sched=0, direct_depth=4, heavy_kpost=1),
+    "static": dict(chunk=512, depth=8, grid_per_cu=8, sched=1, heavy_kpost=0),
+}
+
+
+@pytest.mark.parametrize("mode", sorted(BATCHED_MODES))
+@pytest.mark.parametrize("name", ["tiny4k_5x4", "small200k_8x8", "uniform1m_5x4"])
+def test_batched_modes(cases, name, mode):
+    """Each scheduling mode of the BATCHED form against the reference's goldens."""
+    hx, ix, gd = cases[name]
+    g = gd["guides"]
+    ix.set_tuning(**BATCHED_MODES[mode])
+    try:
+        for m in (issl.METHOD_MIT_AND_CFD, issl.METHOD_AVG_MIT_CFD):
+            for t in THRESHOLDS:
+                r = ix.score(g, method=m, threshold=float(t), form=issl.FORM_BATCHED,
+                             precision=issl.PREC_FP64, flags=issl.FLAG_DISTINCT)
+                key = f"m{m}_t{t}"
+                if mode == "direct":
+                    # one wave adds a guide's hits in the reference's order
+                    assert (bits(r.mit) == bits(gd[key + "_mit"])).all(), key
+                    assert (bits(r.cfd) == bits(gd[key + "_cfd"])).all(), key
+                else:
+                    np.testing.assert_allclose(r.mit, gd[key + "_mit"], rtol=1e-12)
+                    np.testing.assert_allclose(r.cfd, gd[key + "_cfd"], rtol=1e-12)
+                assert (r.passed == gd[key + "_pass"]).all(), key
+            o = orc.score(hx, g, m, 0.0)
+            r = ix.score(g, method=m, form=issl.FORM_BATCHED, precision=issl.PREC_FP32C)
+            assert (r.site_count == o["site_count"]).all()
+            assert (r.occ_count == o["occ_count"]).all()
+            np.testing.assert_allclose(r.cfd, o["cfd"], rtol=1e-6)
+    finally:
+        ix.set_tuning()
+
+
 @pytest.mark.parametrize("name", ["legacy39_5x4", "tiny4k_5x4", "mid50k_10x10"])
 def test_candidate_sets_match_brute_force(cases, name):
     hx, ix, gd = cases[name]
