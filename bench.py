@@ -8,7 +8,7 @@ MIT+CFD (method mitAndCfd), score threshold disabled, maxDist 4.
 
 One step = scoring the 10,000 guides once, with the index and the guides
 already resident in HBM (issl_score_device).  `value` is guides/s of the
-BATCHED form (persistent work-stealing kernel, compensated FP32 sums); the
+BATCHED form (persistent tile kernel, compensated FP32 sums); the
 PLAIN form (one guide per workgroup, FP64) and the CPU scorer (the oracle's
 OpenMP restatement of ISSLScoringModule::run on the host cores, same
 workload) are reported beside it.
@@ -68,6 +68,36 @@ def postings_per_guide(ix, guides: np.ndarray) -> np.ndarray:
     return total
 
 
+def copy_calibration(hiprt) -> int:
+    """A copy of known size: 1 GiB read and 1 GiB written per dispatch."""
+    nbytes = 1 << 30
+    src = hiprt.DeviceArray(nbytes, np.uint8)
+    dst = hiprt.DeviceArray(nbytes, np.uint8)
+    for r in range(3):
+        t0 = time.perf_counter()
+        hiprt.copy_device(dst, src, nbytes)
+        hiprt.synchronize()
+        log(f"copy {r}: 1 GiB device-to-device in {(time.perf_counter() - t0) * 1e3:.3f} ms")
+    return 0
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per k_tiles launch from the committed counter summary.
+
+    The summary holds FETCH_SIZE + WRITE_SIZE per dispatch of k_tiles on the
+    named workload, each scaled by the reading of a 1 GiB copy.  Returns None
+    when no summary covers this workload.
+    """
+    path = os.path.join(ROOT, "profiles", f"k_tiles_{workload}_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1].strip().replace(",", ""))
+    except OSError:
+        pass
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,11 +110,14 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--latency-requests", type=int, default=200)
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "gpurun_out", "issl_tune.txt"))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device three times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the scorer is single-device; see DESIGN.md for how it would shard).",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the scorer "
+              "runs on one device and this benchmark reports that device's throughput.",
               file=sys.stderr)
         return 2
 
@@ -94,6 +127,8 @@ def main() -> int:
     if issl.device_count() <= 0:
         print("no HIP device", file=sys.stderr)
         return 1
+    if args.copy_calibration:
+        return copy_calibration(hiprt)
     threads = host_threads()
     t_setup = time.time()
     log(f"generating workload {args.workload} with {threads} threads")
@@ -119,7 +154,7 @@ def main() -> int:
     os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
     tn = ix.autotune(guides, args.tune_cache)
     log(f"tuning: chunk {tn.chunk} depth {tn.depth} grid/CU {tn.grid_per_cu} "
-        f"sched {tn.sched} steal {tn.steal} stream depth {tn.direct_depth} "
+        f"stream depth {tn.direct_depth} "
         f"heavy {tn.heavy_kpost} ({tn.us_per_guide * 1e3:.2f} ns/guide)")
     setup_s = time.time() - t_setup
 
@@ -231,12 +266,12 @@ def main() -> int:
                    "max_dist": 4, "form": "batched", "precision": "fp32c",
                    "postings_per_step": p_total,
                    "tuning": {"chunk": tn.chunk, "depth": tn.depth,
-                              "grid_per_cu": tn.grid_per_cu, "sched": tn.sched,
-                              "steal": tn.steal, "stream_depth": tn.direct_depth,
+                              "grid_per_cu": tn.grid_per_cu,
+                              "stream_depth": tn.direct_depth,
                               "heavy_kpost": tn.heavy_kpost}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_tiles",
+                     "traffic": measured_traffic(args.workload), "kernel": "k_tiles",
                      "kernel_us": round(k_avg_ms * 1e3, 2),
                      "bytes_per_launch": bytes_inline,
                      "survey_bytes_per_launch": bytes_survey},

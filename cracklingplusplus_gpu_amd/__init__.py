@@ -327,11 +327,9 @@ class Index:
         p = self.params(method, threshold, form, precision, flags, chunk)
         check(lib().issl_score_device(self._h, d_guides.ptr, n, byref(p), byref(res), stream))
 
-    def set_tuning(self, chunk: int = 2048, depth: int = 8, grid_per_cu: int = 4,
-                   sched: int = 1, steal: int = 8, direct_depth: int = 8,
-                   heavy_kpost: int = _capi.HEAVY_NEVER) -> None:
-        """heavy_kpost: guides above heavy_kpost*1024 postings use the tile kernel
-        (0: every guide does; HEAVY_NEVER: none)."""
-        t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, sched, steal, direct_depth,
-                              heavy_kpost, 0.0)
+    def set_tuning(self, chunk: int = 1024, depth: int = 8, grid_per_cu: int = 6,
+                   direct_depth: int = 8, heavy_kpost: int = 0) -> None:
+        """heavy_kpost: guides above heavy_kpost*1024 postings use the tile kernel,
+        the rest one wave per guide (0: every guide uses tiles; HEAVY_NEVER: none)."""
+        t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, direct_depth, heavy_kpost, 0.0)
         check(lib().issl_set_tuning(self._h, byref(t)))

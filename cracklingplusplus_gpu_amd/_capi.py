@@ -50,8 +50,8 @@ class issl_index_info(Structure):
 
 class issl_tuning(Structure):
     _fields_ = [("chunk", c_int32), ("block", c_int32), ("depth", c_int32),
-                ("grid_per_cu", c_int32), ("sched", c_int32), ("steal", c_int32),
-                ("direct_depth", c_int32), ("heavy_kpost", c_int32),
+                ("grid_per_cu", c_int32), ("direct_depth", c_int32),
+                ("heavy_kpost", c_int32),
                 ("us_per_guide", c_double)]
 
 

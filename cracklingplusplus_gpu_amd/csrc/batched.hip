@@ -1,16 +1,19 @@
-// batched.hip -- BATCHED scoring form: a persistent, work-stealing kernel over
-// (guide, chunk) tiles of each guide's slice-major candidate sequence.
+// batched.hip -- BATCHED scoring form: persistent waves over (guide, chunk)
+// tiles of each guide's slice-major candidate sequence.
 //
-// Pipeline (all on one stream, capturable in a hipGraph; no memsets):
-//   k_plan     one thread per guide: slice keys -> list ranges, prefix P_g,
-//              tile length C_g = max(chunk, ceil(P_g/kMaxTiles)) and tiles K_g
-//   k_scan     one workgroup: exclusive scan of K_g -> tile_base, total T,
-//              queue counter reset
-//   k_fillmap  tile -> (guide, k) map
-//   k_tiles    persistent waves pop tiles from the atomic queue; each streams
-//              its tile's site words (sig | occ<<40, no signature gather),
-//              folds, filters, dedups by the slice-order rule and sums its
-//              hits; one partial per tile, reduced in a fixed lane order
+// Pipeline (all on one stream, no memsets):
+//   k_plan     one wave per guide, one lane per slice: slice keys -> list
+//              ranges, their prefix, the posting offsets, the tile length
+//              C_g = max(chunk, ceil(P_g/64)) and the tile count K_g
+//   k_tiles    persistent waves.  Tile (g, k) is virtual slot v = k*n + g;
+//              wave w takes slots w, w+W, w+2W, ... and keeps those with
+//              k < K_g, checking 64 slots per load.  For each tile it streams
+//              the site words (sig | occ<<40: no signature gather), folds and
+//              filters them (dist <= 4), queues the candidates with their
+//              words in an LDS ring, scores them 64 at a time under the
+//              slice-order first-seen rule and writes one partial per tile,
+//              reduced in a fixed lane order.  The next tile's plan is loaded
+//              while the current tile streams.
 //   k_finalize one wave per guide: adds the tile partials in tile order and,
 //              when the early exit is armed, rescans the tile in which the
 //              running total first meets the exit test, hit by hit, to stop
@@ -18,6 +21,11 @@
 //              (lib/ISSLScoringModule.cpp:365-401)
 // Results therefore depend only on the guide and the tile length, never on
 // which wave ran which tile or on what else was in the batch.
+//
+// Limits: a launch takes at most kMaxBatch guides (larger calls run as
+// several launches) and P_g < 2^31 (checked against the index's longest
+// lists), so positions, tile-relative positions (< 2^26) and slots fit
+// 32 bits.
 //
 // Precision: ISSL_PREC_FP32C forms each site's MIT/CFD product in FP32 and
 // carries every sum as an unevaluated pair (hi, lo) of FP32 built with
@@ -51,30 +59,17 @@ struct TilePart {
     uint64_t sites;
 };
 
-// Queue counters sit 128 bytes apart so concurrent atomics on different
-// counters never share a line.
-constexpr uint32_t kCounterStride = 32;
-constexpr uint32_t kMaxWaves = 256 * 32 * kTileWaves;  // CUs x grid/CU x waves
+// Guides per launch: slots n * kMaxTiles stay below 2^32 and the partials
+// (kMaxTiles * n * 32 B) stay modest.
+constexpr uint64_t kMaxBatch = 1ULL << 18;
 
 struct BatchWs {
-    uint64_t *pre;       // [n][S+1]
+    uint32_t *pre;       // [n][S+1] prefix of the list lengths; [S] = P_g
     uint64_t *delta;     // [n][S]   posting index = position + delta
-    uint32_t *gchunk;    // [n]
-    uint32_t *gtiles;    // [n]
-    uint32_t *tile_base; // [n+1]
-    uint32_t *ctrl;      // [0] global queue counter, [1] total tiles
-    uint32_t *cnt;       // [waves * kCounterStride] per-wave range counters
-    uint64_t *tile_map;  // [n*kMaxTiles] guide<<32 | k
-    TilePart *part;      // [n*kMaxTiles]
+    uint32_t *gchunk;    // [n] tile length C_g
+    uint32_t *gtiles;    // [n] tile count K_g
+    TilePart *part;      // [kMaxTiles][n] partial of tile (g, k) at slot k*n + g
 };
-
-// Tile scheduling of k_tiles.
-//   kSchedQueue : every wave pops the next tile from one global counter
-//   kSchedStatic: wave w takes tiles w, w+W, w+2W, ... (no atomics)
-//   kSchedSteal : wave w owns tiles [wT/W, (w+1)T/W) behind its own counter,
-//                 then steals from up to `steal` other waves' ranges; each
-//                 owner drains its own range, so every tile is processed
-enum { kSchedQueue = 0, kSchedStatic = 1, kSchedSteal = 2 };
 
 // Candidate ring entries: position within the tile (tiles are shorter than
 // 2^26 postings for any guide with P < 2^32) and the slice index (< 64).
@@ -114,33 +109,87 @@ __device__ __forceinline__ uint64_t bfold(uint64_t x) {
     return ((x >> 1) & kFoldLo) | (x & kFoldLo);
 }
 
-__device__ __forceinline__ uint32_t brank(uint64_t f, int k) {
-    uint32_t r = (k == 1) ? 0u : (k == 2) ? 20u : (k == 3) ? 210u : 1350u;
-    for (int i = 1; i <= k; ++i) {
-        const uint32_t p = static_cast<uint32_t>(__builtin_ctzll(f)) >> 1;
+// Occurrence count as a scoring operand (a single 32-bit conversion).
+template <typename T>
+__device__ __forceinline__ T to_real(uint32_t v);
+template <>
+__device__ __forceinline__ float to_real<float>(uint32_t v) {
+    return __uint2float_rn(v);
+}
+template <>
+__device__ __forceinline__ double to_real<double>(uint32_t v) {
+    return __uint2double_rn(v);
+}
+
+// Sentinel position for absent mismatches: its slice mask is 0 and its CFD
+// factor is 1, so table reads need no predicate.
+constexpr uint32_t kNoPos = 32;
+constexpr uint32_t kCfdOne = 336;   // CFD table entry holding 1.0
+constexpr int kCfdEntries = kCfdTable;
+
+// Mismatch positions of a fold word (bit 2p set for each mismatching position
+// p; at most four), ascending; kNoPos past the count.
+__device__ __forceinline__ void mismatches(uint64_t f, uint32_t p[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        p[i] = f ? static_cast<uint32_t>(__builtin_ctzll(f)) >> 1 : kNoPos;
         f &= f - 1;
-        const uint32_t c = (i == 1)   ? p
-                           : (i == 2) ? p * (p - 1) / 2
-                           : (i == 3) ? p * (p - 1) * (p - 2) / 6
-                                      : p * (p - 1) * (p - 2) * (p - 3) / 24;
-        r += c;
     }
+}
+
+// Rank of a k-subset (k = 1..4, positions ascending) in the MIT table: all
+// smaller subsets first (sum of C(20, j) for 0 < j < k), then combinadic
+// order (sum of C(p_i, i+1)).
+__device__ __forceinline__ uint32_t mit_rank(const uint32_t p[4], int k) {
+    const uint32_t c2 = p[1] * (p[1] - 1) / 2;
+    const uint32_t c3 = p[2] * (p[2] - 1) * (p[2] - 2) / 6;
+    const uint32_t c4 = p[3] * (p[3] - 1) * (p[3] - 2) * (p[3] - 3) / 24;
+    uint32_t r = p[0];
+    r += k >= 2 ? 20u + c2 : 0u;
+    r += k >= 3 ? 190u + c3 : 0u;
+    r += k >= 4 ? 1140u + c4 : 0u;
     return r;
 }
 
+// CFD product of a site at distance k: 1 for an exact match, else the NGG PAM
+// factor times one factor per mismatch, multiplied in position order
+// (lib/ISSLScoringModule.cpp:301-363).  cfd holds kCfdEntries values, the
+// last one 1.0 for absent mismatches (an exact multiplication).
 template <typename T>
-__device__ __forceinline__ T cfd_product(uint64_t q, uint64_t site, uint64_t f, int dist,
+__device__ __forceinline__ T cfd_product(uint64_t q, uint64_t site, const uint32_t p[4], int k,
                                          const T *cfd) {
-    if (dist == 0) return T(1);
-    T c = cfd[320 + 10];
-    while (f) {
-        const uint32_t p = static_cast<uint32_t>(__builtin_ctzll(f)) >> 1;
-        f &= f - 1;
-        const uint32_t g = static_cast<uint32_t>(q >> (2 * p)) & 3u;
-        const uint32_t o = static_cast<uint32_t>(site >> (2 * p)) & 3u;
-        c *= cfd[(p << 4) | (g << 2) | (o ^ 3u)];
+    T t[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t g = static_cast<uint32_t>(q >> (2 * p[i])) & 3u;
+        const uint32_t o = static_cast<uint32_t>(site >> (2 * p[i])) & 3u;
+        t[i] = cfd[std::min((p[i] << 4) | (g << 2) | (o ^ 3u), kCfdOne)];
     }
-    return c;
+    const T c = cfd[320 + 10] * t[0] * t[1] * t[2] * t[3];
+    return k == 0 ? T(1) : c;
+}
+
+// Slice-order first-seen rule (lib/ISSLScoringModule.cpp:252-258): a site
+// found in slice sl is scored there only if no earlier slice holds it, i.e.
+// every earlier slice contains one of its mismatch positions.  posmask[j] is
+// the set of slices containing position j (posmask[kNoPos] = 0).
+__device__ __forceinline__ bool first_seen(const uint32_t p[4], int sl, const uint64_t *posmask) {
+    const uint64_t nm = posmask[p[0]] | posmask[p[1]] | posmask[p[2]] | posmask[p[3]];
+    const uint64_t earlier = sl ? (~0ULL >> (64 - sl)) : 0ULL;
+    return (earlier & ~nm) == 0;
+}
+
+// posmask (kNoPos + 1 entries) for the first-seen rule, built by threads
+// 0..kNoPos of a workgroup.
+__device__ __forceinline__ void build_posmask(uint64_t *posmask, const uint64_t *fold, int S,
+                                              int tid) {
+    if (tid <= static_cast<int>(kNoPos)) {
+        uint64_t m = 0;
+        if (tid < static_cast<int>(kNoPos))
+            for (int s = 0; s < S; ++s)
+                if ((fold[s] >> (2 * tid)) & 1ULL) m |= 1ULL << s;
+        posmask[tid] = m;
+    }
 }
 
 __device__ __forceinline__ bool bexit(int method, double m, double c, double cut) {
@@ -195,65 +244,17 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a) {
     const uint64_t excl = incl - len;
     const uint64_t run = a.S > 0 ? shfl64(incl, a.S - 1) : 0;
     if (lane < a.S) {
-        a.ws.pre[g * (a.S + 1) + lane] = excl;
+        a.ws.pre[g * (a.S + 1) + lane] = static_cast<uint32_t>(excl);
         // posting index of position p in slice s: s*N + b + (p - pre[s])
         a.ws.delta[g * a.S + lane] = static_cast<uint64_t>(lane) * a.n_sites + b - excl;
     }
     if (lane == 0) {
-        a.ws.pre[g * (a.S + 1) + a.S] = run;
+        a.ws.pre[g * (a.S + 1) + a.S] = static_cast<uint32_t>(run);
         uint64_t c = (run + kMaxTiles - 1) / kMaxTiles;
         c = (c + 63) / 64 * 64;
         if (c < a.chunk) c = a.chunk;
         a.ws.gchunk[g] = static_cast<uint32_t>(c);
         a.ws.gtiles[g] = static_cast<uint32_t>((run + c - 1) / c);
-    }
-}
-
-constexpr int kScanBlock = 1024;
-
-// One workgroup: exclusive scan of the per-guide tile counts (tile_base,
-// total), the tile -> (guide, k) map, and reset of the per-wave range
-// counters (n_counters > 0 only when k_tiles steals).  Each thread owns a
-// contiguous run of guides; the run totals are scanned with wave shuffles
-// and one LDS pass over the 16 wave totals.
-__global__ __launch_bounds__(kScanBlock) void k_scan(BatchWs ws, uint64_t n, uint32_t n_counters) {
-    __shared__ uint32_t s_wave[kScanBlock / 64];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    for (uint32_t i = t; i < n_counters; i += kScanBlock) ws.cnt[i * kCounterStride] = 0;
-    const uint64_t per = (n + kScanBlock - 1) / kScanBlock;
-    const uint64_t lo = std::min<uint64_t>(n, per * t), hi = std::min<uint64_t>(n, lo + per);
-    uint32_t sum = 0;
-    for (uint64_t i = lo; i < hi; ++i) sum += ws.gtiles[i];
-    uint32_t incl = sum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(incl), lane >= off ? lane - off : lane));
-        if (lane >= off) incl += v;
-    }
-    if (lane == 63) s_wave[wv] = incl;
-    __syncthreads();
-    if (wv == 0) {
-        uint32_t x = lane < kScanBlock / 64 ? s_wave[lane] : 0u;
-#pragma unroll
-        for (int off = 1; off < kScanBlock / 64; off <<= 1) {
-            const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(x), lane >= off ? lane - off : lane));
-            if (lane >= off) x += v;
-        }
-        if (lane < kScanBlock / 64) s_wave[lane] = x;  // inclusive wave totals
-    }
-    __syncthreads();
-    uint32_t run = (wv > 0 ? s_wave[wv - 1] : 0u) + incl - sum;
-    for (uint64_t g = lo; g < hi; ++g) {
-        const uint32_t k = ws.gtiles[g];
-        ws.tile_base[g] = run;
-        for (uint32_t i = 0; i < k; ++i) ws.tile_map[run + i] = (g << 32) | i;
-        run += k;
-    }
-    if (t == kScanBlock - 1) {
-        const uint32_t total = s_wave[kScanBlock / 64 - 1];
-        ws.tile_base[n] = total;
-        ws.ctrl[0] = 0;      // global queue counter
-        ws.ctrl[1] = total;  // total tiles
     }
 }
 
@@ -266,108 +267,137 @@ struct TileArgs {
     const SliceMeta *meta;
     const void *mit;        // float or double by precision
     const void *cfd;
+    uint32_t n;             // guides; n * kMaxTiles < 2^32
     int S;
     int mark_distinct;      // mark distinct sites here (no early exit armed)
     int calc_mit, calc_cfd;
-    int sched;              // kSched*
-    int steal;              // victims tried after the own range (kSchedSteal)
     uint64_t *distinct;
     BatchWs ws;
 };
 
-// Postings in flight per lane.
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+// Candidate ring per wave: fewer than 64 pending plus kRingChk loads of 64.
+constexpr int kRingChk = 3;
+constexpr uint32_t kRing = 64 * (kRingChk + 1);
+
+// Postings in flight per lane: DEPTH.
 template <int PREC, int DEPTH>
 __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
-    __shared__ Real s_cfd[336];
+    __shared__ Real s_cfd[kCfdEntries];
     __shared__ uint64_t s_fold[kMaxSlices];
-    __shared__ uint64_t s_pre[kTileWaves][kMaxSlices + 1];
-    __shared__ uint64_t s_delta[kTileWaves][kMaxSlices];
-    // Candidate ring: holds < 64 pending plus one round's 64 * DEPTH.
-    constexpr uint32_t kRingD = 64 * (DEPTH + 1);
-    __shared__ uint32_t s_ring[kTileWaves][kRingD];   // tile-relative position | slice << 26
+    __shared__ uint64_t s_posmask[kNoPos + 1];
+    // Tile plans, double-buffered: the next tile's plan lands in the other half.
+    __shared__ uint32_t s_pre[kTileWaves][2][kMaxSlices + 1];
+    __shared__ uint64_t s_delta[kTileWaves][2][kMaxSlices];
+    // Candidate ring: tile-relative position | slice << kRelBits, and the site word.
+    __shared__ uint32_t s_rpos[kTileWaves][kRing];
+    __shared__ uint64_t s_rword[kTileWaves][kRing];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Real *cfdg = static_cast<const Real *>(a.cfd);
     const Real *mitg = static_cast<const Real *>(a.mit);
-    for (int i = tid; i < 336; i += kTileBlock) s_cfd[i] = cfdg[i];
+    for (int i = tid; i < kCfdEntries; i += kTileBlock) s_cfd[i] = cfdg[i];
     if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
     __syncthreads();
+    build_posmask(s_posmask, s_fold, a.S, tid);
+    __syncthreads();
     const int S = a.S;
-    const uint32_t total = a.ws.ctrl[1];
+    const uint32_t n = a.n;
+    const uint32_t V = n * kMaxTiles;
     const uint32_t W = gridDim.x * kTileWaves;
     const uint32_t w = blockIdx.x * kTileWaves + wave;
-    uint32_t iter = 0;   // kSchedStatic
-    int tries = 0;       // kSchedSteal: ranges visited after the own one
-    uint32_t v = w;      // kSchedSteal: range being drained
+
+    // Slots of this wave: w + j*W.  64 of them are checked per load; the real
+    // ones (k < K_g) are taken in increasing order.
+    uint32_t j0 = 0, vl = 0;
+    uint64_t smask = 0;
+    auto next_slot = [&]() -> uint32_t {
+        while (smask == 0) {
+            if (static_cast<uint64_t>(j0) * W + w >= V) return kNoSlot;
+            const uint64_t vv = static_cast<uint64_t>(j0 + lane) * W + w;
+            bool real = false;
+            if (vv < V) {
+                vl = static_cast<uint32_t>(vv);
+                real = vl / n < a.ws.gtiles[vl % n];
+            }
+            smask = __ballot(real);
+            j0 += 64;
+        }
+        const int l = __builtin_ctzll(smask);
+        smask &= smask - 1;
+        return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vl), l));
+    };
+
+    // A tile's plan: guide, tile index, length, and per slice (lane) the
+    // prefix and the posting offset.
+    struct Plan {
+        uint64_t q;
+        uint32_t g, k, P, C;
+        uint32_t pre;
+        uint64_t dl;
+    };
+    auto plan_load = [&](uint32_t v, Plan &m) {
+        m.g = v % n;
+        m.k = v / n;
+        const size_t row = static_cast<size_t>(m.g) * (S + 1);
+        m.q = a.guides[m.g] & kSigBits;
+        m.P = a.ws.pre[row + S];
+        m.C = a.ws.gchunk[m.g];
+        m.pre = 0;
+        m.dl = 0;
+        if (lane < S) {
+            m.pre = a.ws.pre[row + lane];
+            m.dl = a.ws.delta[static_cast<size_t>(m.g) * S + lane];
+        }
+    };
+    auto plan_commit = [&](const Plan &m, int b) {
+        if (lane < S) {
+            s_pre[wave][b][lane] = m.pre;
+            s_delta[wave][b][lane] = m.dl;
+        }
+        if (lane == 0) s_pre[wave][b][S] = m.P;  // S may be 64: no lane S
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    uint32_t vc = next_slot();
+    if (vc == kNoSlot) return;  // uniform per wave; no workgroup barrier follows
+    Plan mc, mn;
+    plan_load(vc, mc);
+    plan_commit(mc, 0);
+    uint32_t vn = next_slot();
+    if (vn != kNoSlot) plan_load(vn, mn);
+    int buf = 0;
 
     for (;;) {
-        uint32_t t = total;
-        if (a.sched == kSchedStatic) {
-            const uint64_t tt = w + static_cast<uint64_t>(iter++) * W;
-            t = tt < total ? static_cast<uint32_t>(tt) : total;
-        } else if (a.sched == kSchedSteal) {
-            for (;;) {
-                const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(v) * total / W);
-                const uint32_t hi = static_cast<uint32_t>((static_cast<uint64_t>(v) + 1) * total / W);
-                uint32_t k = 0xFFFFFFFFu;
-                if (lane == 0 && lo < hi) k = atomicAdd(a.ws.cnt + static_cast<size_t>(v) * kCounterStride, 1u);
-                k = static_cast<uint32_t>(__shfl(static_cast<int>(k), 0));
-                if (lo < hi && k < hi - lo) {
-                    t = lo + k;
-                    break;
-                }
-                if (tries >= a.steal) break;
-                ++tries;
-                v = static_cast<uint32_t>((w + static_cast<uint64_t>(tries) * 61) % W);
-            }
-        } else {
-            uint32_t k = 0;
-            if (lane == 0) k = atomicAdd(a.ws.ctrl, 1u);
-            t = static_cast<uint32_t>(__shfl(static_cast<int>(k), 0));
-        }
-        if (t >= total) break;
-        const uint64_t gm = a.ws.tile_map[t];
-        const uint64_t g = gm >> 32;
-        const uint32_t k = static_cast<uint32_t>(gm);
-        const uint64_t q = a.guides[g] & kSigBits;
-        const uint64_t C = a.ws.gchunk[g];
-        if (lane < S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
-        if (lane == 0) s_pre[wave][S] = a.ws.pre[g * (S + 1) + S];  // S may be 64
-        if (lane < S) s_delta[wave][lane] = a.ws.delta[g * S + lane];
-        __builtin_amdgcn_wave_barrier();
-        const uint64_t P = s_pre[wave][S];
-        const uint64_t a0 = k * C;
-        const uint64_t a1 = std::min<uint64_t>(P, a0 + C);
+        const uint64_t q = mc.q;
+        const uint32_t a0 = mc.k * mc.C;
+        const uint32_t a1 = std::min(mc.P, a0 + mc.C);
 
         Real am = 0, ac = 0;
         F2 fm{0.f, 0.f}, fc{0.f, 0.f};
         uint64_t occ_sum = 0, sites = 0;
 
-        // Candidates (dist <= 4) are queued in a per-wave LDS ring in position
-        // order and scored 64 at a time, one per lane, so the rare scoring
-        // path runs with every lane busy instead of once per round.  An entry
-        // is the tile-relative position and the slice; scoring reloads the
-        // site word (a cache hit).
+        // Candidates (dist <= 4) wait in the ring in position order and are
+        // scored 64 at a time, one per lane, so the rare scoring path runs
+        // with every lane busy.
         uint32_t head = 0, tail = 0;  // wave-uniform ring cursors
-        auto drain = [&](uint32_t n) {
+        auto drain = [&](uint32_t cnt) {
             __builtin_amdgcn_wave_barrier();
-            if (static_cast<uint32_t>(lane) < n) {
-                const uint32_t ent = s_ring[wave][(head + lane) % kRingD];
+            if (static_cast<uint32_t>(lane) < cnt) {
+                const uint32_t e = (head + lane) & (kRing - 1);
+                const uint32_t ent = s_rpos[wave][e];
+                const uint64_t wv = s_rword[wave][e];
                 const int sl = static_cast<int>(ent >> kRelBits);
-                const uint64_t addr = a0 + (ent & ((1u << kRelBits) - 1)) + s_delta[wave][sl];
-                const uint64_t wv = a.site[addr];
                 const uint64_t sg = wv & kSigBits;
                 const uint64_t f = bfold(q ^ sg);
                 const int dist = __popcll(f);
-                bool first = true;
-                for (int s2 = 0; s2 < sl; ++s2)
-                    if ((f & s_fold[s2]) == 0) {
-                        first = false;
-                        break;
-                    }
-                if (first) {
+                uint32_t mp[4];
+                mismatches(f, mp);
+                if (first_seen(mp, sl, s_posmask)) {
                     uint32_t occ = static_cast<uint32_t>(wv >> 40);
                     if (occ == kOccEscape || a.mark_distinct) {
+                        const uint64_t addr =
+                            a0 + (ent & ((1u << kRelBits) - 1)) + s_delta[wave][buf][sl];
                         const uint64_t pe = a.post[addr];
                         occ = static_cast<uint32_t>(pe >> 32);
                         if (a.mark_distinct)
@@ -377,10 +407,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     }
                     occ_sum += occ;
                     sites += 1;
-                    const Real ro = static_cast<Real>(occ);
+                    const Real ro = to_real<Real>(occ);
                     Real tm = 0, tc = 0;
-                    if (a.calc_mit && dist > 0) tm = mitg[brank(f, dist)] * ro;
-                    if (a.calc_cfd) tc = cfd_product<Real>(q, sg, f, dist, s_cfd) * ro;
+                    if (a.calc_mit && dist > 0) tm = mitg[mit_rank(mp, dist)] * ro;
+                    if (a.calc_cfd) tc = cfd_product<Real>(q, sg, mp, dist, s_cfd) * ro;
                     if constexpr (PREC == ISSL_PREC_FP32C) {
                         fm = f2_add(fm, tm);
                         fc = f2_add(fc, tc);
@@ -390,35 +420,36 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     }
                 }
             }
-            head += n;
+            head += cnt;
             __builtin_amdgcn_wave_barrier();
         };
 
         // Per-lane slice cursor: the next slice boundary and this slice's
         // posting offset stay in registers; LDS is read only at boundaries.
         int cur = 0;
-        uint64_t nb, dl;
+        uint32_t nb;
+        uint64_t dl;
         {
-            const uint64_t p0 = a0 + lane;
-            while (cur < S - 1 && p0 >= s_pre[wave][cur + 1]) ++cur;
-            nb = s_pre[wave][cur + 1];
-            dl = s_delta[wave][cur];
+            const uint32_t p0 = a0 + lane;
+            while (cur < S - 1 && p0 >= s_pre[wave][buf][cur + 1]) ++cur;
+            nb = s_pre[wave][buf][cur + 1];
+            dl = s_delta[wave][buf][cur];
         }
-        for (uint64_t base = a0; base < a1; base += 64 * DEPTH) {
+        for (uint32_t base = a0; base < a1; base += 64 * DEPTH) {
             uint64_t wv[DEPTH];
             int sl[DEPTH];
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
-                const uint64_t pos = base + 64 * d + lane;
+                const uint32_t pos = base + 64 * d + lane;
                 sl[d] = -1;
                 wv[d] = 0;
                 if (pos < a1) {
                     if (pos >= nb) {
                         do {
                             ++cur;
-                            nb = s_pre[wave][cur + 1];
+                            nb = s_pre[wave][buf][cur + 1];
                         } while (pos >= nb);
-                        dl = s_delta[wave][cur];
+                        dl = s_delta[wave][buf][cur];
                     }
                     sl[d] = cur;
                     wv[d] = a.site[pos + dl];
@@ -434,13 +465,16 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                         static_cast<uint32_t>(b >> 32),
                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
                     if (cand) {
-                        const uint32_t rel = static_cast<uint32_t>(base - a0) + 64 * d + lane;
-                        s_ring[wave][(tail + r) % kRingD] = rel | (static_cast<uint32_t>(sl[d]) << kRelBits);
+                        const uint32_t e = (tail + r) & (kRing - 1);
+                        s_rpos[wave][e] = (base - a0 + 64 * d + lane) |
+                                          (static_cast<uint32_t>(sl[d]) << kRelBits);
+                        s_rword[wave][e] = wv[d];
                     }
                     tail += static_cast<uint32_t>(__popcll(b));
                 }
+                if ((d + 1) % kRingChk == 0 || d == DEPTH - 1)
+                    while (tail - head >= 64) drain(64);
             }
-            while (tail - head >= 64) drain(64);
         }
         while (tail != head) drain(std::min<uint32_t>(64, tail - head));
         // Fixed butterfly: every lane ends with the same, order-determined value.
@@ -471,9 +505,15 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             }
             tp.occ = occ_sum;
             tp.sites = sites;
-            a.ws.part[t] = tp;
+            a.ws.part[vc] = tp;
         }
-        __builtin_amdgcn_wave_barrier();
+        if (vn == kNoSlot) break;
+        buf ^= 1;
+        plan_commit(mn, buf);
+        vc = vn;
+        mc = mn;
+        vn = next_slot();
+        if (vn != kNoSlot) plan_load(vn, mn);
     }
 }
 
@@ -500,19 +540,21 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
     constexpr int W = 256 / 64;
     __shared__ uint64_t s_fold[kMaxSlices];
+    __shared__ uint64_t s_posmask[kNoPos + 1];
     __shared__ uint64_t s_pre[W][kMaxSlices + 1];
     __shared__ uint64_t s_delta[W][kMaxSlices];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
     __syncthreads();
+    build_posmask(s_posmask, s_fold, a.S, tid);
+    __syncthreads();
     const uint64_t g = blockIdx.x * static_cast<uint64_t>(W) + wave;
     if (g >= a.n) return;
     const int S = a.S;
-    const uint32_t base = a.ws.tile_base[g];
-    const uint32_t K = a.ws.tile_base[g + 1] - base;
+    const uint32_t K = a.ws.gtiles[g];
     if (K == 0 && a.skip_untiled) return;
     TilePart mine{};
-    if (lane < static_cast<int>(K)) mine = a.ws.part[base + lane];
+    if (lane < static_cast<int>(K)) mine = a.ws.part[static_cast<uint64_t>(lane) * a.n + g];
 
     F2 fm{0.f, 0.f}, fc{0.f, 0.f};
     double dm = 0, dc = 0;
@@ -574,20 +616,15 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
                 const uint64_t sg = w & kSigBits;
                 const uint64_t f = bfold(q ^ sg);
                 const int dist = __popcll(f);
-                if (dist <= 4) {
-                    hit = true;
-                    for (int s2 = 0; s2 < cur; ++s2)
-                        if ((f & s_fold[s2]) == 0) {
-                            hit = false;
-                            break;
-                        }
-                }
+                uint32_t mp[4];
+                mismatches(f, mp);
+                hit = dist <= 4 && first_seen(mp, cur, s_posmask);
                 if (hit) {
                     occ = static_cast<uint32_t>(w >> 40);
                     if (occ == kOccEscape) occ = static_cast<uint32_t>(a.post[addr] >> 32);
-                    const Real ro = static_cast<Real>(occ);
-                    if (a.sc.calc_mit && dist > 0) tm = mitg[brank(f, dist)] * ro;
-                    if (a.sc.calc_cfd) tc = cfd_product<Real>(q, sg, f, dist, cfdg) * ro;
+                    const Real ro = to_real<Real>(occ);
+                    if (a.sc.calc_mit && dist > 0) tm = mitg[mit_rank(mp, dist)] * ro;
+                    if (a.sc.calc_cfd) tc = cfd_product<Real>(q, sg, mp, dist, cfdg) * ro;
                 }
             }
             uint64_t bal = __ballot(hit);
@@ -632,14 +669,11 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
             while (pos >= s_pre[wave][cur + 1]) ++cur;
             const uint64_t addr = pos + s_delta[wave][cur];
             const uint64_t f = bfold(q ^ (a.site[addr] & kSigBits));
-            if (__popcll(f) > 4) continue;
-            bool first = true;
-            for (int s2 = 0; s2 < cur; ++s2)
-                if ((f & s_fold[s2]) == 0) {
-                    first = false;
-                    break;
-                }
-            if (!first) continue;
+            const int dist = __popcll(f);
+            if (dist > 4) continue;
+            uint32_t mp[4];
+            mismatches(f, mp);
+            if (!first_seen(mp, cur, s_posmask)) continue;
             const uint32_t id = static_cast<uint32_t>(a.post[addr]);
             atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (id >> 6), 1ULL << (id & 63));
         }
@@ -707,21 +741,24 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
     constexpr int WAVES = 4;
     constexpr uint32_t RING = 64 * (DEPTH + 1);
-    __shared__ Real s_cfd[336];
+    __shared__ Real s_cfd[kCfdEntries];
     __shared__ uint64_t s_fold[kMaxSlices];
     __shared__ uint32_t s_lbase[kMaxSlices];
     __shared__ uint64_t s_pre[WAVES][2][kMaxSlices + 1];
     __shared__ uint64_t s_delta[WAVES][2][kMaxSlices];
     __shared__ uint32_t s_rpos[WAVES][RING];
     __shared__ uint8_t s_rsl[WAVES][RING];
+    __shared__ uint64_t s_posmask[kNoPos + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Real *cfdg = static_cast<const Real *>(a.cfd);
     const Real *mitg = static_cast<const Real *>(a.mit);
-    for (int i = tid; i < 336; i += 256) s_cfd[i] = cfdg[i];
+    for (int i = tid; i < kCfdEntries; i += 256) s_cfd[i] = cfdg[i];
     if (tid < a.S) {
         s_fold[tid] = a.meta[tid].fold;
         s_lbase[tid] = a.meta[tid].lbase;
     }
+    __syncthreads();
+    build_posmask(s_posmask, s_fold, a.S, tid);
     __syncthreads();
     const int S = a.S;
     const uint64_t W = gridDim.x * static_cast<uint64_t>(WAVES);
@@ -760,11 +797,11 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
         if (a.heavy) {
             if (P > a.heavy) {
                 if (lane < S) {
-                    a.ws.pre[gg * (S + 1) + lane] = excl;
+                    a.ws.pre[gg * (S + 1) + lane] = static_cast<uint32_t>(excl);
                     a.ws.delta[gg * S + lane] = dl_s;
                 }
                 if (lane == 0) {
-                    a.ws.pre[gg * (S + 1) + S] = P;
+                    a.ws.pre[gg * (S + 1) + S] = static_cast<uint32_t>(P);
                     uint64_t c = (P + kMaxTiles - 1) / kMaxTiles;
                     c = (c + 63) / 64 * 64;
                     if (c < a.chunk) c = a.chunk;
@@ -820,18 +857,15 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
                     const uint64_t sg = wv & kSigBits;
                     const uint64_t f = bfold(q ^ sg);
                     const int dist = __popcll(f);
-                    hit = true;
-                    for (int s2 = 0; s2 < sl; ++s2)
-                        if ((f & s_fold[s2]) == 0) {
-                            hit = false;
-                            break;
-                        }
+                    uint32_t mp[4];
+                    mismatches(f, mp);
+                    hit = first_seen(mp, sl, s_posmask);
                     if (hit) {
                         occ = static_cast<uint32_t>(wv >> 40);
                         if (occ == kOccEscape) occ = static_cast<uint32_t>(a.post[addr] >> 32);
-                        const Real ro = static_cast<Real>(occ);
-                        if (a.sc.calc_mit && dist > 0) tm = mitg[brank(f, dist)] * ro;
-                        if (a.sc.calc_cfd) tc = cfd_product<Real>(q, sg, f, dist, s_cfd) * ro;
+                        const Real ro = to_real<Real>(occ);
+                        if (a.sc.calc_mit && dist > 0) tm = mitg[mit_rank(mp, dist)] * ro;
+                        if (a.sc.calc_cfd) tc = cfd_product<Real>(q, sg, mp, dist, s_cfd) * ro;
                     }
                 }
                 uint64_t hb = __ballot(hit);
@@ -962,20 +996,14 @@ namespace {
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 int carve(Workspace &ws, uint64_t n, int S, BatchWs *b) {
-    const size_t sz_pre = align256(n * (S + 1) * 8);
+    const size_t sz_pre = align256(n * (S + 1) * 4);
     const size_t sz_delta = align256(n * std::max(S, 1) * 8);
     const size_t sz_u32 = align256(n * 4);
-    const size_t sz_base = align256((n + 1) * 4);
-    const size_t sz_ctrl = 256;
-    const size_t sz_cnt = align256(static_cast<size_t>(kMaxWaves) * kCounterStride * 4);
-    const size_t sz_map = align256(n * kMaxTiles * 8);
     const size_t sz_part = align256(n * kMaxTiles * sizeof(TilePart));
-    const size_t total =
-        sz_pre + sz_delta + 2 * sz_u32 + sz_base + sz_ctrl + sz_cnt + sz_map + sz_part;
-    int rc = ws.tiles.reserve(total);
+    int rc = ws.tiles.reserve(sz_pre + sz_delta + 2 * sz_u32 + sz_part);
     if (rc != ISSL_OK) return rc;
     char *p = static_cast<char *>(ws.tiles.p);
-    b->pre = reinterpret_cast<uint64_t *>(p);
+    b->pre = reinterpret_cast<uint32_t *>(p);
     p += sz_pre;
     b->delta = reinterpret_cast<uint64_t *>(p);
     p += sz_delta;
@@ -983,52 +1011,20 @@ int carve(Workspace &ws, uint64_t n, int S, BatchWs *b) {
     p += sz_u32;
     b->gtiles = reinterpret_cast<uint32_t *>(p);
     p += sz_u32;
-    b->tile_base = reinterpret_cast<uint32_t *>(p);
-    p += sz_base;
-    b->ctrl = reinterpret_cast<uint32_t *>(p);
-    p += sz_ctrl;
-    b->cnt = reinterpret_cast<uint32_t *>(p);
-    p += sz_cnt;
-    b->tile_map = reinterpret_cast<uint64_t *>(p);
-    p += sz_map;
     b->part = reinterpret_cast<TilePart *>(p);
     return ISSL_OK;
-}
-
-template <int PREC, int DEPTH>
-void launch_tiles(const TileArgs &ta, int grid, hipStream_t st) {
-    hipLaunchKernelGGL((k_tiles<PREC, DEPTH>), dim3(grid), dim3(kTileBlock), 0, st, ta);
 }
 
 template <int PREC>
 int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st) {
     switch (depth) {
-    case 1: launch_tiles<PREC, 1>(ta, grid, st); break;
-    case 2: launch_tiles<PREC, 2>(ta, grid, st); break;
-    case 4: launch_tiles<PREC, 4>(ta, grid, st); break;
-    case 8: launch_tiles<PREC, 8>(ta, grid, st); break;
-    case 16: launch_tiles<PREC, 16>(ta, grid, st); break;
-    default: return fail(ISSL_ERR_ARG, "tuning depth must be 1, 2, 4, 8 or 16");
+    case 4: hipLaunchKernelGGL((k_tiles<PREC, 4>), dim3(grid), dim3(kTileBlock), 0, st, ta); break;
+    case 8: hipLaunchKernelGGL((k_tiles<PREC, 8>), dim3(grid), dim3(kTileBlock), 0, st, ta); break;
+    case 16: hipLaunchKernelGGL((k_tiles<PREC, 16>), dim3(grid), dim3(kTileBlock), 0, st, ta); break;
+    default: return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
     }
     return ISSL_OK;
 }
-
-}  // namespace
-
-issl_tuning default_tuning() {
-    issl_tuning t{};
-    t.chunk = 2048;
-    t.block = kTileBlock;
-    t.depth = 8;
-    t.grid_per_cu = 4;
-    t.sched = kSchedStatic;
-    t.steal = 8;
-    t.direct_depth = 8;
-    t.heavy_kpost = kHeavyNever;
-    return t;
-}
-
-namespace {
 
 template <int PREC>
 int launch_direct_depth(const StreamArgs &da, int depth, uint32_t grid, hipStream_t st) {
@@ -1052,19 +1048,22 @@ int cu_count(int device) {
     return cus;
 }
 
-}  // namespace
+DevResults offset_results(const DevResults &o, uint64_t i) {
+    DevResults r = o;
+    if (r.mit) r.mit += i;
+    if (r.cfd) r.cfd += i;
+    if (r.mit_sum) r.mit_sum += i;
+    if (r.cfd_sum) r.cfd_sum += i;
+    if (r.occ) r.occ += i;
+    if (r.sites) r.sites += i;
+    if (r.pass) r.pass += i;
+    return r;
+}
 
-int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
-                   int precision, int chunk, const issl_tuning &tn_in, const DevResults &out,
-                   uint64_t *distinct, Workspace &ws, hipStream_t st) {
-    if (n == 0) return ISSL_OK;
-    if (n > (0xFFFFFFFFULL / kMaxTiles)) return fail(ISSL_ERR_ARG, "too many guides for one launch");
-    issl_tuning tn = tn_in;
-    if (tn.chunk <= 0) tn = default_tuning();
-    if (tn.sched < 0 || tn.sched > kSchedSteal) return fail(ISSL_ERR_ARG, "unknown tile schedule");
-    if (chunk > 0) tn.chunk = chunk;
-    if (tn.chunk % 64 != 0 || tn.chunk > (1 << 20))
-        return fail(ISSL_ERR_ARG, "chunk must be a multiple of 64 up to 2^20");
+// One launch sequence over n <= kMaxBatch guides.
+int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
+                 int precision, const issl_tuning &tn, const DevResults &out, uint64_t *distinct,
+                 Workspace &ws, hipStream_t st) {
     const int S = static_cast<int>(ix->S);
     BatchWs b;
     int rc = carve(ws, n, S, &b);
@@ -1074,14 +1073,6 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     const void *cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
     const bool direct = tn.heavy_kpost != 0;
     const bool tiles = !direct || tn.heavy_kpost != kHeavyNever;
-
-    // Persistent grid: grid_per_cu workgroups per CU, no more waves than tiles can need.
-    const uint64_t max_tiles = n * kMaxTiles;
-    uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
-    grid = std::min<uint64_t>(grid, kMaxWaves / kTileWaves);
-    const uint64_t waves_needed = std::max<uint64_t>(1, std::min<uint64_t>(max_tiles, grid * kTileWaves));
-    grid = std::min<uint64_t>(grid, (waves_needed + kTileWaves - 1) / kTileWaves);
-    const uint32_t n_waves = static_cast<uint32_t>(grid * kTileWaves);
     const uint32_t gw = static_cast<uint32_t>((n + 3) / 4);
 
     if (direct) {
@@ -1118,8 +1109,6 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
         PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk), b};
         hipLaunchKernelGGL(k_plan, dim3(gw), dim3(256), 0, st, pa);
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, st, b, n,
-                       tn.sched == kSchedSteal ? n_waves : 0u);
 
     TileArgs ta;
     ta.guides = d_guides;
@@ -1128,14 +1117,16 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     ta.meta = ix->d_meta;
     ta.mit = mit;
     ta.cfd = cfd;
+    ta.n = static_cast<uint32_t>(n);
     ta.S = S;
     ta.mark_distinct = (distinct && !sc.use_cut) ? 1 : 0;
     ta.calc_mit = sc.calc_mit;
     ta.calc_cfd = sc.calc_cfd;
-    ta.sched = tn.sched;
-    ta.steal = tn.steal;
     ta.distinct = distinct;
     ta.ws = b;
+    // Persistent grid: grid_per_cu workgroups per CU, no more waves than slots.
+    uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
+    grid = std::min<uint64_t>(grid, (n * kMaxTiles + kTileWaves - 1) / kTileWaves);
     if (!direct) timing_mark(ix, st);
     rc = fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st)
               : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st);
@@ -1161,6 +1152,40 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     else
         hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP64>), dim3(gw), dim3(256), 0, st, fa);
     ISSL_HIP(hipGetLastError());
+    return ISSL_OK;
+}
+
+}  // namespace
+
+issl_tuning default_tuning() {
+    issl_tuning t{};
+    t.chunk = 1024;
+    t.block = kTileBlock;
+    t.depth = 8;
+    t.grid_per_cu = 6;
+    t.direct_depth = 8;
+    t.heavy_kpost = 0;  // every guide on the tile kernel
+    return t;
+}
+
+int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
+                   int precision, int chunk, const issl_tuning &tn_in, const DevResults &out,
+                   uint64_t *distinct, Workspace &ws, hipStream_t st) {
+    if (n == 0) return ISSL_OK;
+    if (ix->p_max >= (1ULL << 31))
+        return fail(ISSL_ERR_ARG, "index lists are too long for the batched form (a guide could "
+                                  "visit 2^31 postings): use the plain form");
+    issl_tuning tn = tn_in;
+    if (tn.chunk <= 0) tn = default_tuning();
+    if (chunk > 0) tn.chunk = chunk;
+    if (tn.chunk % 64 != 0 || tn.chunk > (1 << 20))
+        return fail(ISSL_ERR_ARG, "chunk must be a multiple of 64 up to 2^20");
+    for (uint64_t i = 0; i < n; i += kMaxBatch) {
+        const uint64_t m = std::min<uint64_t>(kMaxBatch, n - i);
+        const int rc = launch_batch(ix, d_guides + i, m, sc, precision, tn, offset_results(out, i),
+                                    distinct, ws, st);
+        if (rc != ISSL_OK) return rc;
+    }
     return ISSL_OK;
 }
 

@@ -607,11 +607,13 @@ extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64
         const uint64_t nk = 1ULL << (2 * w);
         if (lstart.size() + nk + 1 > 0xFFFFFFFFULL)
             return fail(ISSL_ERR_FORMAT, "too many lists for 32-bit list offsets");
-        uint64_t run = 0;
+        uint64_t run = 0, longest = 0;
         for (uint64_t k = 0; k < nk; ++k) {
             lstart.push_back(static_cast<uint32_t>(run));
             run += list_sizes[sz_base + k];
+            longest = std::max<uint64_t>(longest, list_sizes[sz_base + k]);
         }
+        ix->p_max += longest;
         if (run != n_sites)
             return fail(ISSL_ERR_FORMAT, "slice " + std::to_string(s) + " list sizes sum to " +
                                              std::to_string(run) + ", not N");
@@ -634,8 +636,8 @@ extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64
     if ((rc = dev_alloc(ix.get(), &ix->d_meta, std::max<uint64_t>(n_slices, 1))) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix.get(), &ix->d_mit64, kMitEntries)) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix.get(), &ix->d_mit32, kMitEntries)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_cfd64, 336)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_cfd32, 336)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_cfd64, kCfdTable)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix.get(), &ix->d_cfd32, kCfdTable)) != ISSL_OK) return rc;
     ix->distinct_words = n_sites / 64 + 1;
     if ((rc = dev_alloc(ix.get(), &ix->d_distinct, ix->distinct_words)) != ISSL_OK) return rc;
 
@@ -645,16 +647,16 @@ extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64
     if (n_slices)
         ISSL_HIP(hipMemcpy(ix->d_meta, ix->meta_h.data(), n_slices * sizeof(SliceMeta),
                            hipMemcpyHostToDevice));
-    std::vector<float> m32(kMitEntries), c32(336);
-    std::vector<double> c64(336);
+    std::vector<float> m32(kMitEntries), c32(kCfdTable);
+    std::vector<double> c64(kCfdTable, 1.0);  // [336] = 1.0: factor of an absent mismatch
     for (int r = 0; r < kMitEntries; ++r) m32[r] = static_cast<float>(mit_by_rank()[r]);
     std::memcpy(c64.data(), cfd_pos_table(), 320 * 8);
     std::memcpy(c64.data() + 320, cfd_pam_table(), 16 * 8);
-    for (int i = 0; i < 336; ++i) c32[i] = static_cast<float>(c64[i]);
+    for (int i = 0; i < kCfdTable; ++i) c32[i] = static_cast<float>(c64[i]);
     ISSL_HIP(hipMemcpy(ix->d_mit64, mit_by_rank(), kMitEntries * 8, hipMemcpyHostToDevice));
     ISSL_HIP(hipMemcpy(ix->d_mit32, m32.data(), kMitEntries * 4, hipMemcpyHostToDevice));
-    ISSL_HIP(hipMemcpy(ix->d_cfd64, c64.data(), 336 * 8, hipMemcpyHostToDevice));
-    ISSL_HIP(hipMemcpy(ix->d_cfd32, c32.data(), 336 * 4, hipMemcpyHostToDevice));
+    ISSL_HIP(hipMemcpy(ix->d_cfd64, c64.data(), kCfdTable * 8, hipMemcpyHostToDevice));
+    ISSL_HIP(hipMemcpy(ix->d_cfd32, c32.data(), kCfdTable * 4, hipMemcpyHostToDevice));
     ISSL_HIP(hipMemset(ix->d_distinct, 0, ix->distinct_words * 8));
     if ((rc = build_inline_layout(ix.get())) != ISSL_OK) return rc;
     ISSL_HIP(hipStreamSynchronize(ix->stream));

@@ -51,6 +51,9 @@ constexpr int32_t kHeavyNever = ISSL_HEAVY_NEVER;
 constexpr uint64_t kSigBits = (1ULL << 40) - 1;
 constexpr uint32_t kOccEscape = 0xFFFFFF;
 
+// Device CFD table: 320 position factors, 16 PAM factors, then 1.0.
+constexpr int kCfdTable = 337;
+
 // Grow-only device buffer.
 struct DevBuf {
     void *p = nullptr;
@@ -95,6 +98,7 @@ struct issl_index {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t n = 0, seq_len = 0, S = 0, n_lists = 0;
+    uint64_t p_max = 0;             // sum over slices of the longest list: bounds P(g)
     int max_w = 0;
     std::vector<uint64_t> masks;
     std::vector<issl::SliceMeta> meta_h;
@@ -106,7 +110,7 @@ struct issl_index {
     issl::SliceMeta *d_meta = nullptr;
     double *d_mit64 = nullptr;      // [6195] by combinadic rank
     float *d_mit32 = nullptr;
-    double *d_cfd64 = nullptr;      // [320 pos | 16 pam]
+    double *d_cfd64 = nullptr;      // [320 pos | 16 pam | 1.0]
     float *d_cfd32 = nullptr;
     uint64_t *d_distinct = nullptr; // [n/64 + 1]
     uint64_t distinct_words = 0;

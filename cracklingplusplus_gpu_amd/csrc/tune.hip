@@ -1,7 +1,8 @@
 // tune.hip -- launch-shape autotuner for the BATCHED form.
 //
-// Candidates: tile length (chunk), postings in flight per lane (depth) and
-// persistent workgroups per CU.  Each is timed with HIP events on a sample of
+// Candidates: tile length (chunk), postings in flight per lane (depth),
+// persistent workgroups per CU, and the one-wave-per-guide kernel with or
+// without hand-off of long guides to the tile kernel.  Each is timed with HIP events on a sample of
 // the caller's guides against the resident index; the fastest is kept on the
 // index and, when a cache path is given, appended to a text cache keyed by
 // the index shape and the device, so later processes skip the sweep.
@@ -31,7 +32,7 @@ std::string shape_key(const issl_index *ix) {
     for (char &c : arch)
         if (c == ' ' || c == ':') c = '_';
     std::ostringstream k;
-    k << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
+    k << "t2/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
     for (const SliceMeta &m : ix->meta_h) k << m.width << ',';
     return k.str();
 }
@@ -45,8 +46,8 @@ bool cache_lookup(const char *path, const std::string &key, issl_tuning *t) {
         std::istringstream ls(line);
         std::string k;
         issl_tuning v{};
-        if (ls >> k >> v.chunk >> v.block >> v.depth >> v.grid_per_cu >> v.sched >> v.steal >>
-                v.direct_depth >> v.heavy_kpost >> v.us_per_guide &&
+        if (ls >> k >> v.chunk >> v.block >> v.depth >> v.grid_per_cu >> v.direct_depth >>
+                v.heavy_kpost >> v.us_per_guide &&
             k == key) {
             *t = v;
             found = true;  // the last entry for a key wins
@@ -67,9 +68,7 @@ extern "C" int issl_set_tuning(issl_index *ix, const issl_tuning *in) {
     if (!ix || !in) return fail(ISSL_ERR_ARG, "null argument");
     const int d = in->depth;
     if (in->chunk < 64 || in->chunk % 64 != 0 || in->chunk > (1 << 20) ||
-        !(d == 1 || d == 2 || d == 4 || d == 8 || d == 16) || in->grid_per_cu < 1 ||
-        in->grid_per_cu > 32 || in->sched < 0 || in->sched > 2 || in->steal < 0 ||
-        in->steal > 1024 ||
+        !(d == 4 || d == 8 || d == 16) || in->grid_per_cu < 1 || in->grid_per_cu > 32 ||
         !(in->direct_depth == 4 || in->direct_depth == 8 || in->direct_depth == 16) ||
         in->heavy_kpost < 0)
         return fail(ISSL_ERR_ARG, "tuning out of range");
@@ -139,21 +138,19 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
         if (t.us_per_guide < top.us_per_guide) top = t;
         return ISSL_OK;
     };
-    // (1) every guide on the persistent tile kernel: tile shape and schedule
-    const int chunks[] = {1024, 2048, 4096};
-    const int depths[] = {4, 8};
-    const int grids[] = {4, 7, 8};
-    const int scheds[][2] = {{1, 0}, {2, 8}};
+    // (1) every guide on the persistent tile kernel: tile shape and occupancy
+    const int chunks[] = {512, 1024, 2048};
+    const int depths[] = {4, 8, 16};
+    const int grids[] = {4, 6, 8};
     issl_tuning tiles_best{};
     tiles_best.us_per_guide = 1e300;
     for (int c : chunks)
         for (int d : depths)
-            for (const auto &sch : scheds)
-                for (int gpc : grids) {
-                    issl_tuning t{c, 256, d, gpc, sch[0], sch[1], 8, 0, 0};
-                    if ((rc = measure(t)) != ISSL_OK) return rc;
-                    if (t.us_per_guide < tiles_best.us_per_guide) tiles_best = t;
-                }
+            for (int gpc : grids) {
+                issl_tuning t{c, 256, d, gpc, 8, 0, 0};
+                if ((rc = measure(t)) != ISSL_OK) return rc;
+                if (t.us_per_guide < tiles_best.us_per_guide) tiles_best = t;
+            }
     if (tiles_best.us_per_guide < top.us_per_guide) top = tiles_best;
     // (2) one wave per guide, optionally sending long guides to the tile kernel
     const int ddepths[] = {4, 8, 16};
@@ -176,9 +173,9 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%s %d %d %d %d %d %d %d %d %.6g\n", key.c_str(), top.chunk, top.block,
-                    top.depth, top.grid_per_cu, top.sched, top.steal, top.direct_depth,
-                    top.heavy_kpost, top.us_per_guide);
+            fprintf(f, "%s %d %d %d %d %d %d %.6g\n", key.c_str(), top.chunk, top.block,
+                    top.depth, top.grid_per_cu, top.direct_depth, top.heavy_kpost,
+                    top.us_per_guide);
             fclose(f);
         }
     }

@@ -71,6 +71,11 @@ class DeviceArray:
             pass
 
 
+def copy_device(dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
+    """Device-to-device copy of nbytes (the runtime's own copy kernel)."""
+    _ok(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
+
+
 def synchronize():
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 

@@ -49,9 +49,9 @@ extern "C" {
  *   PLAIN   : one guide per workgroup, FP64, hits accumulated in the
  *             reference's slice-major posting order (bit-exact sums and the
  *             reference's early-exit point).
- *   BATCHED : persistent work-stealing kernel over (guide, chunk) tiles of
- *             the guide's slice-major posting sequence, sites streamed from
- *             a signature-inline device layout; precision selectable. */
+ *   BATCHED : persistent waves over (guide, chunk) tiles of the guide's
+ *             slice-major posting sequence, sites streamed from a
+ *             signature-inline device layout; precision selectable. */
 #define ISSL_FORM_PLAIN    0
 #define ISSL_FORM_BATCHED  1
 
@@ -170,11 +170,8 @@ typedef struct {
     int32_t chunk;        /* postings per tile (minimum; heavy guides get longer
                              tiles so no guide has more than 64) */
     int32_t block;        /* threads per workgroup (fixed at 256) */
-    int32_t depth;        /* postings in flight per lane: 1, 2, 4, 8 or 16 */
+    int32_t depth;        /* postings in flight per lane: 4, 8 or 16 */
     int32_t grid_per_cu;  /* persistent workgroups per CU: 1..32 */
-    int32_t sched;        /* tile schedule: 0 one global queue, 1 static
-                             round-robin, 2 per-wave ranges + stealing */
-    int32_t steal;        /* ranges of other waves tried after the own one */
     int32_t direct_depth; /* postings in flight per lane in the one-wave-per-
                              guide kernel: 4, 8 or 16 */
     int32_t heavy_kpost;  /* guides with more than heavy_kpost*1024 postings go

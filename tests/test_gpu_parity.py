@@ -114,13 +114,14 @@ def test_batched_exit_point_matches_reference(cases, name):
 
 
 BATCHED_MODES = {
-    # every guide through the persistent tile kernel
-    "tiles": dict(chunk=1024, depth=4, grid_per_cu=4, sched=2, steal=8, heavy_kpost=0),
+    # every guide through the persistent tile kernel, three tile shapes
+    "tiles": dict(chunk=1024, depth=4, grid_per_cu=4, heavy_kpost=0),
+    "tiles_short": dict(chunk=64, depth=16, grid_per_cu=1, heavy_kpost=0),
+    "tiles_deep": dict(chunk=512, depth=16, grid_per_cu=8, heavy_kpost=0),
     # one wave per guide only
     "direct": dict(direct_depth=8, heavy_kpost=issl._capi.HEAVY_NEVER),
     # guides with more than 1,024 postings go to the tile kernel
-    "hybrid": dict(chunk=512, depth=4, grid_per_cu=2, sched=0, direct_depth=4, heavy_kpost=1),
-    "static": dict(chunk=512, depth=8, grid_per_cu=8, sched=1, heavy_kpost=0),
+    "hybrid": dict(chunk=512, depth=4, grid_per_cu=2, direct_depth=4, heavy_kpost=1),
 }
 
 
