@@ -302,11 +302,13 @@ class Index:
         check(lib().issl_get_tuning(self._h, byref(t)))
         return t
 
-    def timing_enable(self, enable: bool = True) -> None:
-        check(lib().issl_timing_enable(self._h, 1 if enable else 0))
+    def timing_enable(self, enable=True, repeat: int = 1) -> None:
+        """Time the dominant kernel of each scoring launch; repeat > 1 dispatches it
+        that many times back to back between one event pair (results unchanged)."""
+        check(lib().issl_timing_enable(self._h, max(1, int(repeat)) if enable else 0))
 
     def timing_read(self):
-        """(summed dominant-kernel ms, launches) since timing_enable."""
+        """(summed dominant-kernel ms, dispatches timed) since timing_enable."""
         ms, n = ctypes.c_double(), c_uint64()
         check(lib().issl_timing_read(self._h, byref(ms), byref(n)))
         return ms.value, int(n.value)

@@ -1015,23 +1015,28 @@ int carve(Workspace &ws, uint64_t n, int S, BatchWs *b) {
     return ISSL_OK;
 }
 
+// e0/e1: timing events attached to the dispatch (null when timing is off).
 template <int PREC>
-int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st) {
+int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st, hipEvent_t e0,
+                       hipEvent_t e1) {
+    const dim3 g(grid), b(kTileBlock);
     switch (depth) {
-    case 4: hipLaunchKernelGGL((k_tiles<PREC, 4>), dim3(grid), dim3(kTileBlock), 0, st, ta); break;
-    case 8: hipLaunchKernelGGL((k_tiles<PREC, 8>), dim3(grid), dim3(kTileBlock), 0, st, ta); break;
-    case 16: hipLaunchKernelGGL((k_tiles<PREC, 16>), dim3(grid), dim3(kTileBlock), 0, st, ta); break;
+    case 4: hipExtLaunchKernelGGL((k_tiles<PREC, 4>), g, b, 0, st, e0, e1, 0, ta); break;
+    case 8: hipExtLaunchKernelGGL((k_tiles<PREC, 8>), g, b, 0, st, e0, e1, 0, ta); break;
+    case 16: hipExtLaunchKernelGGL((k_tiles<PREC, 16>), g, b, 0, st, e0, e1, 0, ta); break;
     default: return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
     }
     return ISSL_OK;
 }
 
 template <int PREC>
-int launch_direct_depth(const StreamArgs &da, int depth, uint32_t grid, hipStream_t st) {
+int launch_direct_depth(const StreamArgs &da, int depth, uint32_t grid, hipStream_t st,
+                        hipEvent_t e0, hipEvent_t e1) {
+    const dim3 g(grid), b(256);
     switch (depth) {
-    case 4: hipLaunchKernelGGL((k_stream<PREC, 4>), dim3(grid), dim3(256), 0, st, da); break;
-    case 8: hipLaunchKernelGGL((k_stream<PREC, 8>), dim3(grid), dim3(256), 0, st, da); break;
-    case 16: hipLaunchKernelGGL((k_stream<PREC, 16>), dim3(grid), dim3(256), 0, st, da); break;
+    case 4: hipExtLaunchKernelGGL((k_stream<PREC, 4>), g, b, 0, st, e0, e1, 0, da); break;
+    case 8: hipExtLaunchKernelGGL((k_stream<PREC, 8>), g, b, 0, st, e0, e1, 0, da); break;
+    case 16: hipExtLaunchKernelGGL((k_stream<PREC, 16>), g, b, 0, st, e0, e1, 0, da); break;
     default: return fail(ISSL_ERR_ARG, "direct depth must be 4, 8 or 16");
     }
     return ISSL_OK;
@@ -1093,14 +1098,15 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
         da.heavy = tiles ? static_cast<uint64_t>(tn.heavy_kpost) * 1024 : 0;
         da.chunk = static_cast<uint32_t>(tn.chunk);
         da.ws = b;
-        timing_mark(ix, st);
         // Persistent guide streams: grid_per_cu workgroups per CU, at most one wave per guide.
         const uint64_t sgrid = std::max<uint64_t>(
             1, std::min<uint64_t>(gw, static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu));
-        rc = fp32 ? launch_direct_depth<ISSL_PREC_FP32C>(da, tn.direct_depth, static_cast<uint32_t>(sgrid), st)
-                  : launch_direct_depth<ISSL_PREC_FP64>(da, tn.direct_depth, static_cast<uint32_t>(sgrid), st);
+        const uint32_t sg = static_cast<uint32_t>(sgrid);
+        rc = timed_dispatch(ix, st, [&](hipEvent_t e0, hipEvent_t e1) {
+            return fp32 ? launch_direct_depth<ISSL_PREC_FP32C>(da, tn.direct_depth, sg, st, e0, e1)
+                        : launch_direct_depth<ISSL_PREC_FP64>(da, tn.direct_depth, sg, st, e0, e1);
+        });
         if (rc != ISSL_OK) return rc;
-        timing_mark(ix, st);
         if (!tiles) {
             ISSL_HIP(hipGetLastError());
             return ISSL_OK;
@@ -1127,11 +1133,12 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     // Persistent grid: grid_per_cu workgroups per CU, no more waves than slots.
     uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
     grid = std::min<uint64_t>(grid, (n * kMaxTiles + kTileWaves - 1) / kTileWaves);
-    if (!direct) timing_mark(ix, st);
-    rc = fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st)
-              : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st);
+    auto tiles_launch = [&](hipEvent_t e0, hipEvent_t e1) {
+        return fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st, e0, e1)
+                    : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st, e0, e1);
+    };
+    rc = direct ? tiles_launch(nullptr, nullptr) : timed_dispatch(ix, st, tiles_launch);
     if (rc != ISSL_OK) return rc;
-    if (!direct) timing_mark(ix, st);
 
     FinArgs fa;
     fa.guides = d_guides;

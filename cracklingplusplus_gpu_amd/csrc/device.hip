@@ -357,11 +357,13 @@ int launch_plain(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     a.distinct = distinct;
     a.cand_off = cand_off;
     a.cand_ids = cand_ids;
-    timing_mark(ix, stream);
-    hipLaunchKernelGGL((k_plain<kPlainBlock, kPlainPer>), dim3(static_cast<uint32_t>(n)),
-                       dim3(kPlainBlock), 0, stream, a);
+    const int rc = timed_dispatch(ix, stream, [&](hipEvent_t e0, hipEvent_t e1) {
+        hipExtLaunchKernelGGL((k_plain<kPlainBlock, kPlainPer>), dim3(static_cast<uint32_t>(n)),
+                              dim3(kPlainBlock), 0, stream, e0, e1, 0, a);
+        return ISSL_OK;
+    });
+    if (rc != ISSL_OK) return rc;
     ISSL_HIP(hipGetLastError());
-    timing_mark(ix, stream);
     return ISSL_OK;
 }
 
@@ -409,14 +411,16 @@ __global__ void k_validate(const uint64_t *__restrict__ post, const uint64_t *__
     }
 }
 
-void timing_mark(issl_index *ix, hipStream_t stream) {
+void timing_events(issl_index *ix, hipEvent_t *start, hipEvent_t *stop) {
+    *start = *stop = nullptr;
     if (!ix->timing) return;
-    if (ix->ev_used == ix->ev_pool.size()) {
+    while (ix->ev_pool.size() < ix->ev_used + 2) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
         ix->ev_pool.push_back(e);
     }
-    (void)hipEventRecord(ix->ev_pool[ix->ev_used++], stream);
+    *start = ix->ev_pool[ix->ev_used++];
+    *stop = ix->ev_pool[ix->ev_used++];
 }
 
 int build_inline_layout(issl_index *ix) {
@@ -894,7 +898,8 @@ extern "C" int issl_timing_enable(issl_index *ix, int enable) {
     DeviceGuard guard(ix->device);
     for (size_t i = 0; i < ix->ev_used; ++i) ISSL_HIP(hipEventSynchronize(ix->ev_pool[i]));
     ix->ev_used = 0;
-    ix->timing = enable != 0;
+    if (enable < 0 || enable > 1024) return fail(ISSL_ERR_ARG, "timing repeat must be 0..1024");
+    ix->timing = enable;
     return ok();
 }
 
@@ -910,6 +915,6 @@ extern "C" int issl_timing_read(issl_index *ix, double *kernel_ms, uint64_t *lau
         total += ms;
     }
     *kernel_ms = total;
-    *launches = ix->ev_used / 2;
+    *launches = ix->ev_used / 2 * static_cast<uint64_t>(std::max(ix->timing, 1));
     return ok();
 }

@@ -1,6 +1,7 @@
 // device.hpp -- resident device index and kernel interfaces of libissl_gpu.so.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -118,7 +119,7 @@ struct issl_index {
     issl_tuning tuning{};
     bool tuned = false;
     // Kernel timing: event pairs around the dominant kernel of each launch.
-    bool timing = false;
+    int timing = 0;                 // 0 off; else dispatches per event pair
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::mutex mu;
@@ -137,7 +138,28 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
 int build_inline_layout(issl_index *ix);
 // Launch shape used before any tuning.
 issl_tuning default_tuning();
-// Record a timing event on `stream` when timing is enabled (no-op otherwise).
-void timing_mark(issl_index *ix, hipStream_t stream);
+// Event pair for the dominant kernel of a launch when timing is enabled
+// (both null otherwise).  The pair is attached to that kernel's dispatch with
+// hipExtLaunchKernelGGL, so it times the kernel itself, not the gaps around it.
+void timing_events(issl_index *ix, hipEvent_t *start, hipEvent_t *stop);
+
+// Dispatch the dominant kernel of a launch under the timing mode.  launch(e0,
+// e1) issues one dispatch with those events attached (null: none).  With
+// timing at N > 1 the kernel, which is idempotent, is dispatched N times back
+// to back between two recorded events, so the per-dispatch average carries
+// 1/N of the event overhead; results are unchanged.
+template <typename F>
+int timed_dispatch(issl_index *ix, hipStream_t st, F &&launch) {
+    hipEvent_t e0, e1;
+    timing_events(ix, &e0, &e1);
+    if (ix->timing <= 1 || !e0) return launch(e0, e1);
+    ISSL_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < ix->timing; ++r) {
+        const int rc = launch(nullptr, nullptr);
+        if (rc != ISSL_OK) return rc;
+    }
+    ISSL_HIP(hipEventRecord(e1, st));
+    return ISSL_OK;
+}
 
 }  // namespace issl

@@ -195,10 +195,14 @@ int issl_set_tuning(issl_index *ix, const issl_tuning *in);
 
 /* Kernel timing (no reference counterpart; the reference prints wall-clock
  * phase timers, ISSLScoreOfftargets.cpp:185-188,296,507-510).  While enabled,
- * HIP events bracket the dominant kernel of every scoring launch on this
- * index (k_plain for PLAIN, k_tiles for BATCHED) on the launch's stream.
- * Enabling clears earlier marks; read synchronizes and returns the summed
- * kernel time and the number of bracketed launches. */
+ * a pair of HIP events is attached to the dispatch of the dominant kernel of
+ * every scoring launch on this index (k_plain for PLAIN, k_tiles or k_stream
+ * for BATCHED; hipExtLaunchKernelGGL), so they time that kernel alone.
+ * enable = N > 1 dispatches that kernel N times back to back (it is
+ * idempotent; results are unchanged) between one recorded event pair, which
+ * spreads the events' own overhead over N dispatches.  Enabling clears
+ * earlier marks; read synchronizes and returns the summed kernel time and the
+ * number of dispatches timed. */
 int issl_timing_enable(issl_index *ix, int enable);
 int issl_timing_read(issl_index *ix, double *kernel_ms, uint64_t *launches);
 

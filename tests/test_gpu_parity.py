@@ -242,3 +242,21 @@ def test_autotune_keeps_results(cases, tmp_path):
     assert (after.site_count == before.site_count).all()
     t2 = ix.autotune(g, str(tmp_path / "tune.txt"))  # from the cache
     assert (t2.chunk, t2.depth, t2.grid_per_cu) == (t.chunk, t.depth, t.grid_per_cu)
+
+
+@pytest.mark.parametrize("form", [issl.FORM_PLAIN, issl.FORM_BATCHED])
+def test_timing_repeat_keeps_results(cases, form):
+    """Timed launches (one or several dispatches per event pair) give the same results."""
+    hx, ix, gd = cases["mid50k_10x10"]
+    g = gd["guides"]
+    ref = ix.score(g, form=form)
+    for repeat in (1, 3):
+        ix.timing_enable(True, repeat=repeat)
+        r = ix.score(g, form=form)
+        r = ix.score(g, form=form)
+        ms, n = ix.timing_read()
+        ix.timing_enable(False)
+        assert n == 2 * repeat and ms > 0
+        assert (bits(r.cfd) == bits(ref.cfd)).all()
+        assert (bits(r.mit) == bits(ref.mit)).all()
+        assert (r.site_count == ref.site_count).all()
