@@ -28,7 +28,8 @@ __all__ = [
     "write_index", "read_index", "parse_masks", "count_sorted_sites", "score_tables",
     "synth", "device_count", "METHOD_MIT", "METHOD_CFD", "METHOD_MIT_AND_CFD",
     "METHOD_MIT_OR_CFD", "METHOD_AVG_MIT_CFD", "FORM_PLAIN", "FORM_BATCHED", "PREC_FP64",
-    "PREC_FP32C", "FLAG_DISTINCT", "METHOD_NAMES", "CLI_METHODS",
+    "PREC_FP32C", "FLAG_DISTINCT", "METHOD_NAMES", "CLI_METHODS", "Server", "Request",
+    "device_memory",
 ]
 
 # include/util.hpp:20 and lib/util.cpp:13-19
@@ -335,3 +336,98 @@ class Index:
         the rest one wave per guide (0: every guide uses tiles; HEAVY_NEVER: none)."""
         t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, direct_depth, heavy_kpost, 0.0)
         check(lib().issl_set_tuning(self._h, byref(t)))
+
+
+# ------------------------------------------------------------------ serving
+
+class Request:
+    """A submitted request: its ticket and the host buffers the results land in."""
+
+    def __init__(self, server: "Server", ticket: int, results: Results):
+        self.server = server
+        self.ticket = ticket
+        self.results = results
+        self.status: Optional[int] = None
+
+    def wait(self, timeout: float = -1.0) -> Results:
+        """Block until done; raise IsslError if the request failed."""
+        st = ctypes.c_int32()
+        check(lib().issl_server_wait(self.server._h, self.ticket, timeout, byref(st)))
+        self.status = int(st.value)
+        if self.status != _capi.ISSL_OK:
+            raise _capi.IsslError(self.status, lib().issl_last_error().decode())
+        return self.results
+
+    def poll(self) -> bool:
+        if self.status is not None:
+            return True
+        done, st = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().issl_server_poll(self.server._h, self.ticket, byref(done), byref(st)))
+        if done.value:
+            self.status = int(st.value)
+        return bool(done.value)
+
+
+class Server:
+    """Request loop over resident indexes on one device (issl_server_*)."""
+
+    def __init__(self, device: int = 0, max_batch: int = 65536, hbm_reserve: int = 0):
+        cfg = _capi.issl_server_config(device, 0, max_batch, hbm_reserve)
+        h = c_void_p()
+        check(lib().issl_server_create(byref(cfg), byref(h)))
+        self._h = h
+        self._inflight = {}
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            check(lib().issl_server_destroy(self._h))
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_index(self, source) -> int:
+        """Admit an index from a .issl path or an IndexArrays; returns its id."""
+        iid = ctypes.c_int32()
+        if isinstance(source, str):
+            check(lib().issl_server_add_index(self._h, source.encode(), byref(iid)))
+        else:
+            ix = source
+            check(lib().issl_server_add_index_arrays(
+                self._h, ix.n_sites, ix.seq_len, ix.n_slices, _ptr(ix.sigs), _ptr(ix.masks),
+                _ptr(ix.list_sizes), _ptr(ix.entries), byref(iid)))
+        return int(iid.value)
+
+    def remove_index(self, index_id: int) -> None:
+        check(lib().issl_server_remove_index(self._h, index_id))
+
+    def submit(self, index_id: int, guides, method: int = METHOD_MIT_AND_CFD,
+               threshold: float = 0.0, form: int = FORM_BATCHED, precision: int = PREC_FP32C,
+               flags: int = 0, chunk: int = 0, max_dist: int = 4) -> Request:
+        g = np.ascontiguousarray(guides, dtype=np.uint64)
+        n = g.shape[0]
+        r = Results(np.zeros(n), np.zeros(n), np.zeros(n, np.uint64), np.zeros(n, np.uint64),
+                    np.zeros(n, np.uint8), np.zeros(n), np.zeros(n))
+        res = _capi.issl_results(_ptr(r.mit), _ptr(r.cfd), _ptr(r.occ_count),
+                                 _ptr(r.site_count), _ptr(r.passed), _ptr(r.mit_sum),
+                                 _ptr(r.cfd_sum))
+        p = Index.params(method, threshold, form, precision, flags, chunk, max_dist)
+        t = c_uint64()
+        check(lib().issl_server_submit(self._h, index_id, _ptr(g), n, byref(p), byref(res),
+                                       byref(t)))
+        return Request(self, int(t.value), r)
+
+    def stats(self, reset: bool = False) -> _capi.issl_server_stats:
+        out = _capi.issl_server_stats()
+        check(lib().issl_server_get_stats(self._h, 1 if reset else 0, byref(out)))
+        return out
+
+
+def device_memory(device: int = 0):
+    """(free, total) HBM bytes of a device."""
+    fr, tot = c_uint64(), c_uint64()
+    check(lib().issl_device_memory(device, byref(fr), byref(tot)))
+    return int(fr.value), int(tot.value)

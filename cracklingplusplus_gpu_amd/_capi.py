@@ -70,15 +70,16 @@ class issl_synth_cfg(Structure):
 
 
 class issl_server_config(Structure):
-    _fields_ = [("device", c_int32), ("slots", c_int32), ("max_guides_per_launch", c_uint64),
-                ("memory_budget", c_uint64), ("params", issl_params)]
+    _fields_ = [("device", c_int32), ("pad", c_int32), ("max_batch", c_uint64),
+                ("hbm_reserve", c_uint64)]
 
 
 class issl_server_stats(Structure):
-    _fields_ = [("requests_done", c_uint64), ("guides_done", c_uint64),
-                ("launches", c_uint64), ("coalesced_requests", c_uint64),
-                ("busy_seconds", c_double), ("peak_inflight_guides", c_uint64),
-                ("admission_waits", c_uint64)]
+    _fields_ = [("requests_done", c_uint64), ("requests_failed", c_uint64),
+                ("guides_done", c_uint64), ("launches", c_uint64), ("queued", c_uint64),
+                ("resident_bytes", c_uint64), ("hbm_free", c_uint64), ("hbm_total", c_uint64),
+                ("p50_ms", c_double), ("p99_ms", c_double), ("max_ms", c_double),
+                ("mean_ms", c_double), ("mean_batch_guides", c_double)]
 
 
 # (name, restype, argtypes) of every exported entry point.
@@ -120,6 +121,20 @@ SIGNATURES = [
     ("issl_count_sorted_sites", c_int, [c_char_p, c_uint64, c_uint64, c_uint64, c_void_p,
                                         c_void_p, c_uint64, POINTER(c_uint64)]),
     ("issl_score_tables", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    # issl_server.h
+    ("issl_server_create", c_int, [POINTER(issl_server_config), POINTER(c_void_p)]),
+    ("issl_server_destroy", c_int, [c_void_p]),
+    ("issl_server_add_index", c_int, [c_void_p, c_char_p, POINTER(c_int32)]),
+    ("issl_server_add_index_arrays", c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_void_p,
+                                             c_void_p, c_void_p, c_void_p, POINTER(c_int32)]),
+    ("issl_server_remove_index", c_int, [c_void_p, c_int32]),
+    ("issl_server_submit", c_int, [c_void_p, c_int32, c_void_p, c_uint64, POINTER(issl_params),
+                                   POINTER(issl_results), POINTER(c_uint64)]),
+    ("issl_server_wait", c_int, [c_void_p, c_uint64, c_double, POINTER(c_int32)]),
+    ("issl_server_poll", c_int, [c_void_p, c_uint64, POINTER(c_int32), POINTER(c_int32)]),
+    ("issl_server_get_stats", c_int, [c_void_p, c_int, POINTER(issl_server_stats)]),
+    ("issl_device_memory", c_int, [c_int, POINTER(c_uint64), POINTER(c_uint64)]),
+    ("issl_index_bytes_needed", c_uint64, [c_uint64, c_uint64, c_uint64]),
     # issl_synth.h
     ("issl_synth", c_int, [POINTER(issl_synth_cfg), c_void_p, c_void_p, c_void_p, c_void_p,
                            c_void_p, c_int]),

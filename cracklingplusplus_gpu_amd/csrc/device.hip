@@ -23,6 +23,7 @@
 
 #include "device.hpp"
 #include "issl_index.h"
+#include "issl_server.h"
 
 namespace issl {
 
@@ -553,6 +554,24 @@ extern "C" int issl_device_count(void) {
 }
 
 extern "C" const char *issl_version(void) { return "issl-mi355x 0.1.0 (gfx950)"; }
+
+extern "C" int issl_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(ISSL_ERR_ARG, "null argument");
+    DeviceGuard guard(device);
+    size_t fr = 0, tot = 0;
+    ISSL_HIP(hipMemGetInfo(&fr, &tot));
+    *free_bytes = fr;
+    *total_bytes = tot;
+    return ok();
+}
+
+extern "C" uint64_t issl_index_bytes_needed(uint64_t n_sites, uint64_t n_slices, uint64_t n_lists) {
+    // signatures, postings, signature-inline words, list starts, slice meta,
+    // MIT (f64 + f32), CFD (f64 + f32), distinct bitmap
+    return n_sites * 8 + 2 * n_slices * n_sites * 8 + (n_lists + n_slices) * 4 +
+           n_slices * sizeof(SliceMeta) + kMitEntries * 12 + kCfdTable * 12 +
+           (n_sites / 64 + 1) * 8;
+}
 
 extern "C" int issl_encode(const char *text, uint64_t n, uint64_t stride, uint64_t *out) {
     if ((n && (!text || !out)) || stride < 20) return fail(ISSL_ERR_ARG, "issl_encode: bad arguments");

@@ -1,0 +1,130 @@
+"""Serving loop (issl_server_*): isolation under concurrent load, continuous
+batching, admission by free memory, failure containment (MI355X).
+
+Results of a served request must be bit-identical to scoring the same guides
+alone through issl_score with the same parameters: each guide's result depends
+on the guide only, so merging requests must be invisible.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import cracklingplusplus_gpu_amd as issl
+from conftest import golden_index, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.fixture(scope="module")
+def served(gpu_available):
+    names = ["tiny4k_5x4", "mid50k_10x10"]
+    hx = {n: golden_index(n) for n in names}
+    gd = {n: load_golden(n) for n in names}
+    srv = issl.Server(max_batch=4096)
+    ids = {n: srv.add_index(hx[n]) for n in names}
+    direct = {n: issl.Index.from_arrays(hx[n]) for n in names}
+    yield srv, ids, direct, gd
+    srv.close()
+    for ix in direct.values():
+        ix.close()
+
+
+def same(a, b):
+    return ((bits(a.mit) == bits(b.mit)).all() and (bits(a.cfd) == bits(b.cfd)).all()
+            and (a.occ_count == b.occ_count).all() and (a.site_count == b.site_count).all()
+            and (a.passed == b.passed).all())
+
+
+@pytest.mark.parametrize("precision", [issl.PREC_FP32C, issl.PREC_FP64])
+def test_concurrent_requests_match_direct_scoring(served, precision):
+    srv, ids, direct, gd = served
+    rng = np.random.default_rng(11)
+    jobs = []
+    for k in range(48):
+        name = ["tiny4k_5x4", "mid50k_10x10"][k % 2]
+        g = gd[name]["guides"]
+        lo = int(rng.integers(0, len(g)))
+        n = int(rng.integers(1, 300))
+        t = [0.0, 75.0][(k // 2) % 2]
+        jobs.append((name, np.resize(g[lo:], n) if lo + n > len(g) else g[lo:lo + n], t))
+    got = [None] * len(jobs)
+
+    def client(idx):
+        reqs = []
+        for j in idx:
+            name, g, t = jobs[j]
+            reqs.append((j, srv.submit(ids[name], g, threshold=t, precision=precision)))
+        for j, r in reqs:
+            got[j] = r.wait(timeout=60.0)
+
+    threads = [threading.Thread(target=client, args=(list(range(i, len(jobs), 4)),))
+               for i in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    for j, (name, g, t) in enumerate(jobs):
+        want = direct[name].score(g, threshold=t, form=issl.FORM_BATCHED, precision=precision)
+        assert got[j] is not None and same(got[j], want), j
+
+
+def test_queued_requests_are_merged(served):
+    srv, ids, direct, gd = served
+    srv.stats(reset=True)
+    g = gd["mid50k_10x10"]["guides"]
+    reqs = [srv.submit(ids["mid50k_10x10"], g[i:i + 10]) for i in range(0, len(g), 10)]
+    for r in reqs:
+        r.wait(timeout=60.0)
+    st = srv.stats(reset=True)
+    assert st.requests_done == len(reqs) and st.requests_failed == 0
+    assert st.launches < len(reqs)           # at least some requests shared a launch
+    assert st.guides_done == len(g)
+    assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
+    assert st.resident_bytes > 0 and st.hbm_total >= st.hbm_free
+
+
+def test_bad_request_fails_alone(served):
+    srv, ids, direct, gd = served
+    g = gd["tiny4k_5x4"]["guides"][:50]
+    good = srv.submit(ids["tiny4k_5x4"], g)
+    with pytest.raises(issl.IsslError):
+        srv.submit(ids["tiny4k_5x4"], g, max_dist=3)
+    with pytest.raises(issl.IsslError):
+        srv.submit(12345, g)
+    r = good.wait(timeout=60.0)
+    assert same(r, direct["tiny4k_5x4"].score(g, form=issl.FORM_BATCHED,
+                                              precision=issl.PREC_FP32C))
+
+
+def test_admission_by_free_memory(served):
+    srv, ids, direct, gd = served
+    free, total = issl.device_memory(0)
+    assert 0 < free <= total
+    hx = golden_index("tiny4k_5x4")
+    with issl.Server(hbm_reserve=total) as tight:
+        with pytest.raises(issl.IsslError, match="admission"):
+            tight.add_index(hx)
+    with issl.Server(hbm_reserve=1 << 20) as roomy:
+        iid = roomy.add_index(hx)
+        r = roomy.submit(iid, gd["tiny4k_5x4"]["guides"][:20]).wait(timeout=60.0)
+        assert r.mit.shape == (20,)
+
+
+def test_remove_index_fails_only_its_requests(gpu_available):
+    hx = golden_index("tiny4k_5x4")
+    gd = load_golden("tiny4k_5x4")
+    g = gd["guides"]
+    with issl.Server() as srv:
+        a = srv.add_index(hx)
+        b = srv.add_index(hx)
+        keep = [srv.submit(b, g[:100]) for _ in range(5)]
+        srv.remove_index(a)
+        with pytest.raises(issl.IsslError):
+            srv.submit(a, g[:10])
+        for r in keep:
+            r.wait(timeout=60.0)
