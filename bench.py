@@ -172,13 +172,33 @@ def serving_main(args) -> int:
         res = serve_poisson(issl, srv, ids, pools, rate, n_req, size_of, rng)
         log(f"  p50 {res['p50_ms']} ms  p99 {res['p99_ms']} ms  {res['guides_per_s']:,.0f} guides/s")
         out.append(res)
+    # Baseline: the PLAIN kernel, one request at a time (closed loop), same size mix.
+    plain_lat, plain_guides = [], 0
+    t0 = time.perf_counter()
+    for i in range(args.plain_requests):
+        k = int(rng.integers(0, len(ids)))
+        n = size_of(rng)
+        lo = int(rng.integers(0, max(1, pools[k].shape[0] - n)))
+        g = pools[k][lo:lo + n] if lo + n <= pools[k].shape[0] else np.resize(pools[k][lo:], n)
+        t1 = time.perf_counter()
+        srv.submit(ids[k], g, form=issl.FORM_PLAIN, precision=issl.PREC_FP64).wait(timeout=300.0)
+        plain_lat.append((time.perf_counter() - t1) * 1e3)
+        plain_guides += n
+    wall = time.perf_counter() - t0
+    plain_lat = np.array(plain_lat)
+    plain = {"requests": len(plain_lat), "guides_per_s": round(plain_guides / wall, 1),
+             "p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(plain_lat, 99)), 3),
+             "mode": "PLAIN form, one request at a time"}
+    log(f"plain one-at-a-time: p50 {plain['p50_ms']} ms  p99 {plain['p99_ms']} ms  "
+        f"{plain['guides_per_s']:,.0f} guides/s")
     line = {"metric": "serving latency, two resident indexes (100 M + 300 M sites)",
             "unit": "ms", "data": "synthetic", "n_gpus": 1,
             "config": {"workload": "serve_genome100m+300m", "sizes": "log-uniform 1..50000",
                        "arrivals": "Poisson", "method": "mitAndCfd", "threshold": 0,
                        "form": "batched", "precision": "fp32c", "max_batch": 65536,
                        "resident_GB": round(st.resident_bytes / 1e9, 2)},
-            "rates": out}
+            "rates": out, "plain_one_at_a_time": plain}
     print(json.dumps(line), flush=True)
     srv.close()
     return 0
@@ -203,6 +223,8 @@ def main() -> int:
                     help="only the two-index serving workload (100 M + 300 M sites)")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[10, 50, 100, 250, 500])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--plain-requests", type=int, default=100,
+                    help="closed-loop PLAIN requests in the serving workload's baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="minimum CPU scoring time (the batch is repeated)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -371,8 +371,9 @@ class Request:
 class Server:
     """Request loop over resident indexes on one device (issl_server_*)."""
 
-    def __init__(self, device: int = 0, max_batch: int = 65536, hbm_reserve: int = 0):
-        cfg = _capi.issl_server_config(device, 0, max_batch, hbm_reserve)
+    def __init__(self, device: int = 0, max_batch: int = 65536, hbm_reserve: int = 0,
+                 slots: int = 3):
+        cfg = _capi.issl_server_config(device, slots, max_batch, hbm_reserve)
         h = c_void_p()
         check(lib().issl_server_create(byref(cfg), byref(h)))
         self._h = h

@@ -70,14 +70,14 @@ class issl_synth_cfg(Structure):
 
 
 class issl_server_config(Structure):
-    _fields_ = [("device", c_int32), ("pad", c_int32), ("max_batch", c_uint64),
+    _fields_ = [("device", c_int32), ("slots", c_int32), ("max_batch", c_uint64),
                 ("hbm_reserve", c_uint64)]
 
 
 class issl_server_stats(Structure):
     _fields_ = [("requests_done", c_uint64), ("requests_failed", c_uint64),
                 ("guides_done", c_uint64), ("launches", c_uint64), ("queued", c_uint64),
-                ("resident_bytes", c_uint64), ("hbm_free", c_uint64), ("hbm_total", c_uint64),
+                ("peak_inflight", c_uint64), ("resident_bytes", c_uint64), ("hbm_free", c_uint64), ("hbm_total", c_uint64),
                 ("p50_ms", c_double), ("p99_ms", c_double), ("max_ms", c_double),
                 ("mean_ms", c_double), ("mean_batch_guides", c_double)]
 

@@ -783,23 +783,23 @@ int score_on_device(issl_index *ix, const uint64_t *d_guides, uint64_t n, const 
     return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, tn, o, distinct, ix->ws, st);
 }
 
-// Byte layout of the per-call device outputs for n guides.
-struct OutLayout {
-    size_t mit, cfd, msum, csum, occ, sites, pass, total;
-    explicit OutLayout(uint64_t n) {
-        const size_t d = n * 8;
-        mit = 0;
-        cfd = mit + d;
-        msum = cfd + d;
-        csum = msum + d;
-        occ = csum + d;
-        sites = occ + d;
-        pass = sites + d;
-        total = pass + ((n + 255) / 256) * 256;
-    }
-};
-
 }  // namespace
+
+namespace issl {
+
+int params_to_scalars(const issl_index *ix, const issl_params *p, ScoreScalars *sc) {
+    return check_params(ix, p, sc);
+}
+
+int score_with(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_params *p,
+               const ScoreScalars &sc, const issl_tuning &tn, const DevResults &o, Workspace &ws,
+               hipStream_t st) {
+    uint64_t *distinct = (p->flags & ISSL_FLAG_DISTINCT) ? ix->d_distinct : nullptr;
+    if (p->form == ISSL_FORM_PLAIN) return launch_plain(ix, d_guides, n, sc, o, distinct, nullptr, nullptr, st);
+    return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, tn, o, distinct, ws, st);
+}
+
+}  // namespace issl
 
 extern "C" int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
                           const issl_params *params, issl_results *out) {

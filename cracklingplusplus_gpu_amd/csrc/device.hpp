@@ -136,6 +136,42 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
                    int precision, int chunk, const issl_tuning &tn, const DevResults &out,
                    uint64_t *distinct, Workspace &ws, hipStream_t stream);
 int build_inline_layout(issl_index *ix);
+
+// Parameter check and the scalars the kernels take (the checks of issl_score).
+int params_to_scalars(const issl_index *ix, const issl_params *p, ScoreScalars *sc);
+// One scoring launch sequence on `stream` with an explicit workspace and
+// tuning; takes no lock, so callers with their own workspace and stream may
+// run concurrently on one index (the index arrays are read-only).
+int score_with(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_params *p,
+               const ScoreScalars &sc, const issl_tuning &tn, const DevResults &o, Workspace &ws,
+               hipStream_t stream);
+
+// Byte layout of the per-call result columns for n guides.
+struct OutLayout {
+    size_t mit, cfd, msum, csum, occ, sites, pass, total;
+    explicit OutLayout(uint64_t n) {
+        const size_t d = n * 8;
+        mit = 0;
+        cfd = mit + d;
+        msum = cfd + d;
+        csum = msum + d;
+        occ = csum + d;
+        sites = occ + d;
+        pass = sites + d;
+        total = pass + ((n + 255) / 256) * 256;
+    }
+    DevResults bind(char *base) const {
+        DevResults o;
+        o.mit = reinterpret_cast<double *>(base + mit);
+        o.cfd = reinterpret_cast<double *>(base + cfd);
+        o.mit_sum = reinterpret_cast<double *>(base + msum);
+        o.cfd_sum = reinterpret_cast<double *>(base + csum);
+        o.occ = reinterpret_cast<uint64_t *>(base + occ);
+        o.sites = reinterpret_cast<uint64_t *>(base + sites);
+        o.pass = base + pass;
+        return o;
+    }
+};
 // Launch shape used before any tuning.
 issl_tuning default_tuning();
 // Event pair for the dominant kernel of a launch when timing is enabled

@@ -8,10 +8,16 @@
  *
  *   * admission by free memory: an index is admitted only if the device's
  *     free HBM covers its size plus the configured reserve;
- *   * continuous batching: one worker thread per server takes the oldest
- *     queued request and merges every queued request for the same index and
- *     the same parameters into one scoring launch, up to max_batch guides;
- *     requests arriving during a launch join the next one;
+ *   * continuous batching into slots: a launcher thread fills each free
+ *     slot (a stream with its own buffers) with up to max_batch guides from
+ *     the oldest queued request and every queued request for the same index
+ *     and parameters, splitting requests larger than a slot; up to `slots`
+ *     launches are in flight at once, so requests arriving while others run
+ *     are copied in and queued on the device as slots free up, and a
+ *     completion thread returns results as each launch finishes;
+ *   * admission by memory also for batches: a slot grows only if free HBM
+ *     covers the growth plus the reserve, else its requests fail with
+ *     ISSL_ERR_NOMEM;
  *   * isolation: results of a request never depend on what it was merged
  *     with (each guide's result depends on the guide alone), and a failing
  *     launch fails only the requests in it;
@@ -34,9 +40,11 @@ typedef struct issl_server issl_server;
 
 typedef struct {
     int32_t  device;       /* HIP device ordinal */
-    int32_t  pad;
-    uint64_t max_batch;    /* guides per merged launch (a larger request runs alone); 0 = 65536 */
-    uint64_t hbm_reserve;  /* bytes of HBM to leave free when admitting an index */
+    int32_t  slots;        /* launches in flight at once (own stream and buffers each);
+                              0 = 3, at most 16 */
+    uint64_t max_batch;    /* guides per launch; larger requests are split; 0 = 65536 */
+    uint64_t hbm_reserve;  /* bytes of HBM left free when admitting an index or growing
+                              a slot */
 } issl_server_config;
 
 typedef struct {
@@ -44,7 +52,8 @@ typedef struct {
     uint64_t requests_failed;
     uint64_t guides_done;
     uint64_t launches;         /* scoring launches (merged batches) */
-    uint64_t queued;           /* requests waiting now */
+    uint64_t queued;           /* requests with guides not yet launched, now */
+    uint64_t peak_inflight;    /* most launches in flight at once */
     uint64_t resident_bytes;   /* HBM held by the admitted indexes */
     uint64_t hbm_free, hbm_total;
     double   p50_ms, p99_ms, max_ms, mean_ms;   /* submit -> results in host buffers */

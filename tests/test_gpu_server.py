@@ -128,3 +128,23 @@ def test_remove_index_fails_only_its_requests(gpu_available):
             srv.submit(a, g[:10])
         for r in keep:
             r.wait(timeout=60.0)
+
+
+def test_large_requests_split_over_slots_match_direct(gpu_available):
+    """A request larger than a slot runs as several launches; others interleave."""
+    hx = golden_index("mid50k_10x10")
+    g = load_golden("mid50k_10x10")["guides"]
+    with issl.Index.from_arrays(hx) as ix, issl.Server(max_batch=96, slots=3) as srv:
+        iid = srv.add_index(hx)
+        big = srv.submit(iid, g)                       # 500 guides -> 6 launches
+        small = [srv.submit(iid, g[i:i + 7], threshold=75.0) for i in range(0, 70, 7)]
+        r = big.wait(timeout=60.0)
+        assert same(r, ix.score(g, form=issl.FORM_BATCHED, precision=issl.PREC_FP32C))
+        for k, q in enumerate(small):
+            lo = 7 * k
+            want = ix.score(g[lo:lo + 7], threshold=75.0, form=issl.FORM_BATCHED,
+                            precision=issl.PREC_FP32C)
+            assert same(q.wait(timeout=60.0), want), k
+        st = srv.stats()
+        assert st.launches >= 6 and st.requests_failed == 0
+        assert st.peak_inflight >= 2      # launches overlapped on the device
