@@ -73,19 +73,29 @@ def test_concurrent_requests_match_direct_scoring(served, precision):
         assert got[j] is not None and same(got[j], want), j
 
 
-def test_queued_requests_are_merged(served):
-    srv, ids, direct, gd = served
-    srv.stats(reset=True)
-    g = gd["mid50k_10x10"]["guides"]
-    reqs = [srv.submit(ids["mid50k_10x10"], g[i:i + 10]) for i in range(0, len(g), 10)]
-    for r in reqs:
-        r.wait(timeout=60.0)
-    st = srv.stats(reset=True)
-    assert st.requests_done == len(reqs) and st.requests_failed == 0
-    assert st.launches < len(reqs)           # at least some requests shared a launch
-    assert st.guides_done == len(g)
-    assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
-    assert st.resident_bytes > 0 and st.hbm_total >= st.hbm_free
+def test_queued_requests_are_merged(gpu_available):
+    """Requests queued behind a busy slot share the next launch."""
+    hx = golden_index("mid50k_10x10")
+    g = load_golden("mid50k_10x10")["guides"]
+    with issl.Index.from_arrays(hx) as ix, issl.Server(slots=1, max_batch=1 << 16) as srv:
+        iid = srv.add_index(hx)
+        srv.submit(iid, g).wait(timeout=60.0)        # warm the slot
+        srv.stats(reset=True)
+        busy = srv.submit(iid, np.tile(g, 40))       # keeps the only slot busy
+        reqs = [srv.submit(iid, g[i:i + 10]) for i in range(0, len(g), 10)]
+        busy.wait(timeout=60.0)
+        for r in reqs:
+            r.wait(timeout=60.0)
+        st = srv.stats(reset=True)
+        assert st.requests_done == len(reqs) + 1 and st.requests_failed == 0
+        assert st.launches < len(reqs)               # the small requests shared launches
+        assert st.guides_done == 41 * len(g)
+        assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
+        assert st.resident_bytes > 0 and st.hbm_total >= st.hbm_free
+        want = ix.score(g, form=issl.FORM_BATCHED, precision=issl.PREC_FP32C)
+        for k, r in enumerate(reqs):
+            lo = 10 * k
+            assert (bits(r.results.cfd) == bits(want.cfd[lo:lo + 10])).all(), k
 
 
 def test_bad_request_fails_alone(served):
