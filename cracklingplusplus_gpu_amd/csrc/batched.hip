@@ -279,6 +279,8 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 // Candidate ring per wave: fewer than 64 pending plus kRingChk loads of 64.
 constexpr int kRingChk = 3;
 constexpr uint32_t kRing = 64 * (kRingChk + 1);
+// Hit ring per wave: fewer than 64 pending plus one check's 64.
+constexpr uint32_t kHitRing = 128;
 
 // Postings in flight per lane: DEPTH.
 template <int PREC, int DEPTH>
@@ -293,6 +295,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     // Candidate ring: tile-relative position | slice << kRelBits, and the site word.
     __shared__ uint32_t s_rpos[kTileWaves][kRing];
     __shared__ uint64_t s_rword[kTileWaves][kRing];
+    // Hit ring: first-seen candidates with packed mismatch positions and distance.
+    __shared__ uint32_t s_hpos[kTileWaves][kHitRing];
+    __shared__ uint64_t s_hword[kTileWaves][kHitRing];
+    __shared__ uint32_t s_hmis[kTileWaves][kHitRing];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Real *cfdg = static_cast<const Real *>(a.cfd);
     const Real *mitg = static_cast<const Real *>(a.mit);
@@ -377,51 +383,86 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         F2 fm{0.f, 0.f}, fc{0.f, 0.f};
         uint64_t occ_sum = 0, sites = 0;
 
-        // Candidates (dist <= 4) wait in the ring in position order and are
-        // scored 64 at a time, one per lane, so the rare scoring path runs
-        // with every lane busy.
-        uint32_t head = 0, tail = 0;  // wave-uniform ring cursors
-        auto drain = [&](uint32_t cnt) {
+        // Candidates (dist <= 4) wait in the ring in position order.  They are
+        // checked 64 at a time against the first-seen rule, one per lane; the
+        // sites that count are compacted into the hit ring with their
+        // mismatch positions, and hits are scored 64 at a time.  Both rare
+        // paths thus run with every lane busy.
+        uint32_t head = 0, tail = 0;    // candidate ring cursors (wave-uniform)
+        uint32_t hhead = 0, htail = 0;  // hit ring cursors
+        auto score_hits = [&](uint32_t cnt) {
             __builtin_amdgcn_wave_barrier();
             if (static_cast<uint32_t>(lane) < cnt) {
-                const uint32_t e = (head + lane) & (kRing - 1);
-                const uint32_t ent = s_rpos[wave][e];
-                const uint64_t wv = s_rword[wave][e];
-                const int sl = static_cast<int>(ent >> kRelBits);
+                const uint32_t e = (hhead + lane) & (kHitRing - 1);
+                const uint32_t ent = s_hpos[wave][e];
+                const uint64_t wv = s_hword[wave][e];
+                const uint32_t pk = s_hmis[wave][e];
+                const uint32_t mp[4] = {pk & 63u, (pk >> 6) & 63u, (pk >> 12) & 63u,
+                                        (pk >> 18) & 63u};
+                const int dist = static_cast<int>(pk >> 24);
                 const uint64_t sg = wv & kSigBits;
-                const uint64_t f = bfold(q ^ sg);
-                const int dist = __popcll(f);
-                uint32_t mp[4];
-                mismatches(f, mp);
-                if (first_seen(mp, sl, s_posmask)) {
-                    uint32_t occ = static_cast<uint32_t>(wv >> 40);
-                    if (occ == kOccEscape || a.mark_distinct) {
-                        const uint64_t addr =
-                            a0 + (ent & ((1u << kRelBits) - 1)) + s_delta[wave][buf][sl];
-                        const uint64_t pe = a.post[addr];
-                        occ = static_cast<uint32_t>(pe >> 32);
-                        if (a.mark_distinct)
-                            atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
-                                         (static_cast<uint32_t>(pe) >> 6),
-                                     1ULL << (static_cast<uint32_t>(pe) & 63));
-                    }
-                    occ_sum += occ;
-                    sites += 1;
-                    const Real ro = to_real<Real>(occ);
-                    Real tm = 0, tc = 0;
-                    if (a.calc_mit && dist > 0) tm = mitg[mit_rank(mp, dist)] * ro;
-                    if (a.calc_cfd) tc = cfd_product<Real>(q, sg, mp, dist, s_cfd) * ro;
-                    if constexpr (PREC == ISSL_PREC_FP32C) {
-                        fm = f2_add(fm, tm);
-                        fc = f2_add(fc, tc);
-                    } else {
-                        am += tm;
-                        ac += tc;
-                    }
+                uint32_t occ = static_cast<uint32_t>(wv >> 40);
+                if (occ == kOccEscape || a.mark_distinct) {
+                    const int sl = static_cast<int>(ent >> kRelBits);
+                    const uint64_t addr =
+                        a0 + (ent & ((1u << kRelBits) - 1)) + s_delta[wave][buf][sl];
+                    const uint64_t pe = a.post[addr];
+                    occ = static_cast<uint32_t>(pe >> 32);
+                    if (a.mark_distinct)
+                        atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
+                                     (static_cast<uint32_t>(pe) >> 6),
+                                 1ULL << (static_cast<uint32_t>(pe) & 63));
+                }
+                occ_sum += occ;
+                sites += 1;
+                const Real ro = to_real<Real>(occ);
+                Real tm = 0, tc = 0;
+                if (a.calc_mit && dist > 0) tm = mitg[mit_rank(mp, dist)] * ro;
+                if (a.calc_cfd) tc = cfd_product<Real>(q, sg, mp, dist, s_cfd) * ro;
+                if constexpr (PREC == ISSL_PREC_FP32C) {
+                    fm = f2_add(fm, tm);
+                    fc = f2_add(fc, tc);
+                } else {
+                    am += tm;
+                    ac += tc;
                 }
             }
-            head += cnt;
+            hhead += cnt;
             __builtin_amdgcn_wave_barrier();
+        };
+        auto drain = [&](uint32_t cnt) {
+            __builtin_amdgcn_wave_barrier();
+            bool hit = false;
+            uint32_t ent = 0, pk = 0;
+            uint64_t wv = 0;
+            if (static_cast<uint32_t>(lane) < cnt) {
+                const uint32_t e = (head + lane) & (kRing - 1);
+                ent = s_rpos[wave][e];
+                wv = s_rword[wave][e];
+                const int sl = static_cast<int>(ent >> kRelBits);
+                const uint64_t f = bfold(q ^ (wv & kSigBits));
+                const uint32_t dist = static_cast<uint32_t>(__popcll(f));
+                uint32_t mp[4];
+                mismatches(f, mp);
+                hit = first_seen(mp, sl, s_posmask);
+                pk = mp[0] | (mp[1] << 6) | (mp[2] << 12) | (mp[3] << 18) | (dist << 24);
+            }
+            head += cnt;
+            const uint64_t b = __ballot(hit);
+            if (b) {
+                const uint32_t r = __builtin_amdgcn_mbcnt_hi(
+                    static_cast<uint32_t>(b >> 32),
+                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+                if (hit) {
+                    const uint32_t e = (htail + r) & (kHitRing - 1);
+                    s_hpos[wave][e] = ent;
+                    s_hword[wave][e] = wv;
+                    s_hmis[wave][e] = pk;
+                }
+                htail += static_cast<uint32_t>(__popcll(b));
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (htail - hhead >= 64) score_hits(64);
         };
 
         // Per-lane slice cursor: the next slice boundary and this slice's
@@ -477,6 +518,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             }
         }
         while (tail != head) drain(std::min<uint32_t>(64, tail - head));
+        if (htail != hhead) score_hits(htail - hhead);
         // Fixed butterfly: every lane ends with the same, order-determined value.
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
