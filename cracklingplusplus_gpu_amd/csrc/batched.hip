@@ -476,30 +476,37 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             nb = s_pre[wave][buf][cur + 1];
             dl = s_delta[wave][buf][cur];
         }
+        // The filter works on 32-bit halves: positions 0..15 in the low word,
+        // 16..19 in bits 0..7 of the high word; the mask drops the occurrence
+        // bits above them and the bit carried across the halves by the shift.
+        const uint32_t ql = static_cast<uint32_t>(q), qh = static_cast<uint32_t>(q >> 32);
+        const uint64_t *site_p = a.site + dl;   // this lane's current slice, by position
         for (uint32_t base = a0; base < a1; base += 64 * DEPTH) {
             uint64_t wv[DEPTH];
             int sl[DEPTH];
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
+                // Lanes past the tile read its last posting again and mark it
+                // out of range, so every lane loads without a branch.
                 const uint32_t pos = base + 64 * d + lane;
-                sl[d] = -1;
-                wv[d] = 0;
-                if (pos < a1) {
-                    if (pos >= nb) {
-                        do {
-                            ++cur;
-                            nb = s_pre[wave][buf][cur + 1];
-                        } while (pos >= nb);
-                        dl = s_delta[wave][buf][cur];
-                    }
-                    sl[d] = cur;
-                    wv[d] = a.site[pos + dl];
+                const uint32_t pc = std::min(pos, a1 - 1);
+                if (pc >= nb) {
+                    do {
+                        ++cur;
+                        nb = s_pre[wave][buf][cur + 1];
+                    } while (pc >= nb);
+                    site_p = a.site + s_delta[wave][buf][cur];
                 }
+                sl[d] = pos < a1 ? cur : -1;
+                wv[d] = site_p[pc];
             }
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
-                bool cand = false;
-                if (sl[d] >= 0) cand = __popcll(bfold(q ^ (wv[d] & kSigBits))) <= 4;
+                const uint32_t xl = ql ^ static_cast<uint32_t>(wv[d]);
+                const uint32_t xh = qh ^ static_cast<uint32_t>(wv[d] >> 32);
+                const uint32_t fl = ((xl >> 1) | xl) & 0x55555555u;
+                const uint32_t fh = ((xh >> 1) | xh) & 0x55u;
+                const bool cand = sl[d] >= 0 && __popc(fl) + __popc(fh) <= 4;
                 const uint64_t b = __ballot(cand);
                 if (b) {
                     const uint32_t r = __builtin_amdgcn_mbcnt_hi(
