@@ -381,7 +381,8 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
 
         Real am = 0, ac = 0;
         F2 fm{0.f, 0.f}, fc{0.f, 0.f};
-        uint64_t occ_sum = 0, sites = 0;
+        uint64_t occ_sum = 0;
+        uint32_t sites = 0;   // hits of this tile in this lane (< 2^26)
 
         // Candidates (dist <= 4) wait in the ring in position order.  They are
         // checked 64 at a time against the first-seen rule, one per lane; the
@@ -530,7 +531,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             occ_sum += shfl64(occ_sum, lane ^ off);
-            sites += shfl64(sites, lane ^ off);
+            sites += static_cast<uint32_t>(__shfl_xor(static_cast<int>(sites), off));
             if constexpr (PREC == ISSL_PREC_FP32C) {
                 F2 om{__shfl_xor(fm.s, off), __shfl_xor(fm.c, off)};
                 F2 oc{__shfl_xor(fc.s, off), __shfl_xor(fc.c, off)};

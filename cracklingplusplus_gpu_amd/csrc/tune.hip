@@ -139,7 +139,7 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
         return ISSL_OK;
     };
     // (1) every guide on the persistent tile kernel: tile shape and occupancy
-    const int chunks[] = {512, 1024, 2048};
+    const int chunks[] = {1024, 2048, 4096};
     const int depths[] = {4, 8, 16};
     const int grids[] = {4, 6, 8};
     issl_tuning tiles_best{};
