@@ -28,6 +28,7 @@ METHOD_MIT, METHOD_CFD, METHOD_MIT_AND_CFD, METHOD_MIT_OR_CFD, METHOD_AVG_MIT_CF
 FORM_PLAIN, FORM_BATCHED = 0, 1
 PREC_FP64, PREC_FP32C = 0, 1
 FLAG_DISTINCT = 1
+FLAG_GRAPH = 2
 
 
 class issl_params(Structure):

@@ -466,6 +466,8 @@ void free_index(issl_index *ix) {
     ix->ws.scratch.release();
     ix->ws.pinned.release();
     for (hipEvent_t e : ix->ev_pool) (void)hipEventDestroy(e);
+    for (GraphEntry &gr : ix->graphs) (void)hipGraphExecDestroy(gr.exec);
+    if (ix->ws_done) (void)hipEventDestroy(ix->ws_done);
     if (ix->stream) (void)hipStreamDestroy(ix->stream);
     delete ix;
 }
@@ -773,6 +775,22 @@ extern "C" int issl_distinct_reset(issl_index *ix) {
     return ok();
 }
 
+namespace issl {
+
+int ws_acquire(issl_index *ix, hipStream_t st) {
+    if (ix->ws_stream && ix->ws_stream != st && ix->ws_done) ISSL_HIP(hipStreamWaitEvent(st, ix->ws_done, 0));
+    return ISSL_OK;
+}
+
+int ws_release(issl_index *ix, hipStream_t st) {
+    if (!ix->ws_done) ISSL_HIP(hipEventCreateWithFlags(&ix->ws_done, hipEventDisableTiming));
+    ISSL_HIP(hipEventRecord(ix->ws_done, st));
+    ix->ws_stream = st;
+    return ISSL_OK;
+}
+
+}  // namespace issl
+
 namespace {
 
 int score_on_device(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_params *p,
@@ -818,6 +836,7 @@ extern "C" int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
     char *pin = static_cast<char *>(ix->ws.pinned.p);
     std::memcpy(pin, guides, n * 8);
     hipStream_t st = ix->stream;
+    if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
     ISSL_HIP(hipMemcpyAsync(ix->ws.guides.p, pin, n * 8, hipMemcpyHostToDevice, st));
     char *ob = static_cast<char *>(ix->ws.out.p);
     DevResults o;
@@ -833,6 +852,7 @@ extern "C" int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
         return rc;
     char *hout = pin + n * 8;
     ISSL_HIP(hipMemcpyAsync(hout, ob, L.total, hipMemcpyDeviceToHost, st));
+    if ((rc = ws_release(ix, st)) != ISSL_OK) return rc;
     ISSL_HIP(hipStreamSynchronize(st));
     if (out->mit) std::memcpy(out->mit, hout + L.mit, n * 8);
     if (out->cfd) std::memcpy(out->cfd, hout + L.cfd, n * 8);
@@ -863,8 +883,56 @@ extern "C" int issl_score_device(issl_index *ix, const uint64_t *guides_device, 
     o.sites = out_device->site_count;
     o.pass = out_device->pass;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+    if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
+    const bool use_graph = (params->flags & ISSL_FLAG_GRAPH) && !ix->timing;
+    if (use_graph) {
+        // Replay a captured sequence made for exactly this call.  Entries made
+        // for another workspace address hold freed pointers: drop them.
+        const void *wsp = ix->ws.tiles.p;
+        for (size_t i = 0; i < ix->graphs.size();) {
+            GraphEntry &gr = ix->graphs[i];
+            if (gr.ws != wsp) {
+                (void)hipGraphExecDestroy(gr.exec);
+                ix->graphs.erase(ix->graphs.begin() + static_cast<long>(i));
+                continue;
+            }
+            if (gr.guides == guides_device && gr.n == n && gr.stream == st &&
+                std::memcmp(&gr.params, params, sizeof(issl_params)) == 0 &&
+                std::memcmp(&gr.out, out_device, sizeof(issl_results)) == 0 &&
+                std::memcmp(&gr.tuning, &ix->tuning, sizeof(issl_tuning)) == 0) {
+                ISSL_HIP(hipGraphLaunch(gr.exec, st));
+                if ((rc = ws_release(ix, st)) != ISSL_OK) return rc;
+                return ok();
+            }
+            ++i;
+        }
+    }
     rc = score_on_device(ix, guides_device, n, params, sc, o, st);
     if (rc != ISSL_OK) return rc;
+    if (use_graph) {
+        // The workspace is in place now: capture the same sequence for replay.
+        hipGraph_t graph = nullptr;
+        ISSL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = score_on_device(ix, guides_device, n, params, sc, o, st);
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc != ISSL_OK || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc != ISSL_OK ? rc : fail(ISSL_ERR_DEVICE, std::string("graph capture: ") +
+                                                                  hipGetErrorString(e));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess)
+            return fail(ISSL_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+        if (ix->graphs.size() >= 8) {  // keep the eight most recent
+            (void)hipGraphExecDestroy(ix->graphs.front().exec);
+            ix->graphs.erase(ix->graphs.begin());
+        }
+        ix->graphs.push_back(GraphEntry{guides_device, n, *params, *out_device, st, ix->tuning,
+                                        ix->ws.tiles.p, exec});
+    }
+    if ((rc = ws_release(ix, st)) != ISSL_OK) return rc;
     return ok();
 }
 
@@ -880,6 +948,7 @@ extern "C" int issl_candidates(issl_index *ix, const uint64_t *guides, uint64_t 
     if ((rc = ix->ws.guides.reserve(n * 8)) != ISSL_OK) return rc;
     if ((rc = ix->ws.out.reserve(n * 8 * 2 + 64)) != ISSL_OK) return rc;
     hipStream_t st = ix->stream;
+    if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
     ISSL_HIP(hipMemcpyAsync(ix->ws.guides.p, guides, n * 8, hipMemcpyHostToDevice, st));
     ScoreScalars sc{};
     sc.method = ISSL_METHOD_MIT_AND_CFD;

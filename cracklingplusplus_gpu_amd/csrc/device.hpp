@@ -93,6 +93,19 @@ struct Workspace {
     HostBuf pinned;
 };
 
+// A launch sequence of issl_score_device captured in a hipGraph, with
+// everything it bakes in (buffers, parameters, tuning, workspace address).
+struct GraphEntry {
+    const uint64_t *guides;
+    uint64_t n;
+    issl_params params;
+    issl_results out;
+    hipStream_t stream;
+    issl_tuning tuning;
+    const void *ws;
+    hipGraphExec_t exec;
+};
+
 }  // namespace issl
 
 struct issl_index {
@@ -124,6 +137,11 @@ struct issl_index {
     size_t ev_used = 0;
     std::mutex mu;
     issl::Workspace ws;
+    std::vector<issl::GraphEntry> graphs;   // captured sequences (ISSL_FLAG_GRAPH)
+    // The scoring workspace is shared by every call on the index: a launch on
+    // another stream than the previous one first waits for that one's work.
+    hipStream_t ws_stream = nullptr;
+    hipEvent_t ws_done = nullptr;
 };
 
 namespace issl {
@@ -136,6 +154,12 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
                    int precision, int chunk, const issl_tuning &tn, const DevResults &out,
                    uint64_t *distinct, Workspace &ws, hipStream_t stream);
 int build_inline_layout(issl_index *ix);
+
+// The index's scoring workspace is shared by all its calls: acquire makes a
+// launch on `st` wait for the previous user when that ran on another stream;
+// release records this stream as the last user (index mutex held for both).
+int ws_acquire(issl_index *ix, hipStream_t st);
+int ws_release(issl_index *ix, hipStream_t st);
 
 // Parameter check and the scalars the kernels take (the checks of issl_score).
 int params_to_scalars(const issl_index *ix, const issl_params *p, ScoreScalars *sc);

@@ -98,6 +98,7 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     if ((rc = ix->ws.guides.reserve(n * 8)) != ISSL_OK) return rc;
     if ((rc = ix->ws.out.reserve(n * 8 * 2 + 4096)) != ISSL_OK) return rc;
     hipStream_t st = ix->stream;
+    if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
     ISSL_HIP(hipMemcpyAsync(ix->ws.guides.p, sample, n * 8, hipMemcpyHostToDevice, st));
     const uint64_t *dg = static_cast<const uint64_t *>(ix->ws.guides.p);
     DevResults o;

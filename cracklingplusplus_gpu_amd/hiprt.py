@@ -21,6 +21,7 @@ def hip():
         h.hipMalloc.argtypes = [ctypes.POINTER(c_void_p), c_size_t]
         h.hipFree.argtypes = [c_void_p]
         h.hipMemcpy.argtypes = [c_void_p, c_void_p, c_size_t, c_int]
+        h.hipMemset.argtypes = [c_void_p, c_int, c_size_t]
         h.hipDeviceSynchronize.argtypes = []
         h.hipSetDevice.argtypes = [c_int]
         h.hipStreamCreate.argtypes = [ctypes.POINTER(c_void_p)]
@@ -58,6 +59,9 @@ class DeviceArray:
         _ok(hip().hipMemcpy(out.ctypes.data_as(c_void_p), self.ptr, self.nbytes, D2H),
             "hipMemcpy D2H")
         return out
+
+    def fill_bytes(self, value: int) -> None:
+        _ok(hip().hipMemset(self.ptr, value, self.nbytes), "hipMemset")
 
     def free(self):
         if self.ptr is not None and self.ptr.value:

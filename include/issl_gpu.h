@@ -65,6 +65,11 @@ extern "C" {
 /* Flags */
 #define ISSL_FLAG_DISTINCT  1u  /* also set the index's distinct-site bitmap
                                    (ISSLScoreOfftargets.cpp:202-208,375-377) */
+#define ISSL_FLAG_GRAPH     2u  /* issl_score_device only: the first call runs
+                                   the launch sequence and captures it in a
+                                   hipGraph; later calls with the same buffers,
+                                   stream, parameters and tuning replay it (no
+                                   reference counterpart) */
 
 typedef struct issl_index issl_index;
 
