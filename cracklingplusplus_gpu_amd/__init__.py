@@ -225,6 +225,26 @@ class Index:
                                            _ptr(ix.entries), device, byref(h)))
         return cls(h.value)
 
+    @classmethod
+    def build(cls, sigs, occ, masks, seq_len: int = 20, device: int = 0) -> "Index":
+        """Build the slice lists on the device from distinct sites (issl_index_build)."""
+        sigs = np.ascontiguousarray(sigs, dtype=np.uint64)
+        occ = np.ascontiguousarray(occ, dtype=np.uint32)
+        masks = np.ascontiguousarray(masks, dtype=np.uint64)
+        h = c_void_p()
+        check(lib().issl_index_build(sigs.shape[0], seq_len, masks.shape[0], _ptr(sigs),
+                                     _ptr(occ), _ptr(masks), device, byref(h)))
+        return cls(h.value)
+
+    def export(self):
+        """(list_sizes, entries) of the resident index in the .issl layout."""
+        info = self.info()
+        sizes = np.zeros(max(info.n_lists, 1), dtype=np.uint64)[: info.n_lists]
+        entries = np.zeros(max(info.n_sites * info.n_slices, 1), dtype=np.uint64)[
+            : info.n_sites * info.n_slices]
+        check(lib().issl_index_export(self._h, _ptr(sizes), _ptr(entries)))
+        return sizes, entries
+
     @property
     def handle(self):
         return self._h

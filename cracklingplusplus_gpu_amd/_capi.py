@@ -90,6 +90,9 @@ SIGNATURES = [
     ("issl_index_from_arrays", c_int, [c_uint64, c_uint64, c_uint64, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_int, POINTER(c_void_p)]),
     ("issl_index_close", c_int, [c_void_p]),
+    ("issl_index_build", c_int, [c_uint64, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p,
+                                 c_int, POINTER(c_void_p)]),
+    ("issl_index_export", c_int, [c_void_p, c_void_p, c_void_p]),
     ("issl_index_get_info", c_int, [c_void_p, POINTER(issl_index_info)]),
     ("issl_index_validate", c_int, [c_void_p, POINTER(c_uint64)]),
     ("issl_distinct_count", c_int, [c_void_p, POINTER(c_uint64)]),

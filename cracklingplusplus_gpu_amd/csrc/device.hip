@@ -11,6 +11,7 @@
 // lists its keys select: exactly the order of the reference's nested loops
 // (lib/ISSLScoringModule.cpp:231-250).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -590,85 +591,60 @@ extern "C" int issl_encode(const char *text, uint64_t n, uint64_t stride, uint64
     return ok();
 }
 
-extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
-                                      const uint64_t *sigs, const uint64_t *slice_masks,
-                                      const uint64_t *list_sizes, const uint64_t *entries,
-                                      int device, issl_index **out) {
-    if (!out) return fail(ISSL_ERR_ARG, "null output handle");
-    *out = nullptr;
-    if (n_sites >= (1ULL << 32)) return fail(ISSL_ERR_FORMAT, "ids are 32-bit: N must be < 2^32");
-    if (n_slices > static_cast<uint64_t>(kMaxSlices))
-        return fail(ISSL_ERR_ARG, "at most 64 slices are supported");
-    if ((n_sites && !sigs) || (n_slices && (!slice_masks || !list_sizes)) ||
-        (n_sites && n_slices && !entries))
-        return fail(ISSL_ERR_ARG, "null index array");
-    int ndev = issl_device_count();
-    if (ndev <= 0) return fail(ISSL_ERR_DEVICE, "no HIP device available");
-    if (device < 0 || device >= ndev) return fail(ISSL_ERR_ARG, "device ordinal out of range");
+namespace {
 
-    std::unique_ptr<issl_index, void (*)(issl_index *)> ix(new (std::nothrow) issl_index, free_index);
-    if (!ix) return fail(ISSL_ERR_NOMEM, "host allocation failed");
-    ix->device = device;
+// Slice metadata from the masks (ISSLScoringModule.cpp:100-198): positions,
+// fold masks, widths and each slice's first list start; *lstart_count is the
+// number of list starts (4^w + 1 per slice, with an end sentinel).
+int init_meta(issl_index *ix, uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+              const uint64_t *slice_masks, uint64_t *lstart_count) {
     ix->n = n_sites;
     ix->seq_len = seq_len;
     ix->S = n_slices;
     ix->masks.assign(slice_masks, slice_masks + n_slices);
-
-    // Slice metadata and per-slice list starts (ISSLScoringModule.cpp:100-198).
-    std::vector<uint32_t> lstart;
-    uint64_t sz_base = 0;
+    uint64_t base = 0, lists = 0;
     for (uint64_t s = 0; s < n_slices; ++s) {
         uint8_t pos[64];
         const int w = mask_positions(slice_masks[s], seq_len, pos);
         if (w > kMaxWidth) return fail(ISSL_ERR_FORMAT, "slice wider than 16 positions");
         SliceMeta m{};
         m.width = static_cast<uint32_t>(w);
-        m.lbase = static_cast<uint32_t>(lstart.size());
+        m.lbase = static_cast<uint32_t>(base);
         m.fold = 0;
         for (int j = 0; j < w; ++j) {
             m.pos[j] = pos[j];
             m.fold |= 1ULL << (2 * pos[j]);
         }
         const uint64_t nk = 1ULL << (2 * w);
-        if (lstart.size() + nk + 1 > 0xFFFFFFFFULL)
+        if (base + nk + 1 > 0xFFFFFFFFULL)
             return fail(ISSL_ERR_FORMAT, "too many lists for 32-bit list offsets");
-        uint64_t run = 0, longest = 0;
-        for (uint64_t k = 0; k < nk; ++k) {
-            lstart.push_back(static_cast<uint32_t>(run));
-            run += list_sizes[sz_base + k];
-            longest = std::max<uint64_t>(longest, list_sizes[sz_base + k]);
-        }
-        ix->p_max += longest;
-        if (run != n_sites)
-            return fail(ISSL_ERR_FORMAT, "slice " + std::to_string(s) + " list sizes sum to " +
-                                             std::to_string(run) + ", not N");
-        lstart.push_back(static_cast<uint32_t>(run));
-        sz_base += nk;
+        base += nk + 1;
+        lists += nk;
         ix->max_w = std::max(ix->max_w, w);
         ix->meta_h.push_back(m);
     }
-    ix->n_lists = sz_base;
-    if (n_slices && count_bad_ids(entries, n_slices * n_sites, n_sites) != 0)
-        return fail(ISSL_ERR_FORMAT, "posting entries hold ids >= N");
+    ix->n_lists = lists;
+    *lstart_count = base;
+    return ISSL_OK;
+}
 
-    DeviceGuard guard(device);
+// Device arrays of an index (contents uploaded or built by the caller), the
+// score tables, the slice metadata and a cleared distinct-site bitmap.
+int alloc_index(issl_index *ix, uint64_t lstart_count) {
+    const uint64_t n_sites = ix->n, n_slices = ix->S;
     ISSL_HIP(hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking));
     int rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_sig, n_sites)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_post, n_slices * n_sites)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_inline, n_slices * n_sites)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_lstart, lstart.size())) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_meta, std::max<uint64_t>(n_slices, 1))) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_mit64, kMitEntries)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_mit32, kMitEntries)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_cfd64, kCfdTable)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix.get(), &ix->d_cfd32, kCfdTable)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_sig, n_sites)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_post, n_slices * n_sites)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_inline, n_slices * n_sites)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_lstart, lstart_count)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_meta, std::max<uint64_t>(n_slices, 1))) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_mit64, kMitEntries)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_mit32, kMitEntries)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_cfd64, kCfdTable)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_cfd32, kCfdTable)) != ISSL_OK) return rc;
     ix->distinct_words = n_sites / 64 + 1;
-    if ((rc = dev_alloc(ix.get(), &ix->d_distinct, ix->distinct_words)) != ISSL_OK) return rc;
-
-    if ((rc = upload(ix.get(), ix->d_sig, sigs, n_sites * 8)) != ISSL_OK) return rc;
-    if ((rc = upload(ix.get(), ix->d_post, entries, n_slices * n_sites * 8)) != ISSL_OK) return rc;
-    if ((rc = upload(ix.get(), ix->d_lstart, lstart.data(), lstart.size() * 4)) != ISSL_OK) return rc;
+    if ((rc = dev_alloc(ix, &ix->d_distinct, ix->distinct_words)) != ISSL_OK) return rc;
     if (n_slices)
         ISSL_HIP(hipMemcpy(ix->d_meta, ix->meta_h.data(), n_slices * sizeof(SliceMeta),
                            hipMemcpyHostToDevice));
@@ -683,6 +659,65 @@ extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64
     ISSL_HIP(hipMemcpy(ix->d_cfd64, c64.data(), kCfdTable * 8, hipMemcpyHostToDevice));
     ISSL_HIP(hipMemcpy(ix->d_cfd32, c32.data(), kCfdTable * 4, hipMemcpyHostToDevice));
     ISSL_HIP(hipMemset(ix->d_distinct, 0, ix->distinct_words * 8));
+    return ISSL_OK;
+}
+
+int check_index_args(uint64_t n_sites, uint64_t n_slices, int device) {
+    if (n_sites >= (1ULL << 32)) return fail(ISSL_ERR_FORMAT, "ids are 32-bit: N must be < 2^32");
+    if (n_slices > static_cast<uint64_t>(kMaxSlices))
+        return fail(ISSL_ERR_ARG, "at most 64 slices are supported");
+    const int ndev = issl_device_count();
+    if (ndev <= 0) return fail(ISSL_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(ISSL_ERR_ARG, "device ordinal out of range");
+    return ISSL_OK;
+}
+
+}  // namespace
+
+extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+                                      const uint64_t *sigs, const uint64_t *slice_masks,
+                                      const uint64_t *list_sizes, const uint64_t *entries,
+                                      int device, issl_index **out) {
+    if (!out) return fail(ISSL_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int rc = check_index_args(n_sites, n_slices, device);
+    if (rc != ISSL_OK) return rc;
+    if ((n_sites && !sigs) || (n_slices && (!slice_masks || !list_sizes)) ||
+        (n_sites && n_slices && !entries))
+        return fail(ISSL_ERR_ARG, "null index array");
+    std::unique_ptr<issl_index, void (*)(issl_index *)> ix(new (std::nothrow) issl_index, free_index);
+    if (!ix) return fail(ISSL_ERR_NOMEM, "host allocation failed");
+    ix->device = device;
+    uint64_t lstart_count = 0;
+    if ((rc = init_meta(ix.get(), n_sites, seq_len, n_slices, slice_masks, &lstart_count)) != ISSL_OK)
+        return rc;
+    // Per-slice list starts from the list sizes (ISSLScoringModule.cpp:181-198).
+    std::vector<uint32_t> lstart(lstart_count);
+    uint64_t sz_base = 0;
+    for (uint64_t s = 0; s < n_slices; ++s) {
+        const SliceMeta &m = ix->meta_h[s];
+        const uint64_t nk = 1ULL << (2 * m.width);
+        uint64_t run = 0, longest = 0;
+        for (uint64_t k = 0; k < nk; ++k) {
+            lstart[m.lbase + k] = static_cast<uint32_t>(run);
+            run += list_sizes[sz_base + k];
+            longest = std::max<uint64_t>(longest, list_sizes[sz_base + k]);
+        }
+        if (run != n_sites)
+            return fail(ISSL_ERR_FORMAT, "slice " + std::to_string(s) + " list sizes sum to " +
+                                             std::to_string(run) + ", not N");
+        lstart[m.lbase + nk] = static_cast<uint32_t>(run);
+        ix->p_max += longest;
+        sz_base += nk;
+    }
+    if (n_slices && count_bad_ids(entries, n_slices * n_sites, n_sites) != 0)
+        return fail(ISSL_ERR_FORMAT, "posting entries hold ids >= N");
+
+    DeviceGuard guard(device);
+    if ((rc = alloc_index(ix.get(), lstart_count)) != ISSL_OK) return rc;
+    if ((rc = upload(ix.get(), ix->d_sig, sigs, n_sites * 8)) != ISSL_OK) return rc;
+    if ((rc = upload(ix.get(), ix->d_post, entries, n_slices * n_sites * 8)) != ISSL_OK) return rc;
+    if ((rc = upload(ix.get(), ix->d_lstart, lstart.data(), lstart.size() * 4)) != ISSL_OK) return rc;
     if ((rc = build_inline_layout(ix.get())) != ISSL_OK) return rc;
     ISSL_HIP(hipStreamSynchronize(ix->stream));
     // Release the staging buffer used for the upload.
@@ -709,6 +744,142 @@ extern "C" int issl_index_open(const char *path, int device, issl_index **out) {
     if (rc != ISSL_OK) return rc;
     return issl_index_from_arrays(n, L, S, sigs.data(), masks.data(), sizes.data(), entries.data(),
                                   device, out);
+}
+
+// ------------------------------------------------------------ device build
+
+namespace {
+
+// Slice key and id of every site (ids are array positions).
+__global__ void k_keys(const uint64_t *__restrict__ sig, uint64_t n, uint64_t fold,
+                       uint32_t *__restrict__ keys, uint32_t *__restrict__ ids) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        keys[i] = slice_key(sig[i], fold);
+        ids[i] = static_cast<uint32_t>(i);
+    }
+}
+
+// List sizes of one slice.
+__global__ void k_count(const uint32_t *__restrict__ keys, uint64_t n, uint32_t *__restrict__ count) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        atomicAdd(count + keys[i], 1u);
+}
+
+// Posting entries occ<<32 | id in list order.
+__global__ void k_entries(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ occ,
+                          uint64_t n, uint64_t *__restrict__ post) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t id = ids[i];
+        post[i] = (static_cast<uint64_t>(occ[id]) << 32) | id;
+    }
+}
+
+}  // namespace
+
+extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+                                const uint64_t *sigs, const uint32_t *occ,
+                                const uint64_t *slice_masks, int device, issl_index **out) {
+    if (!out) return fail(ISSL_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int rc = check_index_args(n_sites, n_slices, device);
+    if (rc != ISSL_OK) return rc;
+    if ((n_sites && (!sigs || !occ)) || (n_slices && !slice_masks))
+        return fail(ISSL_ERR_ARG, "null index array");
+    if (n_sites > 0x7FFFFFFFULL)
+        return fail(ISSL_ERR_ARG, "device build takes at most 2^31 - 1 sites (build on the host)");
+    std::unique_ptr<issl_index, void (*)(issl_index *)> ix(new (std::nothrow) issl_index, free_index);
+    if (!ix) return fail(ISSL_ERR_NOMEM, "host allocation failed");
+    ix->device = device;
+    uint64_t lstart_count = 0;
+    if ((rc = init_meta(ix.get(), n_sites, seq_len, n_slices, slice_masks, &lstart_count)) != ISSL_OK)
+        return rc;
+    DeviceGuard guard(device);
+    if ((rc = alloc_index(ix.get(), lstart_count)) != ISSL_OK) return rc;
+    if ((rc = upload(ix.get(), ix->d_sig, sigs, n_sites * 8)) != ISSL_OK) return rc;
+    hipStream_t st = ix->stream;
+    const uint64_t n = n_sites;
+    uint64_t max_nk = 1;
+    int max_bits = 0;
+    for (const SliceMeta &m : ix->meta_h) {
+        max_nk = std::max<uint64_t>(max_nk, 1ULL << (2 * m.width));
+        max_bits = std::max(max_bits, static_cast<int>(2 * m.width));
+    }
+    // Scratch: occurrences, keys and ids (and their sorted copies), list sizes,
+    // the longest list, and the sort / scan / reduce temporaries.
+    DevBuf b_occ, b_k, b_ks, b_i, b_is, b_cnt, b_tmp;
+    if ((rc = b_occ.reserve(std::max<uint64_t>(n, 1) * 4)) != ISSL_OK) return rc;
+    if ((rc = upload(ix.get(), b_occ.p, occ, n * 4)) != ISSL_OK) return rc;
+    const size_t kb = std::max<uint64_t>(n, 1) * 4;
+    if ((rc = b_k.reserve(kb)) != ISSL_OK || (rc = b_ks.reserve(kb)) != ISSL_OK ||
+        (rc = b_i.reserve(kb)) != ISSL_OK || (rc = b_is.reserve(kb)) != ISSL_OK ||
+        (rc = b_cnt.reserve((max_nk + 1) * 4 + 256)) != ISSL_OK)
+        return rc;
+    uint32_t *keys = static_cast<uint32_t *>(b_k.p), *keys_s = static_cast<uint32_t *>(b_ks.p);
+    uint32_t *ids = static_cast<uint32_t *>(b_i.p), *ids_s = static_cast<uint32_t *>(b_is.p);
+    uint32_t *cnt = static_cast<uint32_t *>(b_cnt.p);
+    uint32_t *dmax = reinterpret_cast<uint32_t *>(static_cast<char *>(b_cnt.p) + (max_nk + 1) * 4);
+    size_t t_sort = 0, t_scan = 0, t_max = 0;
+    ISSL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t_sort, keys, keys_s, ids, ids_s,
+                                                static_cast<int>(n), 0, std::max(max_bits, 1), st));
+    ISSL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, cnt, ix->d_lstart,
+                                              static_cast<int>(max_nk + 1), st));
+    ISSL_HIP(hipcub::DeviceReduce::Max(nullptr, t_max, cnt, dmax, static_cast<int>(max_nk), st));
+    const size_t t_bytes = std::max(t_sort, std::max(t_scan, t_max));
+    if ((rc = b_tmp.reserve(t_bytes)) != ISSL_OK) return rc;
+    const dim3 grid(4096), block(256);
+    for (uint64_t s = 0; s < n_slices; ++s) {
+        const SliceMeta &m = ix->meta_h[s];
+        const uint64_t nk = 1ULL << (2 * m.width);
+        size_t tb = t_bytes;
+        hipLaunchKernelGGL(k_keys, grid, block, 0, st, ix->d_sig, n, m.fold, keys, ids);
+        ISSL_HIP(hipMemsetAsync(cnt, 0, (nk + 1) * 4, st));
+        hipLaunchKernelGGL(k_count, grid, block, 0, st, keys, n, cnt);
+        // list starts with the end sentinel: the exclusive sum over nk + 1 counts, the last 0
+        ISSL_HIP(hipcub::DeviceScan::ExclusiveSum(b_tmp.p, tb, cnt, ix->d_lstart + m.lbase,
+                                                  static_cast<int>(nk + 1), st));
+        tb = t_bytes;
+        ISSL_HIP(hipcub::DeviceReduce::Max(b_tmp.p, tb, cnt, dmax, static_cast<int>(nk), st));
+        uint32_t longest = 0;
+        ISSL_HIP(hipMemcpyAsync(&longest, dmax, 4, hipMemcpyDeviceToHost, st));
+        // LSD radix sort is stable: ids stay ascending inside each list
+        // (ISSLCreateIndex.cpp:450-484 fills lists in id order).
+        tb = t_bytes;
+        ISSL_HIP(hipcub::DeviceRadixSort::SortPairs(b_tmp.p, tb, keys, keys_s, ids, ids_s,
+                                                    static_cast<int>(n), 0,
+                                                    std::max(static_cast<int>(2 * m.width), 1), st));
+        hipLaunchKernelGGL(k_entries, grid, block, 0, st, ids_s, static_cast<const uint32_t *>(b_occ.p),
+                           n, ix->d_post + s * n);
+        ISSL_HIP(hipGetLastError());
+        ISSL_HIP(hipStreamSynchronize(st));
+        ix->p_max += longest;
+    }
+    if ((rc = build_inline_layout(ix.get())) != ISSL_OK) return rc;
+    ISSL_HIP(hipStreamSynchronize(ix->stream));
+    ix->ws.pinned.release();
+    *out = ix.release();
+    return ok();
+}
+
+extern "C" int issl_index_export(issl_index *ix, uint64_t *list_sizes, uint64_t *entries) {
+    if (!ix || (ix->n_lists && !list_sizes) || (ix->S * ix->n && !entries))
+        return fail(ISSL_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    uint64_t lstart_count = 0;
+    for (const SliceMeta &m : ix->meta_h) lstart_count += (1ULL << (2 * m.width)) + 1;
+    std::vector<uint32_t> lstart(lstart_count);
+    ISSL_HIP(hipMemcpy(lstart.data(), ix->d_lstart, lstart_count * 4, hipMemcpyDeviceToHost));
+    uint64_t k = 0;
+    for (const SliceMeta &m : ix->meta_h) {
+        const uint64_t nk = 1ULL << (2 * m.width);
+        for (uint64_t j = 0; j < nk; ++j) list_sizes[k++] = lstart[m.lbase + j + 1] - lstart[m.lbase + j];
+    }
+    if (ix->S * ix->n)
+        ISSL_HIP(hipMemcpy(entries, ix->d_post, ix->S * ix->n * 8, hipMemcpyDeviceToHost));
+    return ok();
 }
 
 extern "C" int issl_index_close(issl_index *ix) {

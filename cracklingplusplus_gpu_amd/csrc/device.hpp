@@ -55,18 +55,26 @@ constexpr uint32_t kOccEscape = 0xFFFFFF;
 // Device CFD table: 320 position factors, 16 PAM factors, then 1.0.
 constexpr int kCfdTable = 337;
 
-// Grow-only device buffer.
+// Grow-only device buffer; freed when it goes out of scope.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
     int reserve(size_t want);
     void release();
 };
 
-// Grow-only pinned host buffer.
+// Grow-only pinned host buffer; freed when it goes out of scope.
 struct HostBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() { release(); }
     int reserve(size_t want);
     void release();
 };

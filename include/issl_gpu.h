@@ -123,6 +123,21 @@ int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices
                            const uint64_t *list_sizes, const uint64_t *entries,
                            int device, issl_index **out);
 
+/* Build the slice lists on the device from distinct sites (id = array
+ * position): for every slice a stable radix sort of the ids by slice key, the
+ * list sizes and starts, and the posting entries occ<<32 | id, ids ascending
+ * inside each list -- ISSLCreateIndex.cpp:450-484 done in HBM.  Only the
+ * signatures (8 B) and occurrence counts (4 B) per site cross PCIe.  The
+ * result is the same resident index issl_index_from_arrays makes from the
+ * host-built lists. */
+int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+                     const uint64_t *sigs, const uint32_t *occ,
+                     const uint64_t *slice_masks, int device, issl_index **out);
+
+/* Copy a resident index's lists back to the host in the .issl layout:
+ * list_sizes[n_lists], entries[n_slices * n_sites] (for issl_write_index). */
+int issl_index_export(issl_index *ix, uint64_t *list_sizes, uint64_t *entries);
+
 /* Release the resident index (no counterpart: the reference frees its
  * function-local vectors at the end of run). */
 int issl_index_close(issl_index *ix);
