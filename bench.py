@@ -255,8 +255,9 @@ def main() -> int:
     log(f"generating workload {args.workload} with {threads} threads")
     sigs, occ, masks, guides, kinds = workloads.make(args.workload, threads=threads)
     log(f"sites {sigs.shape[0]:,}  guides {guides.shape[0]:,}; building slice lists")
+    t_b = time.perf_counter()
     hx = issl.build_lists(sigs, occ, masks, threads=threads)
-    del occ
+    host_build_s = time.perf_counter() - t_b
     log("uploading index")
     ix = issl.Index.from_arrays(hx)
     info = ix.info()
@@ -356,6 +357,20 @@ def main() -> int:
         lat.append((time.perf_counter() - t0) * 1e3)
     lat = np.array(lat)
 
+    # ---- index construction on the device (sites and occurrences uploaded,
+    # lists sorted in HBM) against the host builder above
+    t_b = time.perf_counter()
+    with issl.Index.build(sigs, occ, masks) as dix:
+        device_build_s = time.perf_counter() - t_b
+        dsizes, _ = dix.export(entries=False)
+    del occ
+    build = {"host_lists_s": round(host_build_s, 3), "host_threads": threads,
+             "device_build_s": round(device_build_s, 3),
+             "list_sizes_identical": bool(np.array_equal(dsizes, hx.list_sizes))}
+    log(f"index build: host lists {host_build_s:.2f} s ({threads} threads), device "
+        f"(upload + sort + postings) {device_build_s:.2f} s, sizes identical "
+        f"{build['list_sizes_identical']}")
+
     # ---- serving: Poisson arrivals of 1,000-guide requests through the request loop
     log(f"serving: {args.serve_requests} requests of 1,000 guides at {args.serve_rate:g} req/s")
     srv = issl.Server(max_batch=65536)
@@ -439,6 +454,7 @@ def main() -> int:
                        "mode": "one request at a time, host buffers in/out"},
         "serving": dict(serving, request_guides=1000,
                         mode="request loop, Poisson arrivals, host buffers in/out"),
+        "index_build": build,
         "setup_s": round(setup_s, 1),
     }
     if cpu:

@@ -236,14 +236,17 @@ class Index:
                                      _ptr(occ), _ptr(masks), device, byref(h)))
         return cls(h.value)
 
-    def export(self):
-        """(list_sizes, entries) of the resident index in the .issl layout."""
+    def export(self, entries: bool = True):
+        """(list_sizes, entries) of the resident index in the .issl layout
+        (entries None when not requested)."""
         info = self.info()
         sizes = np.zeros(max(info.n_lists, 1), dtype=np.uint64)[: info.n_lists]
-        entries = np.zeros(max(info.n_sites * info.n_slices, 1), dtype=np.uint64)[
-            : info.n_sites * info.n_slices]
-        check(lib().issl_index_export(self._h, _ptr(sizes), _ptr(entries)))
-        return sizes, entries
+        ent = None
+        if entries:
+            ent = np.zeros(max(info.n_sites * info.n_slices, 1), dtype=np.uint64)[
+                : info.n_sites * info.n_slices]
+        check(lib().issl_index_export(self._h, _ptr(sizes), _ptr(ent)))
+        return sizes, ent
 
     @property
     def handle(self):

@@ -864,8 +864,7 @@ extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_s
 }
 
 extern "C" int issl_index_export(issl_index *ix, uint64_t *list_sizes, uint64_t *entries) {
-    if (!ix || (ix->n_lists && !list_sizes) || (ix->S * ix->n && !entries))
-        return fail(ISSL_ERR_ARG, "null argument");
+    if (!ix || (ix->n_lists && !list_sizes)) return fail(ISSL_ERR_ARG, "null argument");
     std::lock_guard<std::mutex> lk(ix->mu);
     DeviceGuard guard(ix->device);
     uint64_t lstart_count = 0;
@@ -877,7 +876,7 @@ extern "C" int issl_index_export(issl_index *ix, uint64_t *list_sizes, uint64_t 
         const uint64_t nk = 1ULL << (2 * m.width);
         for (uint64_t j = 0; j < nk; ++j) list_sizes[k++] = lstart[m.lbase + j + 1] - lstart[m.lbase + j];
     }
-    if (ix->S * ix->n)
+    if (entries && ix->S * ix->n)
         ISSL_HIP(hipMemcpy(entries, ix->d_post, ix->S * ix->n * 8, hipMemcpyDeviceToHost));
     return ok();
 }

@@ -135,7 +135,8 @@ int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
                      const uint64_t *slice_masks, int device, issl_index **out);
 
 /* Copy a resident index's lists back to the host in the .issl layout:
- * list_sizes[n_lists], entries[n_slices * n_sites] (for issl_write_index). */
+ * list_sizes[n_lists], entries[n_slices * n_sites] (for issl_write_index);
+ * entries may be NULL to fetch the sizes only. */
 int issl_index_export(issl_index *ix, uint64_t *list_sizes, uint64_t *entries);
 
 /* Release the resident index (no counterpart: the reference frees its
