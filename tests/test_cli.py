@@ -122,3 +122,28 @@ def test_module_matches_reference(tmp_path, method, level, gpu_available):
         assert row[4] == chr(gd[key + "_pass"][i])
     failed = sum(1 for i, row in enumerate(res) if row[4] == "0")
     assert f"{failed:,} of {tested:,} failed here." in r.stdout
+
+
+REFTYPES_DEMO = os.path.join(ROOT, "oracle", "_ref", "issl_module_demo_reftypes")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(REFTYPES_DEMO),
+                    reason="built by `make -C oracle ref` where the reference sources are present")
+@pytest.mark.parametrize("method,level", [("mitAndCfd", "medium"), ("avgMitCfd", "ultralow")])
+def test_module_built_on_reference_types_matches(tmp_path, method, level, gpu_available):
+    """The drop-in module compiled against the pipeline's own include/util.hpp
+    (cracklingConfig, guideResults, the CODE_* constants of lib/util.cpp)
+    behaves exactly as the build on the mirror types."""
+    gd = load_golden("tiny4k_5x4")
+    seqs = issl.decode(gd["guides"])
+    rng = np.random.default_rng(9)
+    f = tmp_path / "g.txt"
+    f.write_text("".join(f"{s}AGG {int(rng.random() < 0.8)} {rng.integers(0, 4)} "
+                         f"{rng.choice(['1', '0', '?'])}\n" for s in seqs))
+    args = [TINY, str(f), method, "75", level, "1", "2"]
+    a = run([DEMO] + args)
+    b = run([REFTYPES_DEMO] + args)
+    assert a.returncode == 0 and b.returncode == 0, (a.stderr, b.stderr)
+    assert a.stdout == b.stdout
+    assert sum(1 for l in b.stdout.splitlines() if l.startswith("RESULT")) == len(seqs)
