@@ -258,6 +258,31 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a) {
     }
 }
 
+// Final per-guide results: transform 10000/(100+S), the call and the output
+// columns (lib/ISSLScoringModule.cpp:404-455).
+__device__ __forceinline__ void write_guide(const ScoreScalars &sc, const DevResults &o,
+                                            uint64_t g, double smit, double scfd, uint64_t occ,
+                                            uint64_t sites) {
+    const double vm = 10000.0 / (100.0 + smit);
+    const double vc = 10000.0 / (100.0 + scfd);
+    const double th = sc.threshold;
+    bool reject;
+    switch (sc.method) {
+    case ISSL_METHOD_MIT_AND_CFD: reject = vm < th && vc < th; break;
+    case ISSL_METHOD_MIT_OR_CFD: reject = vm < th || vc < th; break;
+    case ISSL_METHOD_AVG_MIT_CFD: reject = (vm + vc) / 2 < th; break;
+    case ISSL_METHOD_MIT: reject = vm < th; break;
+    default: reject = vc < th; break;
+    }
+    if (o.mit) o.mit[g] = sc.calc_mit ? vm : -1.0;
+    if (o.cfd) o.cfd[g] = sc.calc_cfd ? vc : -1.0;
+    if (o.mit_sum) o.mit_sum[g] = smit;
+    if (o.cfd_sum) o.cfd_sum[g] = scfd;
+    if (o.occ) o.occ[g] = occ;
+    if (o.sites) o.sites[g] = sites;
+    if (o.pass) o.pass[g] = reject ? '0' : '1';
+}
+
 // ------------------------------------------------------------ tiles
 
 struct TileArgs {
@@ -272,6 +297,8 @@ struct TileArgs {
     int mark_distinct;      // mark distinct sites here (no early exit armed)
     int calc_mit, calc_cfd;
     uint64_t *distinct;
+    ScoreScalars sc;        // a guide with one tile and no exit armed is finished here
+    DevResults out;
     BatchWs ws;
 };
 
@@ -338,7 +365,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     // prefix and the posting offset.
     struct Plan {
         uint64_t q;
-        uint32_t g, k, P, C;
+        uint32_t g, k, P, C, K;
         uint32_t pre;
         uint64_t dl;
     };
@@ -349,6 +376,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         m.q = a.guides[m.g] & kSigBits;
         m.P = a.ws.pre[row + S];
         m.C = a.ws.gchunk[m.g];
+        m.K = a.ws.gtiles[m.g];
         m.pre = 0;
         m.dl = 0;
         if (lane < S) {
@@ -556,6 +584,19 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             tp.occ = occ_sum;
             tp.sites = sites;
             a.ws.part[vc] = tp;
+            // A guide with a single tile is complete: its sum is this partial
+            // (what the finalizer would form: 0 + partial, exactly).
+            if (mc.K == 1 && !a.sc.use_cut) {
+                double smit, scfd;
+                if constexpr (PREC == ISSL_PREC_FP32C) {
+                    smit = f2_value(fm);
+                    scfd = f2_value(fc);
+                } else {
+                    smit = am;
+                    scfd = ac;
+                }
+                write_guide(a.sc, a.out, mc.g, smit, scfd, occ_sum, sites);
+            }
         }
         if (vn == kNoSlot) break;
         buf ^= 1;
@@ -603,6 +644,7 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     const int S = a.S;
     const uint32_t K = a.ws.gtiles[g];
     if (K == 0 && a.skip_untiled) return;
+    if (K == 1 && !a.sc.use_cut) return;   // finished by the wave that scored its tile
     TilePart mine{};
     if (lane < static_cast<int>(K)) mine = a.ws.part[static_cast<uint64_t>(lane) * a.n + g];
 
@@ -731,25 +773,7 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     if (lane != 0) return;
     const double smit = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
     const double scfd = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
-    const double vm = 10000.0 / (100.0 + smit);
-    const double vc = 10000.0 / (100.0 + scfd);
-    const double th = a.sc.threshold;
-    bool reject;
-    switch (a.sc.method) {
-    case ISSL_METHOD_MIT_AND_CFD: reject = vm < th && vc < th; break;
-    case ISSL_METHOD_MIT_OR_CFD: reject = vm < th || vc < th; break;
-    case ISSL_METHOD_AVG_MIT_CFD: reject = (vm + vc) / 2 < th; break;
-    case ISSL_METHOD_MIT: reject = vm < th; break;
-    default: reject = vc < th; break;
-    }
-    const DevResults &o = a.out;
-    if (o.mit) o.mit[g] = a.sc.calc_mit ? vm : -1.0;
-    if (o.cfd) o.cfd[g] = a.sc.calc_cfd ? vc : -1.0;
-    if (o.mit_sum) o.mit_sum[g] = smit;
-    if (o.cfd_sum) o.cfd_sum[g] = scfd;
-    if (o.occ) o.occ[g] = occ_sum;
-    if (o.sites) o.sites[g] = sites;
-    if (o.pass) o.pass[g] = reject ? '0' : '1';
+    write_guide(a.sc, a.out, g, smit, scfd, occ_sum, sites);
 }
 
 // ------------------------------------------------------------ direct
@@ -1008,25 +1032,7 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
             if (lane == 0) {
                 const double smit = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
                 const double scfd = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
-                const double vm = 10000.0 / (100.0 + smit);
-                const double vc = 10000.0 / (100.0 + scfd);
-                const double th = a.sc.threshold;
-                bool reject;
-                switch (a.sc.method) {
-                case ISSL_METHOD_MIT_AND_CFD: reject = vm < th && vc < th; break;
-                case ISSL_METHOD_MIT_OR_CFD: reject = vm < th || vc < th; break;
-                case ISSL_METHOD_AVG_MIT_CFD: reject = (vm + vc) / 2 < th; break;
-                case ISSL_METHOD_MIT: reject = vm < th; break;
-                default: reject = vc < th; break;
-                }
-                const DevResults &o = a.out;
-                if (o.mit) o.mit[g] = a.sc.calc_mit ? vm : -1.0;
-                if (o.cfd) o.cfd[g] = a.sc.calc_cfd ? vc : -1.0;
-                if (o.mit_sum) o.mit_sum[g] = smit;
-                if (o.cfd_sum) o.cfd_sum[g] = scfd;
-                if (o.occ) o.occ[g] = occ_sum;
-                if (o.sites) o.sites[g] = sites;
-                if (o.pass) o.pass[g] = reject ? '0' : '1';
+                write_guide(a.sc, a.out, g, smit, scfd, occ_sum, sites);
             }
         }
         if (!has_next) break;
@@ -1179,6 +1185,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     ta.calc_mit = sc.calc_mit;
     ta.calc_cfd = sc.calc_cfd;
     ta.distinct = distinct;
+    ta.sc = sc;
+    ta.out = out;
     ta.ws = b;
     // Persistent grid: grid_per_cu workgroups per CU, no more waves than slots.
     uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
