@@ -208,14 +208,44 @@ class Results:
 class Index:
     """A .issl index resident on one GPU (issl_index_open / issl_index_from_arrays)."""
 
-    def __init__(self, handle: int):
+    def __init__(self, handle: int, keep=None):
         self._h = c_void_p(handle)
+        self._keep = keep  # host arrays a streamed index reads from
 
     @classmethod
     def open(cls, path: str, device: int = 0) -> "Index":
         h = c_void_p()
         check(lib().issl_index_open(path.encode(), device, byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def open_streamed(cls, path: str, device: int = 0, slot_bytes: int = 0) -> "Index":
+        """Memory-map a .issl file and stream its slices through `slot_bytes`
+        of device slice buffers (issl_index_open_streamed; 0: as many as fit)."""
+        h = c_void_p()
+        check(lib().issl_index_open_streamed(path.encode(), device, slot_bytes, byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays_streamed(cls, ix: IndexArrays, device: int = 0,
+                             slot_bytes: int = 0) -> "Index":
+        """Streamed index over host arrays (issl_index_from_arrays_streamed); the
+        arrays are kept referenced until the index is closed."""
+        for a in (ix.sigs, ix.masks, ix.list_sizes, ix.entries):
+            if a.dtype != np.uint64 or not a.flags.c_contiguous:
+                raise ValueError("streamed index arrays must be C-contiguous uint64")
+        h = c_void_p()
+        check(lib().issl_index_from_arrays_streamed(
+            ix.n_sites, ix.seq_len, ix.n_slices, _ptr(ix.sigs), _ptr(ix.masks),
+            _ptr(ix.list_sizes), _ptr(ix.entries), device, slot_bytes, byref(h)))
+        return cls(h.value, keep=ix)
+
+    def stream_stats(self):
+        """(slice buffers, bytes copied into them, windows scored); zeros for a
+        resident index."""
+        a, b, c = c_uint64(), c_uint64(), c_uint64()
+        check(lib().issl_index_stream_stats(self._h, byref(a), byref(b), byref(c)))
+        return a.value, b.value, c.value
 
     @classmethod
     def from_arrays(cls, ix: IndexArrays, device: int = 0) -> "Index":
@@ -256,6 +286,7 @@ class Index:
         if self._h is not None and self._h.value:
             check(lib().issl_index_close(self._h))
         self._h = None
+        self._keep = None
 
     def __enter__(self):
         return self

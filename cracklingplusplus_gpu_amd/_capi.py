@@ -90,6 +90,12 @@ SIGNATURES = [
     ("issl_index_from_arrays", c_int, [c_uint64, c_uint64, c_uint64, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_int, POINTER(c_void_p)]),
     ("issl_index_close", c_int, [c_void_p]),
+    ("issl_index_open_streamed", c_int, [c_char_p, c_int, c_uint64, POINTER(c_void_p)]),
+    ("issl_index_from_arrays_streamed", c_int, [c_uint64, c_uint64, c_uint64, c_void_p, c_void_p,
+                                                c_void_p, c_void_p, c_int, c_uint64,
+                                                POINTER(c_void_p)]),
+    ("issl_index_stream_stats", c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64),
+                                        POINTER(c_uint64)]),
     ("issl_index_build", c_int, [c_uint64, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p,
                                  c_int, POINTER(c_void_p)]),
     ("issl_index_export", c_int, [c_void_p, c_void_p, c_void_p]),

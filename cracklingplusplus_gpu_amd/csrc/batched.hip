@@ -218,6 +218,8 @@ struct PlanArgs {
     int S;
     uint64_t n_sites;
     uint32_t chunk;
+    int s0, s1;                   // slice window: lists outside it are empty
+    const GuideState *state_in;   // guides that took the exit visit nothing
     BatchWs ws;
 };
 
@@ -228,8 +230,9 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a) {
     const uint64_t g = blockIdx.x * 4ULL + (threadIdx.x >> 6);
     if (g >= a.n) return;  // uniform per wave
     const uint64_t q = a.guides[g] & kSigBits;
+    const bool live = !a.state_in || (a.state_in[g].sites & kExited) == 0;
     uint32_t b = 0, len = 0;
-    if (lane < a.S) {
+    if (lane < a.S && lane >= a.s0 && lane < a.s1 && live) {
         const uint32_t lb = a.meta[lane].lbase;
         const uint32_t key = slice_key(q, a.meta[lane].fold);
         b = a.lstart[lb + key];
@@ -245,8 +248,9 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a) {
     const uint64_t run = a.S > 0 ? shfl64(incl, a.S - 1) : 0;
     if (lane < a.S) {
         a.ws.pre[g * (a.S + 1) + lane] = static_cast<uint32_t>(excl);
-        // posting index of position p in slice s: s*N + b + (p - pre[s])
-        a.ws.delta[g * a.S + lane] = static_cast<uint64_t>(lane) * a.n_sites + b - excl;
+        // posting index of position p in slice s: (s - s0)*N + b + (p - pre[s])
+        a.ws.delta[g * a.S + lane] =
+            len ? static_cast<uint64_t>(lane - a.s0) * a.n_sites + b - excl : 0;
     }
     if (lane == 0) {
         a.ws.pre[g * (a.S + 1) + a.S] = static_cast<uint32_t>(run);
@@ -298,7 +302,8 @@ struct TileArgs {
     int calc_mit, calc_cfd;
     uint64_t *distinct;
     ScoreScalars sc;        // a guide with one tile and no exit armed is finished here
-    DevResults out;
+    DevResults out;         // (when finish_single is set)
+    int finish_single;
     BatchWs ws;
 };
 
@@ -586,7 +591,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             a.ws.part[vc] = tp;
             // A guide with a single tile is complete: its sum is this partial
             // (what the finalizer would form: 0 + partial, exactly).
-            if (mc.K == 1 && !a.sc.use_cut) {
+            if (mc.K == 1 && a.finish_single) {
                 double smit, scfd;
                 if constexpr (PREC == ISSL_PREC_FP32C) {
                     smit = f2_value(fm);
@@ -621,8 +626,11 @@ struct FinArgs {
     uint64_t n;
     ScoreScalars sc;
     DevResults out;
-    uint64_t *distinct;     // mark sites up to the exit (early exit armed)
-    int skip_untiled;       // guides without tiles were finished by k_stream
+    uint64_t *distinct;           // mark sites up to the exit (early exit armed)
+    int skip_untiled;             // guides without tiles were finished by k_stream
+    int finish_single;            // guides with one tile were finished by k_tiles
+    const GuideState *state_in;   // carried state (null: start from zero)
+    GuideState *state_out;        // null: write the results
     BatchWs ws;
 };
 
@@ -644,19 +652,47 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     const int S = a.S;
     const uint32_t K = a.ws.gtiles[g];
     if (K == 0 && a.skip_untiled) return;
-    if (K == 1 && !a.sc.use_cut) return;   // finished by the wave that scored its tile
+    if (K == 1 && a.finish_single) return;  // finished by the wave that scored its tile
     TilePart mine{};
     if (lane < static_cast<int>(K)) mine = a.ws.part[static_cast<uint64_t>(lane) * a.n + g];
 
+    // Sums so far: zero, or the state carried from earlier slice windows.
     F2 fm{0.f, 0.f}, fc{0.f, 0.f};
     double dm = 0, dc = 0;
     uint64_t occ_sum = 0, sites = 0;
+    bool exited = false;
+    if (a.state_in) {
+        const GuideState st = a.state_in[g];
+        if constexpr (PREC == ISSL_PREC_FP32C) {
+            fm = F2{st.f.mh, st.f.ml};
+            fc = F2{st.f.ch, st.f.cl};
+        } else {
+            dm = st.d.m;
+            dc = st.d.c;
+        }
+        occ_sum = st.occ;
+        sites = st.sites & ~kExited;
+        exited = (st.sites & kExited) != 0;
+    }
+
+    const uint64_t q = a.guides[g] & kSigBits;
+    const uint64_t C = a.ws.gchunk[g];
+    bool lists = false;
+    auto load_lists = [&]() {  // the guide's list prefix and posting offsets
+        if (lists) return;
+        if (lane < S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
+        if (lane == 0) s_pre[wave][S] = a.ws.pre[g * (S + 1) + S];  // S may be 64
+        if (lane < S) s_delta[wave][lane] = a.ws.delta[g * S + lane];
+        __builtin_amdgcn_wave_barrier();
+        lists = true;
+    };
+
     int exit_tile = -1;
     for (uint32_t k = 0; k < K; ++k) {
         const uint64_t o = shfl64(mine.occ, k);
         const uint64_t si = shfl64(mine.sites, k);
-        F2 pm = fm, pc = fc;
-        double qm = dm, qc = dc;
+        const F2 pm = fm, pc = fc;
+        const double qm = dm, qc = dc;
         if constexpr (PREC == ISSL_PREC_FP32C) {
             fm = f2_merge(fm, F2{__shfl(mine.f.mh, k), __shfl(mine.f.ml, k)});
             fc = f2_merge(fc, F2{__shfl(mine.f.ch, k), __shfl(mine.f.cl, k)});
@@ -666,32 +702,23 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
         }
         const double vm = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
         const double vc = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
-        if (a.sc.use_cut && bexit(a.sc.method, vm, vc, a.sc.cut)) {
-            // Restore the sums before this tile and rescan it hit by hit.
-            fm = pm;
-            fc = pc;
-            dm = qm;
-            dc = qc;
-            exit_tile = static_cast<int>(k);
-            break;
+        if (!(a.sc.use_cut && bexit(a.sc.method, vm, vc, a.sc.cut))) {
+            occ_sum += o;
+            sites += si;
+            continue;
         }
-        occ_sum += o;
-        sites += si;
-    }
-
-    const uint64_t q = a.guides[g] & kSigBits;
-    if (exit_tile >= 0 || a.distinct) {
-        if (lane < S) s_pre[wave][lane] = a.ws.pre[g * (S + 1) + lane];
-        if (lane == 0) s_pre[wave][S] = a.ws.pre[g * (S + 1) + S];  // S may be 64
-        if (lane < S) s_delta[wave][lane] = a.ws.delta[g * S + lane];
-        __builtin_amdgcn_wave_barrier();
-    }
-    const uint64_t C = a.ws.gchunk[g];
-    if (exit_tile >= 0) {
+        // The total meets the exit test within tile k: restore the sums before
+        // it and add its hits one at a time, in order, to stop exactly where
+        // the reference's loop stops (lib/ISSLScoringModule.cpp:365-401).
+        fm = pm;
+        fc = pc;
+        dm = qm;
+        dc = qc;
+        load_lists();
         const Real *mitg = static_cast<const Real *>(a.mit);
         const Real *cfdg = static_cast<const Real *>(a.cfd);
         const uint64_t P = s_pre[wave][S];
-        const uint64_t a0 = static_cast<uint64_t>(exit_tile) * C;
+        const uint64_t a0 = static_cast<uint64_t>(k) * C;
         const uint64_t a1 = std::min<uint64_t>(P, a0 + C);
         int cur = 0;
         bool done = false;
@@ -725,17 +752,17 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
                 bal &= bal - 1;
                 const Real hm = __shfl(tm, h), hc = __shfl(tc, h);
                 const uint32_t ho = static_cast<uint32_t>(__shfl(static_cast<int>(occ), h));
-                double vm, vc;
+                double vm2, vc2;
                 if constexpr (PREC == ISSL_PREC_FP32C) {
                     fm = f2_add(fm, hm);
                     fc = f2_add(fc, hc);
-                    vm = f2_value(fm);
-                    vc = f2_value(fc);
+                    vm2 = f2_value(fm);
+                    vc2 = f2_value(fc);
                 } else {
                     dm += hm;
                     dc += hc;
-                    vm = dm;
-                    vc = dc;
+                    vm2 = dm;
+                    vc2 = dc;
                 }
                 occ_sum += ho;
                 sites += 1;
@@ -743,15 +770,23 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
                     atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
                                  (static_cast<uint32_t>(a.post[addr]) >> 6),
                              1ULL << (static_cast<uint32_t>(a.post[addr]) & 63));
-                if (bexit(a.sc.method, vm, vc, a.sc.cut)) {
+                if (bexit(a.sc.method, vm2, vc2, a.sc.cut)) {
                     done = true;
                     break;
                 }
             }
         }
+        if (done) {
+            exited = true;
+            exit_tile = static_cast<int>(k);
+            break;
+        }
+        // Hit by hit the sums never met the test (the tile's reduced partial
+        // rounds differently): the guide goes on with its next tile.
     }
-    // Distinct sites of the tiles before the exit tile (or all tiles).
+    // Distinct sites of the tiles before the exit tile (or of all tiles).
     if (a.distinct) {
+        load_lists();
         const uint64_t P = s_pre[wave][S];
         const uint64_t end = exit_tile >= 0 ? static_cast<uint64_t>(exit_tile) * C : P;
         int cur = 0;
@@ -771,6 +806,22 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
         }
     }
     if (lane != 0) return;
+    if (a.state_out) {
+        GuideState st;
+        if constexpr (PREC == ISSL_PREC_FP32C) {
+            st.f.mh = fm.s;
+            st.f.ml = fm.c;
+            st.f.ch = fc.s;
+            st.f.cl = fc.c;
+        } else {
+            st.d.m = dm;
+            st.d.c = dc;
+        }
+        st.occ = occ_sum;
+        st.sites = sites | (exited ? kExited : 0);
+        a.state_out[g] = st;
+        return;
+    }
     const double smit = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
     const double scfd = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
     write_guide(a.sc, a.out, g, smit, scfd, occ_sum, sites);
@@ -1124,7 +1175,7 @@ DevResults offset_results(const DevResults &o, uint64_t i) {
 // One launch sequence over n <= kMaxBatch guides.
 int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                  int precision, const issl_tuning &tn, const DevResults &out, uint64_t *distinct,
-                 Workspace &ws, hipStream_t st) {
+                 Workspace &ws, hipStream_t st, const Window &win) {
     const int S = static_cast<int>(ix->S);
     BatchWs b;
     int rc = carve(ws, n, S, &b);
@@ -1132,7 +1183,10 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     const bool fp32 = precision == ISSL_PREC_FP32C;
     const void *mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
     const void *cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
-    const bool direct = tn.heavy_kpost != 0;
+    const bool carry = win.state_in || win.state_out;
+    // Guide streams (k_stream) read the resident index only.
+    const bool direct = tn.heavy_kpost != 0 && !carry && win.s0 == 0 && win.s1 == S;
+    const int finish_single = (!carry && !sc.use_cut) ? 1 : 0;
     const bool tiles = !direct || tn.heavy_kpost != kHeavyNever;
     const uint32_t gw = static_cast<uint32_t>((n + 3) / 4);
 
@@ -1140,8 +1194,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
         StreamArgs da;
         da.guides = d_guides;
         da.n = n;
-        da.site = ix->d_inline;
-        da.post = ix->d_post;
+        da.site = win.inl;   // slice s at s * N: the window is the whole index
+        da.post = win.post;
         da.lstart = ix->d_lstart;
         da.meta = ix->d_meta;
         da.mit = mit;
@@ -1168,14 +1222,15 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
             return ISSL_OK;
         }
     } else {
-        PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk), b};
+        PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk),
+                    win.s0, win.s1, win.state_in, b};
         hipLaunchKernelGGL(k_plan, dim3(gw), dim3(256), 0, st, pa);
     }
 
     TileArgs ta;
     ta.guides = d_guides;
-    ta.site = ix->d_inline;
-    ta.post = ix->d_post;
+    ta.site = win.inl;
+    ta.post = win.post;
     ta.meta = ix->d_meta;
     ta.mit = mit;
     ta.cfd = cfd;
@@ -1187,6 +1242,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     ta.distinct = distinct;
     ta.sc = sc;
     ta.out = out;
+    ta.finish_single = finish_single;
     ta.ws = b;
     // Persistent grid: grid_per_cu workgroups per CU, no more waves than slots.
     uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
@@ -1200,8 +1256,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
 
     FinArgs fa;
     fa.guides = d_guides;
-    fa.site = ix->d_inline;
-    fa.post = ix->d_post;
+    fa.site = win.inl;
+    fa.post = win.post;
     fa.meta = ix->d_meta;
     fa.mit = mit;
     fa.cfd = cfd;
@@ -1211,6 +1267,9 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     fa.out = out;
     fa.distinct = (distinct && sc.use_cut) ? distinct : nullptr;
     fa.skip_untiled = direct ? 1 : 0;
+    fa.finish_single = finish_single;
+    fa.state_in = win.state_in;
+    fa.state_out = win.state_out;
     fa.ws = b;
     if (fp32)
         hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP32C>), dim3(gw), dim3(256), 0, st, fa);
@@ -1235,8 +1294,9 @@ issl_tuning default_tuning() {
 
 int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                    int precision, int chunk, const issl_tuning &tn_in, const DevResults &out,
-                   uint64_t *distinct, Workspace &ws, hipStream_t st) {
+                   uint64_t *distinct, Workspace &ws, hipStream_t st, const Window *win) {
     if (n == 0) return ISSL_OK;
+    if (!win && !ix->d_inline) return fail(ISSL_ERR_ARG, "this operation needs a resident index");
     if (ix->p_max >= (1ULL << 31))
         return fail(ISSL_ERR_ARG, "index lists are too long for the batched form (a guide could "
                                   "visit 2^31 postings): use the plain form");
@@ -1245,10 +1305,22 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     if (chunk > 0) tn.chunk = chunk;
     if (tn.chunk % 64 != 0 || tn.chunk > (1 << 20))
         return fail(ISSL_ERR_ARG, "chunk must be a multiple of 64 up to 2^20");
+    Window w;
+    if (win) {
+        w = *win;
+    } else {
+        w.s0 = 0;
+        w.s1 = static_cast<int>(ix->S);
+        w.post = ix->d_post;
+        w.inl = ix->d_inline;
+    }
     for (uint64_t i = 0; i < n; i += kMaxBatch) {
         const uint64_t m = std::min<uint64_t>(kMaxBatch, n - i);
+        Window wi = w;
+        if (wi.state_in) wi.state_in += i;
+        if (wi.state_out) wi.state_out += i;
         const int rc = launch_batch(ix, d_guides + i, m, sc, precision, tn, offset_results(out, i),
-                                    distinct, ws, st);
+                                    distinct, ws, st, wi);
         if (rc != ISSL_OK) return rc;
     }
     return ISSL_OK;

@@ -166,6 +166,9 @@ struct PlainArgs {
     uint64_t *distinct;
     const uint64_t *cand_off;
     uint32_t *cand_ids;
+    int s0, s1;                     // slice window; postings of slice s at (s - s0) * N
+    const GuideState *state_in;     // carried state (null: start from zero)
+    GuideState *state_out;          // null: write the results
 };
 
 constexpr int kPlainBlock = 256;
@@ -203,14 +206,29 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
     const uint64_t q = a.guides[g] & kSigBits;
     const int S = a.n_slices;
 
+    // Sums and counts so far (earlier slice windows of a streamed index); the
+    // additions below continue the same sequence, so the order is unchanged.
+    double smit = 0.0, scfd = 0.0;  // meaningful in thread 0
+    uint64_t occ_sum = 0, site_sum = 0;
+    bool was_exited = false;
+    if (a.state_in) {
+        const GuideState st = a.state_in[g];
+        smit = st.d.m;
+        scfd = st.d.c;
+        occ_sum = st.occ;
+        site_sum = st.sites & ~kExited;
+        was_exited = (st.sites & kExited) != 0;
+    }
+
     if (tid < S) {
         const uint64_t mf = a.meta[tid].fold;
         const uint32_t lb = a.meta[tid].lbase;
         const uint32_t key = slice_key(q, mf);
         const uint32_t b = a.lstart[lb + key];
         const uint32_t e = a.lstart[lb + key + 1];
-        s_beg[tid] = static_cast<uint64_t>(tid) * a.n_sites + b;
-        s_pre[tid + 1] = e - b;
+        const bool in = tid >= a.s0 && tid < a.s1 && !was_exited;
+        s_beg[tid] = in ? static_cast<uint64_t>(tid - a.s0) * a.n_sites + b : 0;
+        s_pre[tid + 1] = in ? e - b : 0;
         s_fold[tid] = mf;
     }
     __syncthreads();
@@ -222,8 +240,6 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
     __syncthreads();
     const uint64_t P = s_pre[S];
 
-    double smit = 0.0, scfd = 0.0;  // meaningful in thread 0
-    uint64_t occ_sum = 0, site_sum = 0;
     uint64_t emitted = 0;
     const uint64_t cand_base = a.cand_ids ? a.cand_off[g] : 0;
     int cur = 0;
@@ -336,17 +352,33 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
         emitted += acc;
         if (s_exited) break;
     }
-    if (tid == 0) write_result(a.out, g, a.sc, smit, scfd, occ_sum, site_sum);
+    if (tid != 0) return;
+    if (a.state_out) {
+        GuideState st;
+        st.d.m = smit;
+        st.d.c = scfd;
+        st.occ = occ_sum;
+        st.sites = site_sum | ((was_exited || s_exited) ? kExited : 0);
+        a.state_out[g] = st;
+    } else {
+        write_result(a.out, g, a.sc, smit, scfd, occ_sum, site_sum);
+    }
 }
 
 int launch_plain(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                  const DevResults &out, uint64_t *distinct, const uint64_t *cand_off,
-                 uint32_t *cand_ids, hipStream_t stream) {
+                 uint32_t *cand_ids, hipStream_t stream, const Window *win) {
     if (n == 0) return ISSL_OK;
     if (n > 0x7FFFFFFFULL) return fail(ISSL_ERR_ARG, "too many guides for one launch");
+    if (win && cand_ids) return fail(ISSL_ERR_ARG, "candidate lists need a resident index");
+    if (!win && !ix->d_post) return fail(ISSL_ERR_ARG, "this operation needs a resident index");
     PlainArgs a;
     a.sig = ix->d_sig;
-    a.post = ix->d_post;
+    a.post = win ? win->post : ix->d_post;
+    a.s0 = win ? win->s0 : 0;
+    a.s1 = win ? win->s1 : static_cast<int>(ix->S);
+    a.state_in = win ? win->state_in : nullptr;
+    a.state_out = win ? win->state_out : nullptr;
     a.lstart = ix->d_lstart;
     a.meta = ix->d_meta;
     a.mit64 = ix->d_mit64;
@@ -440,23 +472,13 @@ using namespace issl;
 
 // ============================================================ C ABI
 
-namespace {
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
+namespace issl {
 
 void free_index(issl_index *ix) {
     if (!ix) return;
     (void)hipSetDevice(ix->device);
     if (ix->stream) (void)hipStreamSynchronize(ix->stream);
+    stream_free(ix);
     void *ptrs[] = {ix->d_sig, ix->d_post, ix->d_inline, ix->d_lstart, ix->d_meta, ix->d_mit64,
                     ix->d_mit32, ix->d_cfd64, ix->d_cfd32, ix->d_distinct};
     for (void *p : ptrs)
@@ -465,6 +487,7 @@ void free_index(issl_index *ix) {
     ix->ws.out.release();
     ix->ws.tiles.release();
     ix->ws.scratch.release();
+    ix->ws.state.release();
     ix->ws.pinned.release();
     for (hipEvent_t e : ix->ev_pool) (void)hipEventDestroy(e);
     for (GraphEntry &gr : ix->graphs) (void)hipGraphExecDestroy(gr.exec);
@@ -548,7 +571,7 @@ int check_params(const issl_index *ix, const issl_params *p, ScoreScalars *sc) {
     return ISSL_OK;
 }
 
-}  // namespace
+}  // namespace issl
 
 extern "C" int issl_device_count(void) {
     int n = 0;
@@ -591,7 +614,7 @@ extern "C" int issl_encode(const char *text, uint64_t n, uint64_t stride, uint64
     return ok();
 }
 
-namespace {
+namespace issl {
 
 // Slice metadata from the masks (ISSLScoringModule.cpp:100-198): positions,
 // fold masks, widths and each slice's first list start; *lstart_count is the
@@ -630,13 +653,15 @@ int init_meta(issl_index *ix, uint64_t n_sites, uint64_t seq_len, uint64_t n_sli
 
 // Device arrays of an index (contents uploaded or built by the caller), the
 // score tables, the slice metadata and a cleared distinct-site bitmap.
-int alloc_index(issl_index *ix, uint64_t lstart_count) {
+int alloc_index(issl_index *ix, uint64_t lstart_count, bool postings) {
     const uint64_t n_sites = ix->n, n_slices = ix->S;
     ISSL_HIP(hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking));
     int rc;
     if ((rc = dev_alloc(ix, &ix->d_sig, n_sites)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix, &ix->d_post, n_slices * n_sites)) != ISSL_OK) return rc;
-    if ((rc = dev_alloc(ix, &ix->d_inline, n_slices * n_sites)) != ISSL_OK) return rc;
+    if (postings) {
+        if ((rc = dev_alloc(ix, &ix->d_post, n_slices * n_sites)) != ISSL_OK) return rc;
+        if ((rc = dev_alloc(ix, &ix->d_inline, n_slices * n_sites)) != ISSL_OK) return rc;
+    }
     if ((rc = dev_alloc(ix, &ix->d_lstart, lstart_count)) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix, &ix->d_meta, std::max<uint64_t>(n_slices, 1))) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix, &ix->d_mit64, kMitEntries)) != ISSL_OK) return rc;
@@ -662,6 +687,29 @@ int alloc_index(issl_index *ix, uint64_t lstart_count) {
     return ISSL_OK;
 }
 
+// Per-slice list starts from the list sizes (ISSLScoringModule.cpp:181-198);
+// sizes[s] holds slice s's 4^w list sizes.  Also sets p_max.
+int build_lstart(issl_index *ix, const std::vector<const uint64_t *> &sizes,
+                 std::vector<uint32_t> &lstart) {
+    for (uint64_t s = 0; s < ix->S; ++s) {
+        const SliceMeta &m = ix->meta_h[s];
+        const uint64_t nk = 1ULL << (2 * m.width);
+        const uint64_t *sz = sizes[s];
+        uint64_t run = 0, longest = 0;
+        for (uint64_t k = 0; k < nk; ++k) {
+            lstart[m.lbase + k] = static_cast<uint32_t>(run);
+            run += sz[k];
+            longest = std::max<uint64_t>(longest, sz[k]);
+        }
+        if (run != ix->n)
+            return fail(ISSL_ERR_FORMAT, "slice " + std::to_string(s) + " list sizes sum to " +
+                                             std::to_string(run) + ", not N");
+        lstart[m.lbase + nk] = static_cast<uint32_t>(run);
+        ix->p_max += longest;
+    }
+    return ISSL_OK;
+}
+
 int check_index_args(uint64_t n_sites, uint64_t n_slices, int device) {
     if (n_sites >= (1ULL << 32)) return fail(ISSL_ERR_FORMAT, "ids are 32-bit: N must be < 2^32");
     if (n_slices > static_cast<uint64_t>(kMaxSlices))
@@ -672,7 +720,7 @@ int check_index_args(uint64_t n_sites, uint64_t n_slices, int device) {
     return ISSL_OK;
 }
 
-}  // namespace
+}  // namespace issl
 
 extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
                                       const uint64_t *sigs, const uint64_t *slice_masks,
@@ -691,30 +739,18 @@ extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64
     uint64_t lstart_count = 0;
     if ((rc = init_meta(ix.get(), n_sites, seq_len, n_slices, slice_masks, &lstart_count)) != ISSL_OK)
         return rc;
-    // Per-slice list starts from the list sizes (ISSLScoringModule.cpp:181-198).
     std::vector<uint32_t> lstart(lstart_count);
-    uint64_t sz_base = 0;
-    for (uint64_t s = 0; s < n_slices; ++s) {
-        const SliceMeta &m = ix->meta_h[s];
-        const uint64_t nk = 1ULL << (2 * m.width);
-        uint64_t run = 0, longest = 0;
-        for (uint64_t k = 0; k < nk; ++k) {
-            lstart[m.lbase + k] = static_cast<uint32_t>(run);
-            run += list_sizes[sz_base + k];
-            longest = std::max<uint64_t>(longest, list_sizes[sz_base + k]);
-        }
-        if (run != n_sites)
-            return fail(ISSL_ERR_FORMAT, "slice " + std::to_string(s) + " list sizes sum to " +
-                                             std::to_string(run) + ", not N");
-        lstart[m.lbase + nk] = static_cast<uint32_t>(run);
-        ix->p_max += longest;
-        sz_base += nk;
+    std::vector<const uint64_t *> sizes(n_slices);
+    for (uint64_t s = 0, base = 0; s < n_slices; ++s) {
+        sizes[s] = list_sizes + base;
+        base += 1ULL << (2 * ix->meta_h[s].width);
     }
+    if ((rc = build_lstart(ix.get(), sizes, lstart)) != ISSL_OK) return rc;
     if (n_slices && count_bad_ids(entries, n_slices * n_sites, n_sites) != 0)
         return fail(ISSL_ERR_FORMAT, "posting entries hold ids >= N");
 
     DeviceGuard guard(device);
-    if ((rc = alloc_index(ix.get(), lstart_count)) != ISSL_OK) return rc;
+    if ((rc = alloc_index(ix.get(), lstart_count, true)) != ISSL_OK) return rc;
     if ((rc = upload(ix.get(), ix->d_sig, sigs, n_sites * 8)) != ISSL_OK) return rc;
     if ((rc = upload(ix.get(), ix->d_post, entries, n_slices * n_sites * 8)) != ISSL_OK) return rc;
     if ((rc = upload(ix.get(), ix->d_lstart, lstart.data(), lstart.size() * 4)) != ISSL_OK) return rc;
@@ -797,7 +833,7 @@ extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_s
     if ((rc = init_meta(ix.get(), n_sites, seq_len, n_slices, slice_masks, &lstart_count)) != ISSL_OK)
         return rc;
     DeviceGuard guard(device);
-    if ((rc = alloc_index(ix.get(), lstart_count)) != ISSL_OK) return rc;
+    if ((rc = alloc_index(ix.get(), lstart_count, true)) != ISSL_OK) return rc;
     if ((rc = upload(ix.get(), ix->d_sig, sigs, n_sites * 8)) != ISSL_OK) return rc;
     hipStream_t st = ix->stream;
     const uint64_t n = n_sites;
@@ -865,6 +901,7 @@ extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_s
 
 extern "C" int issl_index_export(issl_index *ix, uint64_t *list_sizes, uint64_t *entries) {
     if (!ix || (ix->n_lists && !list_sizes)) return fail(ISSL_ERR_ARG, "null argument");
+    if (ix->src) return fail(ISSL_ERR_ARG, "export needs a resident index");
     std::lock_guard<std::mutex> lk(ix->mu);
     DeviceGuard guard(ix->device);
     uint64_t lstart_count = 0;
@@ -901,6 +938,8 @@ extern "C" int issl_index_get_info(const issl_index *ix, issl_index_info *out) {
 
 extern "C" int issl_index_validate(issl_index *ix, uint64_t *bad_entries) {
     if (!ix || !bad_entries) return fail(ISSL_ERR_ARG, "null argument");
+    if (ix->src) return fail(ISSL_ERR_ARG, "validation needs a resident index (a streamed index "
+                                           "checks its posting ids as it copies them)");
     std::lock_guard<std::mutex> lk(ix->mu);
     DeviceGuard guard(ix->device);
     int rc = ix->ws.scratch.reserve(ix->n * 4 + 64);
@@ -966,6 +1005,7 @@ namespace {
 int score_on_device(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_params *p,
                     const ScoreScalars &sc, const DevResults &o, hipStream_t st) {
     uint64_t *distinct = (p->flags & ISSL_FLAG_DISTINCT) ? ix->d_distinct : nullptr;
+    if (ix->src) return score_streamed(ix, d_guides, n, p, sc, o, distinct, st);
     if (p->form == ISSL_FORM_PLAIN) return launch_plain(ix, d_guides, n, sc, o, distinct, nullptr, nullptr, st);
     issl_tuning tn = ix->tuning;
     return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, tn, o, distinct, ix->ws, st);
@@ -983,6 +1023,7 @@ int score_with(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_
                const ScoreScalars &sc, const issl_tuning &tn, const DevResults &o, Workspace &ws,
                hipStream_t st) {
     uint64_t *distinct = (p->flags & ISSL_FLAG_DISTINCT) ? ix->d_distinct : nullptr;
+    if (ix->src) return fail(ISSL_ERR_ARG, "a streamed index is scored through issl_score or issl_score_device");
     if (p->form == ISSL_FORM_PLAIN) return launch_plain(ix, d_guides, n, sc, o, distinct, nullptr, nullptr, st);
     return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, tn, o, distinct, ws, st);
 }
@@ -1053,6 +1094,9 @@ extern "C" int issl_score_device(issl_index *ix, const uint64_t *guides_device, 
     o.sites = out_device->site_count;
     o.pass = out_device->pass;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+    if ((params->flags & ISSL_FLAG_GRAPH) && ix->src)
+        return fail(ISSL_ERR_ARG, "graph replay needs a resident index (a streamed index copies "
+                                  "postings from the host during the call)");
     if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
     const bool use_graph = (params->flags & ISSL_FLAG_GRAPH) && !ix->timing;
     if (use_graph) {

@@ -22,6 +22,8 @@ namespace issl {
                                                      hipGetErrorString(e_));             \
     } while (0)
 
+struct StreamSource;  // host side of a streamed index (stream.hip)
+
 // Per slice: fold mask (bit 2j per position j), first list-start index,
 // width and positions.
 struct SliceMeta {
@@ -98,7 +100,38 @@ struct ScoreScalars {
 
 struct Workspace {
     DevBuf guides, out, tiles, scratch;
+    DevBuf state;      // carried guide state of a streamed index (two copies)
     HostBuf pinned;
+};
+
+// Running state of one guide between slice windows: the two sums (FP64, or
+// the FP32 pairs of the compensated form), the occurrence and site counts,
+// and in bit 63 of `sites` whether the early exit has been taken.
+struct GuideState {
+    union {
+        struct {
+            float mh, ml, ch, cl;
+        } f;
+        struct {
+            double m, c;
+        } d;
+    };
+    uint64_t occ;
+    uint64_t sites;
+};
+constexpr uint64_t kExited = 1ULL << 63;
+
+// Slices [s0, s1) of an index whose postings sit at `post` / `inl`, slice s
+// at (s - s0) * N.  A launch over a window reads the guide state from
+// `state_in` (null: start from zero) and writes it to `state_out`, or, when
+// `state_out` is null, the final results.  The resident index is the single
+// window [0, S) over its own arrays with no state.
+struct Window {
+    int s0 = 0, s1 = 0;
+    const uint64_t *post = nullptr;
+    const uint64_t *inl = nullptr;
+    const GuideState *state_in = nullptr;
+    GuideState *state_out = nullptr;
 };
 
 // A launch sequence of issl_score_device captured in a hipGraph, with
@@ -150,17 +183,22 @@ struct issl_index {
     // another stream than the previous one first waits for that one's work.
     hipStream_t ws_stream = nullptr;
     hipEvent_t ws_done = nullptr;
+    // Streamed index (issl_index_open_streamed and issl_index_from_arrays_streamed):
+    // d_post and d_inline are null and the postings come from here.
+    issl::StreamSource *src = nullptr;
 };
 
 namespace issl {
 
 // Launchers (device.hip / batched.hip).  All asynchronous on `stream`.
+// `win` null: the whole resident index.
 int launch_plain(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                  const DevResults &out, uint64_t *distinct, const uint64_t *cand_off,
-                 uint32_t *cand_ids, hipStream_t stream);
+                 uint32_t *cand_ids, hipStream_t stream, const Window *win = nullptr);
 int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                    int precision, int chunk, const issl_tuning &tn, const DevResults &out,
-                   uint64_t *distinct, Workspace &ws, hipStream_t stream);
+                   uint64_t *distinct, Workspace &ws, hipStream_t stream,
+                   const Window *win = nullptr);
 int build_inline_layout(issl_index *ix);
 
 // The index's scoring workspace is shared by all its calls: acquire makes a
@@ -206,6 +244,38 @@ struct OutLayout {
 };
 // Launch shape used before any tuning.
 issl_tuning default_tuning();
+
+// Makes `dev` current for the scope, restoring the previous device.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Index construction steps shared by the resident and streamed forms
+// (device.hip): argument checks, slice metadata from the masks, list starts
+// from the list sizes, and the device arrays (with or without the S*N
+// posting arrays), plus a staged host-to-device copy and the destructor.
+int check_index_args(uint64_t n_sites, uint64_t n_slices, int device);
+int init_meta(issl_index *ix, uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+              const uint64_t *slice_masks, uint64_t *lstart_count);
+int build_lstart(issl_index *ix, const std::vector<const uint64_t *> &sizes,
+                 std::vector<uint32_t> &lstart);
+int alloc_index(issl_index *ix, uint64_t lstart_count, bool postings);
+int upload(issl_index *ix, void *dst, const void *src, size_t bytes);
+void free_index(issl_index *ix);
+
+// Streamed index (stream.hip): postings stay in host memory and pass through
+// a bounded set of device slice buffers.
+void stream_free(issl_index *ix);
+int score_streamed(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_params *p,
+                   const ScoreScalars &sc, const DevResults &o, uint64_t *distinct,
+                   hipStream_t st);
 // Event pair for the dominant kernel of a launch when timing is enabled
 // (both null otherwise).  The pair is attached to that kernel's dispatch with
 // hipExtLaunchKernelGGL, so it times the kernel itself, not the gaps around it.

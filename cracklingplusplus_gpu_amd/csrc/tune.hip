@@ -83,6 +83,7 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
                              const char *cache_path, issl_tuning *best) {
     if (!ix || !best) return fail(ISSL_ERR_ARG, "null argument");
     if (n == 0 || !sample) return fail(ISSL_ERR_ARG, "autotune needs sample guides");
+    if (ix->src) return fail(ISSL_ERR_ARG, "autotune needs a resident index");
     const std::string key = shape_key(ix);
     issl_tuning cached{};
     if (cache_lookup(cache_path, key, &cached)) {

@@ -134,6 +134,39 @@ int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
                      const uint64_t *sigs, const uint32_t *occ,
                      const uint64_t *slice_masks, int device, issl_index **out);
 
+/* Streamed index: the posting lists stay in host memory and pass through
+ * slice buffers in HBM, so an index larger than the device memory can be
+ * scored.  Replaces the memory-mapped loader of the reference's MMF scorer
+ * (ISSLScoreOfftargetsMMF/ISSLScoreOfftargetsMMF.cpp:99-153,
+ * lib/ISSLScoringModuleMMF.cpp:53-150): the file is memory-mapped and read
+ * from the mapping.  Only the signatures (8 B per site), the list starts
+ * and `slot_bytes` of slice buffers (16 B per site per buffer, at most
+ * n_slices buffers; 0 takes as many as fit in the free device memory less
+ * 4 GiB) stay on the device.  issl_score and issl_score_device stream the
+ * slices through the buffers in ascending order, a window at a time, with
+ * each guide's sums carried between windows: PLAIN results are bit-identical
+ * to the resident index's, BATCHED within its tolerance.  With n_slices
+ * buffers the slices stay resident after their first use.  Posting ids are
+ * checked against n_sites while they are copied (ISSL_ERR_FORMAT).
+ * Graph replay, candidate lists, validation, export and autotuning need a
+ * resident index and return ISSL_ERR_ARG. */
+int issl_index_open_streamed(const char *path, int device, uint64_t slot_bytes,
+                             issl_index **out);
+
+/* Same, over arrays in host memory in the .issl layout (as for
+ * issl_index_from_arrays); `entries` must stay valid and unchanged until
+ * the index is closed. */
+int issl_index_from_arrays_streamed(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+                                    const uint64_t *sigs, const uint64_t *slice_masks,
+                                    const uint64_t *list_sizes, const uint64_t *entries,
+                                    int device, uint64_t slot_bytes, issl_index **out);
+
+/* Slice buffers of a streamed index, bytes copied into them so far and
+ * windows scored (all 0 for a resident index).  No counterpart in the
+ * reference. */
+int issl_index_stream_stats(issl_index *ix, uint64_t *slots, uint64_t *uploaded_bytes,
+                            uint64_t *windows);
+
 /* Copy a resident index's lists back to the host in the .issl layout:
  * list_sizes[n_lists], entries[n_slices * n_sites] (for issl_write_index);
  * entries may be NULL to fetch the sizes only. */
