@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PCIE_PEAK_GBS = 64.0   # host link of the MI355X: PCIe 5.0 x16, one direction
 TIMING_REPEAT = 10   # dominant-kernel dispatches per HIP event pair
 METRIC = "guides/sec at 100M-site index (MIT+CFD); p99 latency; HBM GB/s fraction"
 
@@ -97,6 +98,58 @@ def measured_traffic(workload: str):
     except OSError:
         pass
     return None
+
+
+def stream_leg(issl, hiprt, hx, guides, ix, args) -> dict:
+    """Streamed index over the step's host arrays: every call copies all S
+    slices (S * N * 8 bytes) through the slice buffers while scoring."""
+    n = hx.n_sites
+    reps = (args.stream_guides + guides.shape[0] - 1) // guides.shape[0]
+    g = np.ascontiguousarray(np.tile(guides, reps)[:args.stream_guides])
+    Qs = g.shape[0]
+    sx = issl.Index.from_arrays_streamed(hx, slot_bytes=args.stream_slots * 16 * n)
+    slots = sx.stream_stats()[0]
+    d_g = hiprt.DeviceArray.from_host(g)
+    d_o = {"mit": hiprt.DeviceArray(Qs, np.float64), "cfd": hiprt.DeviceArray(Qs, np.float64),
+           "occ_count": hiprt.DeviceArray(Qs, np.uint64),
+           "pass": hiprt.DeviceArray(Qs, np.uint8)}
+    out = {"slots": int(slots), "slices": int(hx.n_slices), "guides_per_call": int(Qs),
+           "bytes_per_call": int(hx.n_slices * n * 8), "pcie_peak_GBps": PCIE_PEAK_GBS}
+    for name, kw in (("batched", dict(form=issl.FORM_BATCHED, precision=issl.PREC_FP32C)),
+                     ("plain", dict(form=issl.FORM_PLAIN))):
+        for _ in range(2):
+            sx.score_device(d_g, Qs, d_o, method=issl.METHOD_MIT_AND_CFD, threshold=0.0, **kw)
+        hiprt.synchronize()
+        _, up0, _ = sx.stream_stats()
+        calls = []
+        t0 = time.perf_counter()
+        for _ in range(args.stream_calls):
+            tc = time.perf_counter()
+            sx.score_device(d_g, Qs, d_o, method=issl.METHOD_MIT_AND_CFD, threshold=0.0, **kw)
+            hiprt.synchronize()
+            calls.append((time.perf_counter() - tc) * 1e3)
+        dt = (time.perf_counter() - t0) / args.stream_calls
+        _, up1, _ = sx.stream_stats()
+        gbps = (up1 - up0) / args.stream_calls / dt / 1e9
+        out[name] = {"ms_per_call": round(dt * 1e3, 2), "guides_per_s": round(Qs / dt, 1),
+                     "h2d_GBps": round(gbps, 2), "pcie_frac": round(gbps / PCIE_PEAK_GBS, 3),
+                     "call_ms": [round(c, 1) for c in calls]}
+        log(f"streamed {name}: {slots} of {hx.n_slices} slices in HBM, {Qs:,} guides per call, "
+            f"{dt * 1e3:.1f} ms/call, {Qs / dt:,.0f} guides/s, H2D {gbps:.1f} GB/s")
+    # Results against the resident index on the step's guides.
+    q = np.ascontiguousarray(guides)
+    a = sx.score(q, method=issl.METHOD_MIT_AND_CFD, form=issl.FORM_PLAIN)
+    b = ix.score(q, method=issl.METHOD_MIT_AND_CFD, form=issl.FORM_PLAIN)
+    out["plain_bit_identical_to_resident"] = bool(
+        (a.cfd.view(np.uint64) == b.cfd.view(np.uint64)).all() and
+        (a.mit.view(np.uint64) == b.mit.view(np.uint64)).all() and
+        (a.occ_count == b.occ_count).all())
+    c = sx.score(q, method=issl.METHOD_MIT_AND_CFD, form=issl.FORM_BATCHED,
+                 precision=issl.PREC_FP32C)
+    out["batched_max_rel_vs_resident_plain"] = float(
+        max(np.max(np.abs(c.mit - b.mit) / b.mit), np.max(np.abs(c.cfd - b.cfd) / b.cfd)))
+    sx.close()
+    return out
 
 
 def serve_poisson(issl, srv, index_ids, pools, rate, n_requests, size_of, rng):
@@ -229,6 +282,11 @@ def main() -> int:
                     help="replay the step's kernels from a captured hipGraph")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="minimum CPU scoring time (the batch is repeated)")
+    ap.add_argument("--stream-slots", type=int, default=4,
+                    help="slice buffers of the streamed-index leg (0: skip the leg)")
+    ap.add_argument("--stream-guides", type=int, default=100_000,
+                    help="guides per call of the streamed-index leg")
+    ap.add_argument("--stream-calls", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -371,6 +429,12 @@ def main() -> int:
         f"(upload + sort + postings) {device_build_s:.2f} s, sizes identical "
         f"{build['list_sizes_identical']}")
 
+    # ---- streamed index: postings in host memory, slices streamed through
+    # args.stream_slots HBM slice buffers on every call
+    streamed = None
+    if args.stream_slots > 0:
+        streamed = stream_leg(issl, hiprt, hx, guides, ix, args)
+
     # ---- serving: Poisson arrivals of 1,000-guide requests through the request loop
     log(f"serving: {args.serve_requests} requests of 1,000 guides at {args.serve_rate:g} req/s")
     srv = issl.Server(max_batch=65536)
@@ -455,6 +519,7 @@ def main() -> int:
         "serving": dict(serving, request_guides=1000,
                         mode="request loop, Poisson arrivals, host buffers in/out"),
         "index_build": build,
+        "streamed": streamed,
         "setup_s": round(setup_s, 1),
     }
     if cpu:

@@ -212,14 +212,16 @@ int upload_slice(issl_index *ix, StreamSource &src, int s, int j) {
     return ISSL_OK;
 }
 
-int build_slot_inline(issl_index *ix, StreamSource &src, int j) {
+// Signature-inline words of slot j, on the scoring stream (the copy stream
+// then moves the next window's postings meanwhile).
+int build_slot_inline(issl_index *ix, StreamSource &src, int j, hipStream_t st) {
     const uint64_t N = ix->n;
     if (N == 0) {
         src.slot_inline[static_cast<size_t>(j)] = 1;
         return ISSL_OK;
     }
     const uint64_t off = static_cast<uint64_t>(j) * N;
-    hipLaunchKernelGGL(k_inline_slot, dim3(2048), dim3(256), 0, src.copy, src.d_post + off,
+    hipLaunchKernelGGL(k_inline_slot, dim3(2048), dim3(256), 0, st, src.d_post + off,
                        ix->d_sig, src.d_inline + off, N);
     ISSL_HIP(hipGetLastError());
     src.slot_inline[static_cast<size_t>(j)] = 1;
@@ -265,12 +267,15 @@ int score_streamed(issl_index *ix, const uint64_t *d_guides, uint64_t n, const i
             if (src.slot_slice[static_cast<size_t>(j)] != s &&
                 (rc = upload_slice(ix, src, s, j)) != ISSL_OK)
                 return rc;
-            if (batched && !src.slot_inline[static_cast<size_t>(j)] &&
-                (rc = build_slot_inline(ix, src, j)) != ISSL_OK)
-                return rc;
         }
         ISSL_HIP(hipEventRecord(src.ready[grp], src.copy));
         ISSL_HIP(hipStreamWaitEvent(st, src.ready[grp], 0));
+        for (int s = s0; batched && s < s1; ++s) {
+            const int j = grp * G + (s - s0);
+            if (!src.slot_inline[static_cast<size_t>(j)] &&
+                (rc = build_slot_inline(ix, src, j, st)) != ISSL_OK)
+                return rc;
+        }
 
         Window w;
         w.s0 = s0;
