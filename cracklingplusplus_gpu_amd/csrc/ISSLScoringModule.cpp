@@ -1,9 +1,13 @@
-// ISSLScoringModule.cpp -- the ISSL pipeline stage on the GPU.
+// ISSLScoringModule.cpp -- the ISSL pipeline stage on the GPU, resident and
+// memory-mapped.
 //
 // Mirrors lib/ISSLScoringModule.cpp: constructor method check (:13-50),
 // run() (:52-466) with processGuide gating (:488-513), result fields and
-// codes (:404-459), console messages.  Scoring goes through issl_score.
+// codes (:404-459), console messages; and lib/ISSLScoringModuleMMF.cpp, the
+// same stage over a mapped index (:53-438).  Scoring goes through
+// issl_score.
 #include "ISSLScoringModule.hpp"
+#include "ISSLScoringModuleMMF.hpp"
 
 #include <cstdlib>
 #include <iostream>
@@ -30,7 +34,7 @@ void check(int rc, const char *what) {
 
 }  // namespace
 
-ISSLScoringModule::ISSLScoringModule(const cracklingConfig &cfg)
+ISSLScoringStage::ISSLScoringStage(const cracklingConfig &cfg)
     : config(cfg.offTarget),
       ISSLIndex(cfg.input.offtargetSites),
       toolIsSelected(cfg.offTarget.enabled),
@@ -49,19 +53,30 @@ ISSLScoringModule::ISSLScoringModule(const cracklingConfig &cfg)
     if (const char *d = std::getenv("ISSL_DEVICE")) device = std::atoi(d);
 }
 
-ISSLScoringModule::~ISSLScoringModule() {
+ISSLScoringStage::~ISSLScoringStage() {
     if (index) issl_index_close(index);
 }
 
-void ISSLScoringModule::run(std::vector<guideResults> &candidateGuides) {
+void ISSLScoringStage::score(std::vector<guideResults> &candidateGuides, bool streamed) {
     if (!toolIsSelected) {
         std::cout << "Off-target scoring has been configured not to run. Skipping Off-target scoring"
                   << std::endl;
         return;
     }
     if (!index) {
-        std::cout << "Loading ISSL Index." << std::endl;
-        check(issl_index_open(ISSLIndex.string().c_str(), device, &index), "Error reading index");
+        if (streamed) {
+            // lib/ISSLScoringModuleMMF.cpp:61-130: the file is mapped, not
+            // read; here its slices stream from the mapping through HBM
+            // buffers (ISSL_STREAM_BYTES of them, default: what fits).
+            std::cout << "Mapping ISSL Index." << std::endl;
+            uint64_t budget = 0;
+            if (const char *b = std::getenv("ISSL_STREAM_BYTES")) budget = std::strtoull(b, nullptr, 10);
+            check(issl_index_open_streamed(ISSLIndex.string().c_str(), device, budget, &index),
+                  "Error reading index");
+        } else {
+            std::cout << "Loading ISSL Index." << std::endl;
+            check(issl_index_open(ISSLIndex.string().c_str(), device, &index), "Error reading index");
+        }
         std::cout << "ISSL Index Loaded." << std::endl;
     }
     std::cout << "Beginning Off-target scoring." << std::endl;
@@ -72,7 +87,7 @@ void ISSLScoringModule::run(std::vector<guideResults> &candidateGuides) {
     for (size_t i = 0; i < candidateGuides.size(); ++i) {
         if (!processGuide(candidateGuides[i])) continue;
         picked.push_back(i);
-        sigs.push_back(sequenceToSignature(candidateGuides[i].seq, 20));
+        sigs.push_back(ISSLScoringModule::sequenceToSignature(candidateGuides[i].seq, 20));
     }
     const size_t n = picked.size();
     std::vector<double> mitv(n), cfdv(n);
@@ -102,8 +117,14 @@ void ISSLScoringModule::run(std::vector<guideResults> &candidateGuides) {
               << std::endl;
 }
 
-// lib/ISSLScoringModule.cpp:488-513
-bool ISSLScoringModule::processGuide(const guideResults &guide) const {
+ISSLScoringModule::ISSLScoringModule(const cracklingConfig &cfg) : ISSLScoringStage(cfg) {}
+
+void ISSLScoringModule::run(std::vector<guideResults> &candidateGuides) {
+    score(candidateGuides, false);
+}
+
+// lib/ISSLScoringModule.cpp:488-513 (the MMF module's :461-486 is the same)
+bool ISSLScoringStage::processGuide(const guideResults &guide) const {
     if (optimsationLevel == optimisationLevel::ultralow) return true;
     if (!guide.isUnique) return false;
     if (optimsationLevel == optimisationLevel::medium || optimsationLevel == optimisationLevel::high) {
@@ -128,4 +149,18 @@ std::string ISSLScoringModule::signatureToSequence(uint64_t sig, uint64_t seqLen
     std::string s(seqLen, ' ');
     for (uint64_t j = 0; j < seqLen; ++j) s[j] = "ACGT"[(sig >> (j * 2)) & 3];
     return s;
+}
+
+ISSLScoringModuleMMF::ISSLScoringModuleMMF(const cracklingConfig &cfg) : ISSLScoringStage(cfg) {}
+
+void ISSLScoringModuleMMF::run(std::vector<guideResults> &candidateGuides) {
+    score(candidateGuides, true);
+}
+
+uint64_t ISSLScoringModuleMMF::sequenceToSignature(const std::string &seq, uint64_t seqLen) {
+    return ISSLScoringModule::sequenceToSignature(seq, seqLen);
+}
+
+std::string ISSLScoringModuleMMF::signatureToSequence(uint64_t sig, uint64_t seqLen) {
+    return ISSLScoringModule::signatureToSequence(sig, seqLen);
 }

@@ -1,9 +1,10 @@
-// module_demo.cpp -- drives the drop-in ISSLScoringModule the way the
-// Crackling driver does (src/CracklingPlusPlus.cpp:38,74-81): build a
-// cracklingConfig, fill guideResults, call run(), read the result fields.
+// module_demo.cpp -- drives the drop-in ISSLScoringModule (or, with
+// memoryMapped set, ISSLScoringModuleMMF) the way the Crackling driver does
+// (src/CracklingPlusPlus.cpp:38-39,73-80): build a cracklingConfig, fill
+// guideResults, call run(), read the result fields.
 //
 //   issl_module_demo <index.issl> <guides.txt> <method> <threshold>
-//                    <optimisation> <enabled 0|1> <consensus n>
+//                    <optimisation> <enabled 0|1> <consensus n> [mmf]
 //
 // guides.txt: one guide per line: "SEQ isUnique(0|1) consensusCount passedBowtie2(char)".
 // Output (stdout): "SEQ\t<mit %.17g>\t<cfd %.17g>\t<passedOffTargetScore>" per guide.
@@ -16,10 +17,11 @@
 #include <vector>
 
 #include "ISSLScoringModule.hpp"
+#include "ISSLScoringModuleMMF.hpp"
 
 int main(int argc, char **argv) {
-    if (argc != 8) {
-        fprintf(stderr, "usage: %s index guides method threshold optimisation enabled consensus\n",
+    if (argc != 8 && !(argc == 9 && std::string(argv[8]) == "mmf")) {
+        fprintf(stderr, "usage: %s index guides method threshold optimisation enabled consensus [mmf]\n",
                 argv[0]);
         return 2;
     }
@@ -38,6 +40,7 @@ int main(int argc, char **argv) {
     cfg.general.optimisation = levels.at(argv[5]);
     cfg.offTarget.enabled = atoi(argv[6]) != 0;
     cfg.consensus.n = static_cast<uint8_t>(atoi(argv[7]));
+    cfg.offTarget.memoryMapped = argc == 9;
 
     std::vector<guideResults> guides;
     std::ifstream in(argv[2]);
@@ -53,9 +56,16 @@ int main(int argc, char **argv) {
         guides.push_back(g);
     }
     try {
-        ISSLScoringModule module(cfg);
-        module.run(guides);
-        module.run(guides);  // a second batch reuses the resident index
+        // As the driver does: both modules exist, the flag picks one; a
+        // second batch reuses the index opened by the first.
+        ISSLScoringModule ISSLScoring(cfg);
+        ISSLScoringModuleMMF ISSLScoringMMF(cfg);
+        for (int batch = 0; batch < 2; ++batch) {
+            if (cfg.offTarget.memoryMapped)
+                ISSLScoringMMF.run(guides);
+            else
+                ISSLScoring.run(guides);
+        }
     } catch (const std::exception &e) {
         fprintf(stderr, "error: %s\n", e.what());
         return 1;

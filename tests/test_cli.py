@@ -130,8 +130,9 @@ REFTYPES_DEMO = os.path.join(ROOT, "oracle", "_ref", "issl_module_demo_reftypes"
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(REFTYPES_DEMO),
                     reason="built by `make -C oracle ref` where the reference sources are present")
+@pytest.mark.parametrize("mmf", [False, True])
 @pytest.mark.parametrize("method,level", [("mitAndCfd", "medium"), ("avgMitCfd", "ultralow")])
-def test_module_built_on_reference_types_matches(tmp_path, method, level, gpu_available):
+def test_module_built_on_reference_types_matches(tmp_path, method, level, mmf, gpu_available):
     """The drop-in module compiled against the pipeline's own include/util.hpp
     (cracklingConfig, guideResults, the CODE_* constants of lib/util.cpp)
     behaves exactly as the build on the mirror types."""
@@ -141,9 +142,35 @@ def test_module_built_on_reference_types_matches(tmp_path, method, level, gpu_av
     f = tmp_path / "g.txt"
     f.write_text("".join(f"{s}AGG {int(rng.random() < 0.8)} {rng.integers(0, 4)} "
                          f"{rng.choice(['1', '0', '?'])}\n" for s in seqs))
-    args = [TINY, str(f), method, "75", level, "1", "2"]
+    args = [TINY, str(f), method, "75", level, "1", "2"] + (["mmf"] if mmf else [])
     a = run([DEMO] + args)
     b = run([REFTYPES_DEMO] + args)
     assert a.returncode == 0 and b.returncode == 0, (a.stderr, b.stderr)
     assert a.stdout == b.stdout
     assert sum(1 for l in b.stdout.splitlines() if l.startswith("RESULT")) == len(seqs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slots", [1, 2, 0])
+@pytest.mark.parametrize("method,level", [("mitAndCfd", "medium"), ("cfd", "ultralow")])
+def test_module_mmf_matches_resident(tmp_path, method, level, slots, gpu_available):
+    """The memory-mapped module (the driver's choice when
+    offTarget.memoryMapped is set) streams the index through HBM slice
+    buffers and gives the resident module's results bit for bit."""
+    hx = golden_index("tiny4k_5x4")
+    gd = load_golden("tiny4k_5x4")
+    seqs = issl.decode(gd["guides"])
+    rng = np.random.default_rng(11)
+    f = tmp_path / "g.txt"
+    f.write_text("".join(f"{s}AGG {int(rng.random() < 0.8)} {rng.integers(0, 4)} "
+                         f"{rng.choice(['1', '0', '?'])}\n" for s in seqs))
+    args = [TINY, str(f), method, "75", level, "1", "2"]
+    env = dict(os.environ, ISSL_STREAM_BYTES=str(slots * 16 * hx.n_sites))
+    a = run([DEMO] + args)
+    b = run([DEMO] + args + ["mmf"], env=env)
+    assert a.returncode == 0 and b.returncode == 0, (a.stderr, b.stderr)
+    assert "Mapping ISSL Index." in b.stdout and "Loading ISSL Index." not in b.stdout
+    ra = [l for l in a.stdout.splitlines() if l.startswith("RESULT") or "failed here" in l]
+    rb = [l for l in b.stdout.splitlines() if l.startswith("RESULT") or "failed here" in l]
+    assert ra == rb
+    assert sum(1 for l in rb if l.startswith("RESULT")) == len(seqs)
