@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "ISSLScoringModule.hpp"
+#include "cli_common.hpp"
 #include "issl_gpu.h"
 
 namespace {
@@ -33,13 +34,9 @@ double secs(std::chrono::high_resolution_clock::time_point a,
     return std::chrono::duration<double>(b - a).count();
 }
 
-[[noreturn]] void die(int rc, const char *what) {
-    const char *m = issl_last_error();
-    fprintf(stderr, "%s: %s (%d)\n", what, m ? m : "", rc);
-    exit(1);
-}
-
 }  // namespace
+
+using issl_cli::die;
 
 int main(int argc, char **argv) {
     using clk = std::chrono::high_resolution_clock;
@@ -54,18 +51,9 @@ int main(int argc, char **argv) {
     const int maxDist = atoi(argv[3]);
     if (maxDist != 4) fprintf(stderr, "Note: sites within 4 mismatches are scored (max distance %d ignored)\n", maxDist);
     const double threshold = atof(argv[4]);
-    const std::string m = argv[5];
     int method;
-    bool calcMit = true, calcCfd = true;
-    if (m == "and") method = ISSL_METHOD_MIT_AND_CFD;
-    else if (m == "or") method = ISSL_METHOD_MIT_OR_CFD;
-    else if (m == "avg") method = ISSL_METHOD_AVG_MIT_CFD;
-    else if (m == "mit") { method = ISSL_METHOD_MIT; calcCfd = false; }
-    else if (m == "cfd") { method = ISSL_METHOD_CFD; calcMit = false; }
-    else {
-        fprintf(stderr, "Invalid scoring method. Acceptable options are: 'and', 'or', 'avg', 'mit', 'cfd'");
-        exit(1);
-    }
+    bool calcMit, calcCfd;
+    issl_cli::parse_method(argv[5], &method, &calcMit, &calcCfd);
 
     const auto t_load = clk::now();
     issl_index *ix = nullptr;
@@ -78,34 +66,8 @@ int main(int argc, char **argv) {
     printf("Index loading time: %g seconds\n", secs(t_load, t_loaded));
 
     const auto t_query = clk::now();
-    const size_t lineLen = info.seq_len + 1;
-    std::error_code ec;
-    const uintmax_t fileSize = std::filesystem::file_size(argv[2], ec);
-    if (ec) {
-        fprintf(stderr, "Failed to read in query file.\n");
-        exit(1);
-    }
-    if (fileSize % lineLen != 0) {
-        fprintf(stderr, "Error: query file is not a multiple of the expected line length (%zu)\n", lineLen);
-        fprintf(stderr, "The sequence length may be incorrect; alternatively, the line endings\n");
-        fprintf(stderr, "may be something other than LF, or there may be junk at the end of the file.\n");
-        exit(1);
-    }
-    const size_t q = fileSize / lineLen;
-    std::vector<char> text(fileSize + 1);
-    FILE *fp = fopen(argv[2], "rb");
-    if (!fp || (fileSize && fread(text.data(), fileSize, 1, fp) < 1)) {
-        fprintf(stderr, "Failed to read in query file.\n");
-        exit(1);
-    }
-    fclose(fp);
-    if (lineLen < 21) {
-        fprintf(stderr, "Error: the index holds sequences shorter than 20\n");
-        exit(1);
-    }
-    std::vector<uint64_t> sigs(q);
-    rc = issl_encode(text.data(), q, lineLen, sigs.data());
-    if (rc != ISSL_OK) die(rc, "encode");
+    const std::vector<uint64_t> sigs = issl_cli::read_queries(argv[2], info.seq_len + 1);
+    const size_t q = sigs.size();
     printf("Query loading time: %g seconds\n", secs(t_query, clk::now()));
 
     const auto t_score = clk::now();

@@ -81,6 +81,40 @@ def test_cli_output_matches_reference(tmp_path, name, gpu_available):
     assert f"Total off-targets scored: {int(o['occ_count'].sum())}" in r.stderr
 
 
+CLI_MMF = os.path.join(BIN, "ISSLScoreOfftargetsMMF")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slots", [1, 0])
+@pytest.mark.parametrize("name,t", [("and", "75"), ("avg", "0"), ("mit", "75"), ("cfd", "75")])
+def test_mmf_cli_output_matches_reference(tmp_path, name, t, slots, gpu_available):
+    """ISSLScoreOfftargetsMMF: the reference's stdout lines only, from the
+    streamed index, equal to the golden scores."""
+    gd = load_golden("tiny4k_5x4")
+    hx = golden_index("tiny4k_5x4")
+    seqs = issl.decode(gd["guides"])
+    q = tmp_path / "queries.txt"
+    q.write_text("".join(s + "\n" for s in seqs))
+    env = dict(os.environ, ISSL_STREAM_BYTES=str(slots * 16 * hx.n_sites))
+    r = run([CLI_MMF, TINY, str(q), "4", t, name], env=env)
+    assert r.returncode == 0, r.stderr
+    m = CLI_METHOD[name]
+    key = f"m{m}_t{t}"
+    want = []
+    for i, s in enumerate(seqs):
+        ms = "%f" % gd[key + "_mit"][i] if m != 1 else "-1"
+        cs = "%f" % gd[key + "_cfd"][i] if m != 0 else "-1"
+        want.append(f"{s}\t{ms}\t{cs}")
+    assert r.stdout.splitlines() == want
+
+
+def test_mmf_cli_usage_and_bad_method():
+    r = run([CLI_MMF])
+    assert r.returncode == 1 and "Usage:" in r.stderr
+    r = run([CLI_MMF, TINY, "q.txt", "4", "75", "best"])
+    assert r.returncode == 1 and "Invalid scoring method" in r.stderr
+
+
 @pytest.mark.gpu
 def test_cli_rejects_ragged_query_file(tmp_path, gpu_available):
     q = tmp_path / "q.txt"
