@@ -365,3 +365,62 @@ uint64_t or_candidates_brute(const uint64_t *sigs, uint64_t n_sites,
     }
     return n;
 }
+
+/* ---------------------------------------------------------------- extract */
+
+/* Base of a scanned sequence: the reference upper-cases every line
+ * (ExtractOfftargets.cpp:88-98) and its patterns admit only A, C, G, T. */
+static int or_base(char c)
+{
+    if (c >= 'a' && c <= 'z')
+        c = (char)(c - 'a' + 'A');
+    switch (c) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'G': return 2;
+    case 'T': return 3;
+    default: return -1;
+    }
+}
+
+uint64_t or_extract(const char *seq, uint64_t len, char *out, uint64_t cap)
+{
+    static const char bases[4] = {'A', 'C', 'G', 'T'};
+    uint64_t n = 0;
+    /* forward: (?=([ACG][ACGT]{19}[ACGT][AG]G)), first 20 characters
+     * (ExtractOfftargets.cpp:16,127-141) */
+    for (uint64_t i = 0; i + 23 <= len; ++i) {
+        int ok = 1;
+        for (int j = 0; j < 23 && ok; ++j)
+            ok = or_base(seq[i + j]) >= 0;
+        if (!ok)
+            continue;
+        if (or_base(seq[i]) == 3)
+            continue;
+        const int b21 = or_base(seq[i + 21]);
+        if (!(b21 == 0 || b21 == 2) || or_base(seq[i + 22]) != 2)
+            continue;
+        if (n < cap)
+            for (int j = 0; j < 20; ++j)
+                out[n * 20 + j] = bases[or_base(seq[i + j])];
+        ++n;
+    }
+    /* reverse: (?=(C[CT][ACGT][ACGT]{19}[TGC])), reverse complement of the
+     * 23 characters (lib/util.cpp:50-63), first 20
+     * (ExtractOfftargets.cpp:17,142-156) */
+    for (uint64_t i = 0; i + 23 <= len; ++i) {
+        int ok = 1;
+        for (int j = 0; j < 23 && ok; ++j)
+            ok = or_base(seq[i + j]) >= 0;
+        if (!ok)
+            continue;
+        const int b1 = or_base(seq[i + 1]);
+        if (or_base(seq[i]) != 1 || !(b1 == 1 || b1 == 3) || or_base(seq[i + 22]) == 0)
+            continue;
+        if (n < cap)
+            for (int j = 0; j < 20; ++j)
+                out[n * 20 + j] = bases[3 - or_base(seq[i + 22 - j])];
+        ++n;
+    }
+    return n;
+}

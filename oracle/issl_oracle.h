@@ -18,6 +18,10 @@
  *                           golden vectors; brute force candidate sets
  *   or_candidates_brute  -- closed form over every site (independent of the
  *                           slice lists)
+ *   or_extract           -- Python's regex engine running the reference's
+ *                           two patterns verbatim (lookahead, every
+ *                           position) on random and PAM-dense sequences,
+ *                           and a committed golden output
  * Nothing here is "parity unpinned".
  */
 #ifndef ISSL_ORACLE_H
@@ -89,6 +93,16 @@ uint64_t or_candidates_brute(const uint64_t *sigs, uint64_t n_sites,
                              uint64_t seq_len, const uint64_t *slice_masks,
                              uint64_t n_slices, uint64_t guide,
                              uint32_t *ids_out, uint64_t cap);
+
+/* Off-target sites of one sequence, ExtractOfftargets/ExtractOfftargets.cpp:
+ * 109-160.  Every position where the forward pattern (?=([ACG][ACGT]{19}
+ * [ACGT][AG]G)) matches gives the 20 characters there; then every position
+ * where the reverse pattern (?=(C[CT][ACGT][ACGT]{19}[TGC])) matches gives
+ * the first 20 characters of the 23-mer's reverse complement (lib/util.cpp:
+ * 50-63).  Letters are taken upper-cased.  Sites go to out as 20 characters
+ * each (capacity cap sites), forward ones first; returns the number found
+ * (which may exceed cap). */
+uint64_t or_extract(const char *seq, uint64_t len, char *out, uint64_t cap);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,8 @@ def lib():
                                c_void_p, c_void_p, c_uint64, c_int, c_double, c_int,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p]
+        L.or_extract.restype = c_uint64
+        L.or_extract.argtypes = [c_char_p, c_uint64, c_void_p, c_uint64]
         L.or_candidates_brute.restype = c_uint64
         L.or_candidates_brute.argtypes = [c_void_p, c_uint64, c_uint64, c_void_p, c_uint64,
                                           c_uint64, c_void_p, c_uint64]
@@ -91,3 +93,56 @@ def candidates_brute(ix, guide: int, cap: int = 1 << 20) -> np.ndarray:
     if n > cap:
         raise ValueError("capacity")
     return ids[:n]
+
+
+# ---------------------------------------------------------------- extraction
+
+_WS = b" \t\n\v\f\r"
+
+
+def split_input(text: bytes) -> list:
+    """Sequences of one input file as ExtractOfftargets.cpp:76-109 splits
+    them: FASTA when the first line starts with '>' (every later line
+    trimmed and upper-cased; a '>' line starts the next sequence, other
+    lines are appended), otherwise every line (trimmed, upper-cased) is a
+    sequence of its own."""
+    lines = text.split(b"\n")
+    if text.endswith(b"\n"):
+        lines = lines[:-1]   # getline yields no empty last line
+    if not lines:
+        return [b""]         # the reference's do-while writes one empty file
+    out = []
+    if lines[0][:1] == b">":
+        cur = []
+        for ln in lines[1:]:
+            ln = ln.strip(_WS).upper()
+            if ln[:1] == b">":
+                out.append(b"".join(cur))
+                cur = []
+            else:
+                cur.append(ln)
+        out.append(b"".join(cur))
+    else:
+        out = [ln.strip(_WS).upper() for ln in lines]
+    return out
+
+
+def extract(seq: bytes) -> list:
+    """Sites of one sequence (or_extract): forward ones in position order,
+    then reverse ones."""
+    n = lib().or_extract(seq, len(seq), None, 0)
+    buf = ctypes.create_string_buffer(max(1, 20 * n))
+    lib().or_extract(seq, len(seq), ctypes.cast(buf, c_void_p), n)
+    raw = buf.raw[: 20 * n].decode()
+    return [raw[20 * i: 20 * i + 20] for i in range(n)]
+
+
+def extract_output(texts) -> bytes:
+    """The reference's output file for these input files: every site of
+    every sequence, sorted, one per line (duplicates kept)."""
+    sites = []
+    for t in texts:
+        for seq in split_input(t):
+            sites.extend(extract(seq))
+    sites.sort()
+    return "".join(x + "\n" for x in sites).encode()

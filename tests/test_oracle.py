@@ -2,6 +2,7 @@
 import hashlib
 import itertools
 import os
+import re
 
 import numpy as np
 import pytest
@@ -117,3 +118,62 @@ def test_threshold_zero_disables_exit():
     assert (r["pass"] == ord("1")).all()
     # the sums are full: equal to the brute-force occurrence-weighted CFD count floor
     assert (r["cfd_sum"] >= 0).all()
+
+
+# ---------------------------------------------------------------- extraction
+
+_FWD = re.compile(r"(?=([ACG][ACGT]{19}[ACGT][AG]G))")
+_BWD = re.compile(r"(?=(C[CT][ACGT][ACGT]{19}[TGC]))")
+_COMP = str.maketrans("ACGT", "TGCA")
+
+
+def _regex_sites(seq: str) -> list:
+    """The reference's patterns (ExtractOfftargets.cpp:16-17) run by Python's
+    regex engine: a lookahead match at every position."""
+    out = [m.group(1)[:20] for m in _FWD.finditer(seq)]
+    out += [m.group(1)[::-1].translate(_COMP)[:20] for m in _BWD.finditer(seq)]
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_extract_oracle_matches_reference_patterns(seed):
+    rng = np.random.default_rng(seed)
+    for trial in range(40):
+        n = int(rng.integers(0, 900))
+        if trial % 3 == 0:     # PAM-dense: overlapping matches on both strands
+            alpha, p = list("GCAT"), [0.4, 0.4, 0.1, 0.1]
+        elif trial % 3 == 1:   # mixed case and ambiguity codes
+            alpha, p = list("ACGTacgtNRY"), [0.15] * 4 + [0.08] * 4 + [0.04, 0.02, 0.02]
+        else:
+            alpha, p = list("ACGT"), None
+        seq = "".join(rng.choice(alpha, size=n, p=p))
+        got = orc.extract(seq.encode())
+        assert got == _regex_sites(seq.upper()), (seed, trial)
+
+
+def test_extract_oracle_edge_cases():
+    assert orc.extract(b"") == []
+    assert orc.extract(b"ACGTACGTACGTACGTACGTAG") == []           # 22 bases
+    assert orc.extract(b"TCGTACGTACGTACGTACGTAGG") == []          # T at the first base
+    assert orc.extract(b"ACGTACGTACGTACGTACGTATG") == []          # PAM xTG
+    assert orc.extract(b"ACGTACGTACGTACGTACGTAAG") == ["ACGTACGTACGTACGTACGT"]   # NAG
+    assert orc.extract(b"acgtacgtacgtacgtacgtagg") == ["ACGTACGTACGTACGTACGT"]
+    assert orc.extract(b"ACGTACGTACNTACGTACGTAGG") == []          # N inside the window
+    # 'CCN' then 19 bases then [TGC] -> reverse complement
+    assert orc.extract(b"CCAGTACGTACGTACGTACGTAC") == ["GTACGTACGTACGTACGTAC"]
+    assert orc.extract(b"CCAGTACGTACGTACGTACGTAA") == []          # last base A
+
+
+def test_extract_split_rules():
+    # FASTA: headers start sequences, lines are trimmed, upper-cased, joined.
+    assert orc.split_input(b">a\nacg \r\n tTT\n\n>b x\nGG\n") == [b"ACGTTT", b"GG"]
+    # plain text: one sequence per line, the first line included
+    assert orc.split_input(b"acg\n TT \r\n") == [b"ACG", b"TT"]
+    assert orc.split_input(b"") == [b""]
+
+
+def test_extract_oracle_golden():
+    g = os.path.join(GOLDEN, "extract_")
+    texts = [open(g + "genome.fa", "rb").read(), open(g + "lines.txt", "rb").read()]
+    want = open(g + "expected.txt", "rb").read()
+    assert orc.extract_output(texts) == want
