@@ -12,6 +12,7 @@ Drop-in surface of the reference:
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import byref, c_uint64, c_void_p
 from dataclasses import dataclass
 from typing import Optional
@@ -29,7 +30,7 @@ __all__ = [
     "synth", "device_count", "METHOD_MIT", "METHOD_CFD", "METHOD_MIT_AND_CFD",
     "METHOD_MIT_OR_CFD", "METHOD_AVG_MIT_CFD", "FORM_PLAIN", "FORM_BATCHED", "PREC_FP64",
     "PREC_FP32C", "FLAG_DISTINCT", "FLAG_GRAPH", "METHOD_NAMES", "CLI_METHODS", "Server", "Request",
-    "device_memory",
+    "device_memory", "Extractor",
 ]
 
 # include/util.hpp:20 and lib/util.cpp:13-19
@@ -420,6 +421,65 @@ class Request:
         if done.value:
             self.status = int(st.value)
         return bool(done.value)
+
+
+class Extractor:
+    """Off-target site extraction on one GPU (issl_extract.h): the
+    reference's ExtractOfftargets (ExtractOfftargets.cpp:19-356)."""
+
+    def __init__(self, device: int = 0, chunk_bytes: int = 0):
+        h = c_void_p()
+        check(lib().issl_extractor_create(device, chunk_bytes, byref(h)))
+        self._h = h
+        self._distinct = 0
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            check(lib().issl_extractor_destroy(self._h))
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_sequence(self, seq) -> None:
+        b = seq.encode() if isinstance(seq, str) else bytes(seq)
+        check(lib().issl_extractor_add_sequence(self._h, b, len(b)))
+
+    def add_path(self, path: str) -> None:
+        check(lib().issl_extractor_add_path(self._h, os.fsencode(path)))
+
+    def finish(self):
+        """(sites, distinct sites) after sorting everything added so far."""
+        a, b = c_uint64(), c_uint64()
+        check(lib().issl_extractor_finish(self._h, byref(a), byref(b)))
+        self._distinct = b.value
+        return a.value, b.value
+
+    def sites(self):
+        """Distinct sites in text order: (signatures uint64, occurrences uint32)."""
+        n = self._distinct
+        sigs = np.zeros(max(n, 1), dtype=np.uint64)[:n]
+        occ = np.zeros(max(n, 1), dtype=np.uint32)[:n]
+        check(lib().issl_extractor_sites(self._h, _ptr(sigs), _ptr(occ)))
+        return sigs, occ
+
+    def write(self, path: str) -> None:
+        check(lib().issl_extractor_write(self._h, os.fsencode(path)))
+
+    def build_index(self, masks) -> "Index":
+        masks = np.ascontiguousarray(masks, dtype=np.uint64)
+        h = c_void_p()
+        check(lib().issl_extractor_build_index(self._h, _ptr(masks), masks.shape[0], byref(h)))
+        return Index(h.value)
+
+    def stats(self):
+        """(bases scanned, sites found) so far."""
+        a, b = c_uint64(), c_uint64()
+        check(lib().issl_extractor_stats(self._h, byref(a), byref(b)))
+        return a.value, b.value
 
 
 class Server:

@@ -132,6 +132,15 @@ SIGNATURES = [
                                         c_void_p, c_uint64, POINTER(c_uint64)]),
     ("issl_score_tables", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     # issl_server.h
+    ("issl_extractor_create", c_int, [c_int, c_uint64, POINTER(c_void_p)]),
+    ("issl_extractor_destroy", c_int, [c_void_p]),
+    ("issl_extractor_add_sequence", c_int, [c_void_p, c_char_p, c_uint64]),
+    ("issl_extractor_add_path", c_int, [c_void_p, c_char_p]),
+    ("issl_extractor_finish", c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]),
+    ("issl_extractor_sites", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("issl_extractor_write", c_int, [c_void_p, c_char_p]),
+    ("issl_extractor_build_index", c_int, [c_void_p, c_void_p, c_uint64, POINTER(c_void_p)]),
+    ("issl_extractor_stats", c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]),
     ("issl_server_create", c_int, [POINTER(issl_server_config), POINTER(c_void_p)]),
     ("issl_server_destroy", c_int, [c_void_p]),
     ("issl_server_add_index", c_int, [c_void_p, c_char_p, POINTER(c_int32)]),

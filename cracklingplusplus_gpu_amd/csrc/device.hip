@@ -815,9 +815,11 @@ __global__ void k_entries(const uint32_t *__restrict__ ids, const uint32_t *__re
 
 }  // namespace
 
-extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
-                                const uint64_t *sigs, const uint32_t *occ,
-                                const uint64_t *slice_masks, int device, issl_index **out) {
+namespace issl {
+
+int index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices, const uint64_t *sigs,
+                const uint32_t *occ, bool on_device, const uint64_t *slice_masks, int device,
+                issl_index **out) {
     if (!out) return fail(ISSL_ERR_ARG, "null output handle");
     *out = nullptr;
     int rc = check_index_args(n_sites, n_slices, device);
@@ -834,8 +836,11 @@ extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_s
         return rc;
     DeviceGuard guard(device);
     if ((rc = alloc_index(ix.get(), lstart_count, true)) != ISSL_OK) return rc;
-    if ((rc = upload(ix.get(), ix->d_sig, sigs, n_sites * 8)) != ISSL_OK) return rc;
     hipStream_t st = ix->stream;
+    if (on_device)
+        ISSL_HIP(hipMemcpyAsync(ix->d_sig, sigs, n_sites * 8, hipMemcpyDeviceToDevice, st));
+    else if ((rc = upload(ix.get(), ix->d_sig, sigs, n_sites * 8)) != ISSL_OK)
+        return rc;
     const uint64_t n = n_sites;
     uint64_t max_nk = 1;
     int max_bits = 0;
@@ -847,7 +852,10 @@ extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_s
     // the longest list, and the sort / scan / reduce temporaries.
     DevBuf b_occ, b_k, b_ks, b_i, b_is, b_cnt, b_tmp;
     if ((rc = b_occ.reserve(std::max<uint64_t>(n, 1) * 4)) != ISSL_OK) return rc;
-    if ((rc = upload(ix.get(), b_occ.p, occ, n * 4)) != ISSL_OK) return rc;
+    if (on_device)
+        ISSL_HIP(hipMemcpyAsync(b_occ.p, occ, n * 4, hipMemcpyDeviceToDevice, st));
+    else if ((rc = upload(ix.get(), b_occ.p, occ, n * 4)) != ISSL_OK)
+        return rc;
     const size_t kb = std::max<uint64_t>(n, 1) * 4;
     if ((rc = b_k.reserve(kb)) != ISSL_OK || (rc = b_ks.reserve(kb)) != ISSL_OK ||
         (rc = b_i.reserve(kb)) != ISSL_OK || (rc = b_is.reserve(kb)) != ISSL_OK ||
@@ -896,7 +904,16 @@ extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_s
     ISSL_HIP(hipStreamSynchronize(ix->stream));
     ix->ws.pinned.release();
     *out = ix.release();
-    return ok();
+    return ISSL_OK;
+}
+
+}  // namespace issl
+
+extern "C" int issl_index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
+                                const uint64_t *sigs, const uint32_t *occ,
+                                const uint64_t *slice_masks, int device, issl_index **out) {
+    const int rc = index_build(n_sites, seq_len, n_slices, sigs, occ, false, slice_masks, device, out);
+    return rc == ISSL_OK ? ok() : rc;
 }
 
 extern "C" int issl_index_export(issl_index *ix, uint64_t *list_sizes, uint64_t *entries) {

@@ -269,6 +269,11 @@ int build_lstart(issl_index *ix, const std::vector<const uint64_t *> &sizes,
 int alloc_index(issl_index *ix, uint64_t lstart_count, bool postings);
 int upload(issl_index *ix, void *dst, const void *src, size_t bytes);
 void free_index(issl_index *ix);
+// issl_index_build over host arrays, or (on_device) over device arrays that
+// stay the caller's.
+int index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices, const uint64_t *sigs,
+                const uint32_t *occ, bool on_device, const uint64_t *slice_masks, int device,
+                issl_index **out);
 
 // Streamed index (stream.hip): postings stay in host memory and pass through
 // a bounded set of device slice buffers.

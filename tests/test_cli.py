@@ -208,3 +208,11 @@ def test_module_mmf_matches_resident(tmp_path, method, level, slots, gpu_availab
     rb = [l for l in b.stdout.splitlines() if l.startswith("RESULT") or "failed here" in l]
     assert ra == rb
     assert sum(1 for l in rb if l.startswith("RESULT")) == len(seqs)
+
+
+EXTRACT = os.path.join(BIN, "ExtractOfftargets")
+
+
+def test_extract_cli_usage():
+    r = run([EXTRACT, "out.txt"])
+    assert r.returncode == 1 and "Usage:" in r.stderr
