@@ -152,6 +152,59 @@ def stream_leg(issl, hiprt, hx, guides, ix, args) -> dict:
     return out
 
 
+def extract_leg(issl, args, masks) -> dict:
+    """Off-target extraction over a synthetic genome (41 % GC, uniform
+    bases, 250 Mbase chromosomes): scan, sort and count, the sorted file
+    formatted (to /dev/null) and an index built from the sites, all timed
+    on the device path; the oracle's single-thread scan on a sample beside
+    it."""
+    rng = np.random.default_rng(0xE7)
+    lut = np.frombuffer(b"A" * 29 + b"C" * 20 + b"G" * 21 + b"T" * 30, dtype=np.uint8)
+    n = args.extract_bases
+    genome = lut[rng.integers(0, 100, size=n, dtype=np.uint8)].tobytes()
+    chrom = 250_000_000
+    with issl.Extractor() as warm:  # loads the kernels and the sort's code
+        warm.add_sequence(genome[:1_000_000])
+        warm.finish()
+        warm.write(os.devnull)
+    out = {"bases": n, "chromosome_bases": chrom}
+    with issl.Extractor() as ex:
+        mv = memoryview(genome)
+        t0 = time.perf_counter()
+        for i in range(0, n, chrom):
+            ex.add_sequence(mv[i:i + chrom])
+        ex.stats()
+        t1 = time.perf_counter()
+        sites, distinct = ex.finish()
+        t2 = time.perf_counter()
+        ex.write(os.devnull)
+        t3 = time.perf_counter()
+        with ex.build_index(masks) as dix:
+            t4 = time.perf_counter()
+            info = dix.info()
+    out.update({"sites": sites, "distinct": distinct,
+                "scan_s": round(t1 - t0, 3), "sort_count_s": round(t2 - t1, 3),
+                "write_s": round(t3 - t2, 3), "index_build_s": round(t4 - t3, 3),
+                "scan_Gbases_per_s": round(n / (t1 - t0) / 1e9, 2),
+                "genome_to_sorted_sites_Gbases_per_s": round(n / (t2 - t0) / 1e9, 2),
+                "index_slices": int(info.n_slices)})
+    log(f"extraction: {n / 1e6:.0f} Mbases -> {sites:,} sites ({distinct:,} distinct): scan "
+        f"{t1 - t0:.3f} s, sort+count {t2 - t1:.3f} s, format {t3 - t2:.3f} s, index "
+        f"{t4 - t3:.3f} s")
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        sample = genome[:args.extract_cpu_bases]
+        t0 = time.perf_counter()
+        oracle_lib.lib().or_extract(sample, len(sample), None, 0)
+        dt = time.perf_counter() - t0
+        out["cpu_scan"] = {"bases_per_s": round(len(sample) / dt, 1), "cores": 1, "kind": "port",
+                           "sample": f"the genome's first {len(sample):,} bases, the oracle's "
+                                     f"pattern scan only (no sort)"}
+        log(f"extraction CPU scan: {len(sample) / dt / 1e6:.1f} Mbases/s on one core")
+    return out
+
+
 def serve_poisson(issl, srv, index_ids, pools, rate, n_requests, size_of, rng):
     """Open-loop Poisson arrivals at `rate` req/s through the request loop.
 
@@ -287,6 +340,9 @@ def main() -> int:
     ap.add_argument("--stream-guides", type=int, default=100_000,
                     help="guides per call of the streamed-index leg")
     ap.add_argument("--stream-calls", type=int, default=3)
+    ap.add_argument("--extract-bases", type=int, default=500_000_000,
+                    help="synthetic genome of the extraction leg (0: skip the leg)")
+    ap.add_argument("--extract-cpu-bases", type=int, default=50_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -435,6 +491,9 @@ def main() -> int:
     if args.stream_slots > 0:
         streamed = stream_leg(issl, hiprt, hx, guides, ix, args)
 
+    # ---- off-target extraction (the step before the index) on a synthetic genome
+    extraction = extract_leg(issl, args, masks) if args.extract_bases > 0 else None
+
     # ---- serving: Poisson arrivals of 1,000-guide requests through the request loop
     log(f"serving: {args.serve_requests} requests of 1,000 guides at {args.serve_rate:g} req/s")
     srv = issl.Server(max_batch=65536)
@@ -520,6 +579,7 @@ def main() -> int:
                         mode="request loop, Poisson arrivals, host buffers in/out"),
         "index_build": build,
         "streamed": streamed,
+        "extraction": extraction,
         "setup_s": round(setup_s, 1),
     }
     if cpu:

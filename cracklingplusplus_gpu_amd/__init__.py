@@ -445,8 +445,11 @@ class Extractor:
         self.close()
 
     def add_sequence(self, seq) -> None:
-        b = seq.encode() if isinstance(seq, str) else bytes(seq)
-        check(lib().issl_extractor_add_sequence(self._h, b, len(b)))
+        """Scan one sequence: str, bytes or any byte buffer (not copied here)."""
+        if isinstance(seq, str):
+            seq = seq.encode()
+        a = np.frombuffer(seq, dtype=np.uint8)
+        check(lib().issl_extractor_add_sequence(self._h, _ptr(a), a.shape[0]))
 
     def add_path(self, path: str) -> None:
         check(lib().issl_extractor_add_path(self._h, os.fsencode(path)))

@@ -134,7 +134,7 @@ SIGNATURES = [
     # issl_server.h
     ("issl_extractor_create", c_int, [c_int, c_uint64, POINTER(c_void_p)]),
     ("issl_extractor_destroy", c_int, [c_void_p]),
-    ("issl_extractor_add_sequence", c_int, [c_void_p, c_char_p, c_uint64]),
+    ("issl_extractor_add_sequence", c_int, [c_void_p, c_void_p, c_uint64]),
     ("issl_extractor_add_path", c_int, [c_void_p, c_char_p]),
     ("issl_extractor_finish", c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]),
     ("issl_extractor_sites", c_int, [c_void_p, c_void_p, c_void_p]),

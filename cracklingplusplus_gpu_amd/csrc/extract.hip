@@ -39,6 +39,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "device.hpp"
@@ -50,9 +51,14 @@ struct issl_extractor {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t chunk = 0;            // start positions per launch
-    HostBuf pin;                   // packed sequences or one piece: chunk + 23 bytes
-    uint64_t pending = 0;          // bytes in pin
-    DevBuf d_seq;                  // device copy of pin
+    // Two staging buffers used in turn: while the device copies and scans
+    // one, the host fills the other.
+    HostBuf pin[2];                // packed sequences or one piece: chunk + 23 bytes
+    DevBuf d_seq[2];               // device copies
+    int cur = 0;                   // buffer being filled
+    uint64_t pending = 0;          // bytes in pin[cur]
+    bool in_flight = false;        // a scan is queued whose count is not read yet
+    HostBuf h_count;               // the counter, copied back after each scan
     uint64_t *d_keys = nullptr;    // text-order keys of every site found
     uint64_t cap = 0, n_keys = 0;
     DevBuf d_count;                // device site counter
@@ -185,39 +191,77 @@ int ensure_cap(issl_extractor *ex, uint64_t need) {
     return ISSL_OK;
 }
 
-// Scan the first nbytes of the pinned buffer: start positions [0, npos).
-int scan_pinned(issl_extractor *ex, uint64_t nbytes, uint64_t npos) {
-    if (npos == 0 || nbytes < kWin) return ISSL_OK;
+// Copy with several host threads (staging is the scan's critical path).
+void par_copy(char *dst, const char *src, uint64_t n) {
+    constexpr uint64_t kMinPart = uint64_t(8) << 20;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const uint64_t T = std::min<uint64_t>(std::max(1u, std::min(hw ? hw : 1u, 8u)), n / kMinPart + 1);
+    if (T <= 1) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const uint64_t per = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (uint64_t t = 1; t < T; ++t) {
+        const uint64_t b = std::min(n, t * per), e = std::min(n, b + per);
+        th.emplace_back([=] { std::memcpy(dst + b, src + b, e - b); });
+    }
+    std::memcpy(dst, src, std::min(n, per));
+    for (std::thread &x : th) x.join();
+}
+
+// Wait for the queued scan and take the site count.
+int settle(issl_extractor *ex) {
+    if (!ex->in_flight) return ISSL_OK;
+    ISSL_HIP(hipStreamSynchronize(ex->stream));
+    ex->n_keys = *static_cast<const uint64_t *>(ex->h_count.p);
+    ex->in_flight = false;
+    return ISSL_OK;
+}
+
+// Queue the scan of the first nbytes of pin[cur] (start positions
+// [0, npos)) and switch to the other buffer.  The previous scan is settled
+// first, so its count sizes the key array exactly and the buffer switched
+// to is free again.
+int submit(issl_extractor *ex, uint64_t nbytes, uint64_t npos) {
     int rc;
+    if ((rc = settle(ex)) != ISSL_OK) return rc;
+    const int b = ex->cur;
+    ex->cur ^= 1;
+    if (npos == 0 || nbytes < kWin) return ISSL_OK;
     if ((rc = ensure_cap(ex, ex->n_keys + 2 * npos)) != ISSL_OK) return rc;
-    ISSL_HIP(hipMemcpyAsync(ex->d_seq.p, ex->pin.p, nbytes, hipMemcpyHostToDevice, ex->stream));
+    ISSL_HIP(hipMemcpyAsync(ex->d_seq[b].p, ex->pin[b].p, nbytes, hipMemcpyHostToDevice, ex->stream));
     hipLaunchKernelGGL(k_pam_scan, dim3(grid_for(npos, kScanBlock, 1u << 16)), dim3(kScanBlock), 0,
-                       ex->stream, static_cast<const uint8_t *>(ex->d_seq.p), nbytes, npos, ex->d_keys,
+                       ex->stream, static_cast<const uint8_t *>(ex->d_seq[b].p), nbytes, npos, ex->d_keys,
                        static_cast<unsigned long long *>(ex->d_count.p));
     ISSL_HIP(hipGetLastError());
-    unsigned long long total = 0;
-    ISSL_HIP(hipMemcpyAsync(&total, ex->d_count.p, 8, hipMemcpyDeviceToHost, ex->stream));
-    ISSL_HIP(hipStreamSynchronize(ex->stream));
-    ex->n_keys = total;
+    ISSL_HIP(hipMemcpyAsync(ex->h_count.p, ex->d_count.p, 8, hipMemcpyDeviceToHost, ex->stream));
+    ex->in_flight = true;
     return ISSL_OK;
 }
 
 int flush(issl_extractor *ex) {
     const uint64_t n = ex->pending;
     ex->pending = 0;
-    return scan_pinned(ex, n, n);
+    return n ? submit(ex, n, n) : ISSL_OK;
+}
+
+// Flush and settle: every site added so far is counted.
+int drain(issl_extractor *ex) {
+    const int rc = flush(ex);
+    return rc != ISSL_OK ? rc : settle(ex);
 }
 
 int add_sequence(issl_extractor *ex, const char *seq, uint64_t len) {
     ex->finished = false;
     ex->bases += len;
     if (len < kWin) return ISSL_OK;  // holds no window
-    char *pin = static_cast<char *>(ex->pin.p);
     int rc;
     if (len <= ex->chunk) {
         // Packed with the others, a separator byte after it.
         if (ex->pending + len + 1 > ex->chunk + kWin && (rc = flush(ex)) != ISSL_OK) return rc;
-        std::memcpy(pin + ex->pending, seq, len);
+        char *pin = static_cast<char *>(ex->pin[ex->cur].p);
+        par_copy(pin + ex->pending, seq, len);
         pin[ex->pending + len] = '\n';
         ex->pending += len + 1;
         return ISSL_OK;
@@ -227,8 +271,8 @@ int add_sequence(issl_extractor *ex, const char *seq, uint64_t len) {
     for (uint64_t off = 0; off < len; off += ex->chunk) {
         const uint64_t npos = std::min<uint64_t>(ex->chunk, len - off);
         const uint64_t nb = std::min<uint64_t>(len - off, ex->chunk + kWin - 1);
-        std::memcpy(pin, seq + off, nb);
-        if ((rc = scan_pinned(ex, nb, npos)) != ISSL_OK) return rc;
+        par_copy(static_cast<char *>(ex->pin[ex->cur].p), seq + off, nb);
+        if ((rc = submit(ex, nb, npos)) != ISSL_OK) return rc;
     }
     return ISSL_OK;
 }
@@ -315,10 +359,14 @@ extern "C" int issl_extractor_create(int device, uint64_t chunk_bytes, issl_extr
     DeviceGuard guard(device);
     ISSL_HIP(hipStreamCreateWithFlags(&ex->stream, hipStreamNonBlocking));
     int rc;
-    if ((rc = ex->pin.reserve(ex->chunk + kWin)) != ISSL_OK) return rc;
-    if ((rc = ex->d_seq.reserve(ex->chunk + kWin)) != ISSL_OK) return rc;
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = ex->pin[b].reserve(ex->chunk + kWin)) != ISSL_OK) return rc;
+        if ((rc = ex->d_seq[b].reserve(ex->chunk + kWin)) != ISSL_OK) return rc;
+    }
+    if ((rc = ex->h_count.reserve(8)) != ISSL_OK) return rc;
     if ((rc = ex->d_count.reserve(8)) != ISSL_OK) return rc;
     ISSL_HIP(hipMemsetAsync(ex->d_count.p, 0, 8, ex->stream));
+    *static_cast<uint64_t *>(ex->h_count.p) = 0;
     ISSL_HIP(hipStreamSynchronize(ex->stream));
     *out = ex.release();
     return ok();
@@ -329,7 +377,11 @@ extern "C" int issl_extractor_destroy(issl_extractor *ex) {
     DeviceGuard guard(ex->device);
     if (ex->stream) (void)hipStreamSynchronize(ex->stream);
     if (ex->d_keys) (void)hipFree(ex->d_keys);
-    ex->d_seq.release();
+    for (int b = 0; b < 2; ++b) {
+        ex->d_seq[b].release();
+        ex->pin[b].release();
+    }
+    ex->h_count.release();
     ex->d_count.release();
     ex->d_sorted.release();
     ex->d_uniq.release();
@@ -337,7 +389,6 @@ extern "C" int issl_extractor_destroy(issl_extractor *ex) {
     ex->d_nruns.release();
     ex->d_tmp.release();
     ex->d_out.release();
-    ex->pin.release();
     if (ex->stream) (void)hipStreamDestroy(ex->stream);
     delete ex;
     return ok();
@@ -378,7 +429,7 @@ extern "C" int issl_extractor_add_path(issl_extractor *ex, const char *path) {
 extern "C" int issl_extractor_finish(issl_extractor *ex, uint64_t *n_sites, uint64_t *n_distinct) {
     if (!ex) return fail(ISSL_ERR_ARG, "null extractor");
     DeviceGuard guard(ex->device);
-    int rc = flush(ex);
+    int rc = drain(ex);
     if (rc != ISSL_OK) return rc;
     const uint64_t n = ex->n_keys;
     if (n > kMaxSites)
@@ -485,7 +536,7 @@ extern "C" int issl_extractor_build_index(issl_extractor *ex, const uint64_t *sl
 extern "C" int issl_extractor_stats(issl_extractor *ex, uint64_t *bases, uint64_t *sites) {
     if (!ex) return fail(ISSL_ERR_ARG, "null extractor");
     DeviceGuard guard(ex->device);
-    const int rc = flush(ex);  // packed sequences not yet scanned
+    const int rc = drain(ex);  // packed sequences not yet scanned, the queued scan
     if (rc != ISSL_OK) return rc;
     if (bases) *bases = ex->bases;
     if (sites) *sites = ex->n_keys;
