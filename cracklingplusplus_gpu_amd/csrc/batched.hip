@@ -515,15 +515,18 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         // bits above them and the bit carried across the halves by the shift.
         const uint32_t ql = static_cast<uint32_t>(q), qh = static_cast<uint32_t>(q >> 32);
         const uint64_t *site_p = a.site + dl;   // this lane's current slice, by position
-        for (uint32_t base = a0; base < a1; base += 64 * DEPTH) {
+        // One round: DEPTH loads of 64 consecutive postings, then the filter.
+        // FULL rounds lie inside the tile; in the last round lanes past the
+        // tile read its last posting again (every lane loads without a
+        // branch) and fail the filter.
+        auto round = [&](uint32_t base, auto full_c) {
+            constexpr bool FULL = decltype(full_c)::value;
             uint64_t wv[DEPTH];
-            int sl[DEPTH];
+            uint32_t sl[DEPTH];
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
-                // Lanes past the tile read its last posting again and mark it
-                // out of range, so every lane loads without a branch.
                 const uint32_t pos = base + 64 * d + lane;
-                const uint32_t pc = std::min(pos, a1 - 1);
+                const uint32_t pc = FULL ? pos : std::min(pos, a1 - 1);
                 if (pc >= nb) {
                     do {
                         ++cur;
@@ -531,16 +534,18 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     } while (pc >= nb);
                     site_p = a.site + s_delta[wave][buf][cur];
                 }
-                sl[d] = pos < a1 ? cur : -1;
+                sl[d] = static_cast<uint32_t>(cur);
                 wv[d] = site_p[pc];
             }
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
-                const uint32_t xl = ql ^ static_cast<uint32_t>(wv[d]);
+                // All ones in the low half: 16 differing positions, no candidate.
+                const uint32_t kill = (FULL || base + 64 * d + lane < a1) ? 0u : ~0u;
+                const uint32_t xl = (ql ^ static_cast<uint32_t>(wv[d])) | kill;
                 const uint32_t xh = qh ^ static_cast<uint32_t>(wv[d] >> 32);
                 const uint32_t fl = ((xl >> 1) | xl) & 0x55555555u;
                 const uint32_t fh = ((xh >> 1) | xh) & 0x55u;
-                const bool cand = sl[d] >= 0 && __popc(fl) + __popc(fh) <= 4;
+                const bool cand = __popc(fl) + __popc(fh) <= 4;
                 const uint64_t b = __ballot(cand);
                 if (b) {
                     const uint32_t r = __builtin_amdgcn_mbcnt_hi(
@@ -548,8 +553,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
                     if (cand) {
                         const uint32_t e = (tail + r) & (kRing - 1);
-                        s_rpos[wave][e] = (base - a0 + 64 * d + lane) |
-                                          (static_cast<uint32_t>(sl[d]) << kRelBits);
+                        s_rpos[wave][e] = (base - a0 + 64 * d + lane) | (sl[d] << kRelBits);
                         s_rword[wave][e] = wv[d];
                     }
                     tail += static_cast<uint32_t>(__popcll(b));
@@ -557,7 +561,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 if ((d + 1) % kRingChk == 0 || d == DEPTH - 1)
                     while (tail - head >= 64) drain(64);
             }
-        }
+        };
+        uint32_t base = a0;
+        for (; base + 64 * DEPTH <= a1; base += 64 * DEPTH) round(base, std::true_type{});
+        if (base < a1) round(base, std::false_type{});
         while (tail != head) drain(std::min<uint32_t>(64, tail - head));
         if (htail != hhead) score_hits(htail - hhead);
         // Fixed butterfly: every lane ends with the same, order-determined value.
