@@ -179,6 +179,48 @@ __device__ __forceinline__ bool first_seen(const uint32_t p[4], int sl, const ui
     return (earlier & ~nm) == 0;
 }
 
+// Fold of one site as a 20-bit mask (bit j set iff position j differs),
+// from the two 32-bit halves of guide ^ site word: positions 0..15 in the
+// low half, 16..19 in bits 0..7 of the high half (the bits above are the
+// word's occurrence field and drop out).  Same positions as bfold.
+__device__ __forceinline__ uint32_t fold20(uint32_t xl, uint32_t xh) {
+    uint32_t a = ((xl >> 1) | xl) & 0x55555555u;
+    a = (a | (a >> 1)) & 0x33333333u;
+    a = (a | (a >> 2)) & 0x0F0F0F0Fu;
+    a = (a | (a >> 4)) & 0x00FF00FFu;
+    a = (a | (a >> 8)) & 0x0000FFFFu;
+    uint32_t b = ((xh >> 1) | xh) & 0x55u;
+    b = (b | (b >> 1)) & 0x33u;
+    b = (b | (b >> 2)) & 0x0Fu;
+    return a | (b << 16);
+}
+
+// Mismatch positions of a 20-bit fold, ascending; kNoPos past the count.
+__device__ __forceinline__ void mismatches20(uint32_t c, uint32_t p[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        p[i] = c ? static_cast<uint32_t>(__builtin_ctz(c)) : kNoPos;
+        c &= c - 1;
+    }
+}
+
+// MIT rank terms by position: rank = p0 + t[0][p1] + t[1][p2] + t[2][p3],
+// where t[k-2][p] = sum_{j<k} C(20, j) - 1 + C(p, k) -- the offsets of
+// mit_rank -- and 0 at kNoPos, so absent mismatches add nothing.  Built by
+// the threads of a workgroup.
+__device__ __forceinline__ void build_rank_terms(uint32_t (*t)[kNoPos + 1], int tid, int nthreads) {
+    for (int i = tid; i < 3 * static_cast<int>(kNoPos + 1); i += nthreads) {
+        const uint32_t k = static_cast<uint32_t>(i) / (kNoPos + 1), p = static_cast<uint32_t>(i) % (kNoPos + 1);
+        uint32_t v = 0;
+        if (p < 20) {
+            if (k == 0) v = 20u + p * (p - 1) / 2;
+            else if (k == 1) v = 190u + p * (p - 1) * (p - 2) / 6;
+            else v = 1140u + p * (p - 1) * (p - 2) * (p - 3) / 24;
+        }
+        t[k][p] = v;
+    }
+}
+
 // posmask (kNoPos + 1 entries) for the first-seen rule, built by threads
 // 0..kNoPos of a workgroup.
 __device__ __forceinline__ void build_posmask(uint64_t *posmask, const uint64_t *fold, int S,
@@ -321,6 +363,9 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __shared__ Real s_cfd[kCfdEntries];
     __shared__ uint64_t s_fold[kMaxSlices];
     __shared__ uint64_t s_posmask[kNoPos + 1];
+    __shared__ uint32_t s_rank[3][kNoPos + 1];
+    // The current tile's CFD factors by (position, guide ^ site there).
+    __shared__ Real s_cq[kTileWaves][(kNoPos + 1) * 4];
     // Tile plans, double-buffered: the next tile's plan lands in the other half.
     __shared__ uint32_t s_pre[kTileWaves][2][kMaxSlices + 1];
     __shared__ uint64_t s_delta[kTileWaves][2][kMaxSlices];
@@ -336,9 +381,11 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     const Real *mitg = static_cast<const Real *>(a.mit);
     for (int i = tid; i < kCfdEntries; i += kTileBlock) s_cfd[i] = cfdg[i];
     if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
+    build_rank_terms(s_rank, tid, kTileBlock);
     __syncthreads();
     build_posmask(s_posmask, s_fold, a.S, tid);
     __syncthreads();
+    const Real cfd_pam = s_cfd[320 + 10];   // NGG
     const int S = a.S;
     const uint32_t n = a.n;
     const uint32_t V = n * kMaxTiles;
@@ -411,6 +458,18 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         const uint64_t q = mc.q;
         const uint32_t a0 = mc.k * mc.C;
         const uint32_t a1 = std::min(mc.P, a0 + mc.C);
+        // Entry 4p + x: the factor of a mismatch at position p where guide ^
+        // site = x (lib/ISSLScoringModule.cpp:308-361); 1 at kNoPos.
+        for (int i = lane; i < static_cast<int>(kNoPos + 1) * 4; i += 64) {
+            const uint32_t pp = static_cast<uint32_t>(i) >> 2, x = static_cast<uint32_t>(i) & 3u;
+            Real v = Real(1);
+            if (pp < 20) {
+                const uint32_t g = static_cast<uint32_t>(q >> (2 * pp)) & 3u;
+                v = s_cfd[(pp << 4) | (g << 2) | ((g ^ x) ^ 3u)];
+            }
+            s_cq[wave][i] = v;
+        }
+        __builtin_amdgcn_wave_barrier();
 
         Real am = 0, ac = 0;
         F2 fm{0.f, 0.f}, fc{0.f, 0.f};
@@ -451,8 +510,20 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 sites += 1;
                 const Real ro = to_real<Real>(occ);
                 Real tm = 0, tc = 0;
-                if (a.calc_mit && dist > 0) tm = mitg[mit_rank(mp, dist)] * ro;
-                if (a.calc_cfd) tc = cfd_product<Real>(q, sg, mp, dist, s_cfd) * ro;
+                if (a.calc_mit && dist > 0)
+                    tm = mitg[mp[0] + s_rank[0][mp[1]] + s_rank[1][mp[2]] + s_rank[2][mp[3]]] * ro;
+                if (a.calc_cfd) {
+                    // PAM factor, then one factor per mismatch in position
+                    // order (1 for absent ones): the order of cfd_product.
+                    const uint64_t xw = q ^ sg;
+                    Real c = cfd_pam;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t xv = static_cast<uint32_t>(xw >> ((2 * mp[i]) & 63u)) & 3u;
+                        c = c * s_cq[wave][(mp[i] << 2) | xv];
+                    }
+                    tc = (dist == 0 ? Real(1) : c) * ro;
+                }
                 if constexpr (PREC == ISSL_PREC_FP32C) {
                     fm = f2_add(fm, tm);
                     fc = f2_add(fc, tc);
@@ -474,10 +545,11 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 ent = s_rpos[wave][e];
                 wv = s_rword[wave][e];
                 const int sl = static_cast<int>(ent >> kRelBits);
-                const uint64_t f = bfold(q ^ (wv & kSigBits));
-                const uint32_t dist = static_cast<uint32_t>(__popcll(f));
+                const uint32_t f = fold20(static_cast<uint32_t>(q ^ wv),
+                                          static_cast<uint32_t>((q ^ wv) >> 32));
+                const uint32_t dist = static_cast<uint32_t>(__popc(f));
                 uint32_t mp[4];
-                mismatches(f, mp);
+                mismatches20(f, mp);
                 hit = first_seen(mp, sl, s_posmask);
                 pk = mp[0] | (mp[1] << 6) | (mp[2] << 12) | (mp[3] << 18) | (dist << 24);
             }
