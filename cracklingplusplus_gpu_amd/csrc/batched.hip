@@ -204,20 +204,22 @@ __device__ __forceinline__ void mismatches20(uint32_t c, uint32_t p[4]) {
     }
 }
 
-// MIT rank terms by position: rank = p0 + t[0][p1] + t[1][p2] + t[2][p3],
-// where t[k-2][p] = sum_{j<k} C(20, j) - 1 + C(p, k) -- the offsets of
-// mit_rank -- and 0 at kNoPos, so absent mismatches add nothing.  Built by
-// the threads of a workgroup.
-__device__ __forceinline__ void build_rank_terms(uint32_t (*t)[kNoPos + 1], int tid, int nthreads) {
-    for (int i = tid; i < 3 * static_cast<int>(kNoPos + 1); i += nthreads) {
-        const uint32_t k = static_cast<uint32_t>(i) / (kNoPos + 1), p = static_cast<uint32_t>(i) % (kNoPos + 1);
+// MIT rank terms by doubled position: rank = p0 + t[0][2 p1] + t[1][2 p2] +
+// t[2][2 p3], where t[k-2][2p] = sum_{j<k} C(20, j) - 1 + C(p, k) -- the
+// offsets of mit_rank -- and 0 at kNoPos, so absent mismatches add nothing.
+// Built by the threads of a workgroup.
+constexpr int kRankTerms = 2 * (kNoPos + 1);
+__device__ __forceinline__ void build_rank_terms(uint32_t (*t)[kRankTerms], int tid, int nthreads) {
+    for (int i = tid; i < 3 * kRankTerms; i += nthreads) {
+        const uint32_t k = static_cast<uint32_t>(i) / kRankTerms;
+        const uint32_t p = (static_cast<uint32_t>(i) % kRankTerms) / 2;
         uint32_t v = 0;
         if (p < 20) {
             if (k == 0) v = 20u + p * (p - 1) / 2;
             else if (k == 1) v = 190u + p * (p - 1) * (p - 2) / 6;
             else v = 1140u + p * (p - 1) * (p - 2) * (p - 3) / 24;
         }
-        t[k][p] = v;
+        t[k][i % kRankTerms] = v;
     }
 }
 
@@ -363,7 +365,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __shared__ Real s_cfd[kCfdEntries];
     __shared__ uint64_t s_fold[kMaxSlices];
     __shared__ uint64_t s_posmask[kNoPos + 1];
-    __shared__ uint32_t s_rank[3][kNoPos + 1];
+    __shared__ uint32_t s_rank[3][kRankTerms];
     // The current tile's CFD factors by (position, guide ^ site there).
     __shared__ Real s_cq[kTileWaves][(kNoPos + 1) * 4];
     // Tile plans, double-buffered: the next tile's plan lands in the other half.
@@ -490,9 +492,9 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 const uint32_t ent = s_hpos[wave][e];
                 const uint64_t wv = s_hword[wave][e];
                 const uint32_t pk = s_hmis[wave][e];
-                const uint32_t mp[4] = {pk & 63u, (pk >> 6) & 63u, (pk >> 12) & 63u,
-                                        (pk >> 18) & 63u};
-                const int dist = static_cast<int>(pk >> 24);
+                const uint32_t m2[4] = {pk & 127u, (pk >> 7) & 127u, (pk >> 14) & 127u,
+                                        (pk >> 21) & 127u};   // 2p
+                const int dist = static_cast<int>(pk >> 28);
                 const uint64_t sg = wv & kSigBits;
                 uint32_t occ = static_cast<uint32_t>(wv >> 40);
                 if (occ == kOccEscape || a.mark_distinct) {
@@ -508,10 +510,15 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 }
                 occ_sum += occ;
                 sites += 1;
+                // Keep occ a 32-bit value: otherwise the compiler widens it
+                // back to the 64-bit word it was cut from and converts that.
+                asm("" : "+v"(occ));
                 const Real ro = to_real<Real>(occ);
-                Real tm = 0, tc = 0;
-                if (a.calc_mit && dist > 0)
-                    tm = mitg[mp[0] + s_rank[0][mp[1]] + s_rank[1][mp[2]] + s_rank[2][mp[3]]] * ro;
+                Real tm = 0, tc = 0, mv = 0;
+                // The MIT load goes first so the CFD work covers its latency;
+                // an exact match (dist 0, all positions kNoPos) reads a
+                // valid entry that is then not used.
+                if (a.calc_mit) mv = mitg[(m2[0] >> 1) + s_rank[0][m2[1]] + s_rank[1][m2[2]] + s_rank[2][m2[3]]];
                 if (a.calc_cfd) {
                     // PAM factor, then one factor per mismatch in position
                     // order (1 for absent ones): the order of cfd_product.
@@ -519,11 +526,12 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     Real c = cfd_pam;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const uint32_t xv = static_cast<uint32_t>(xw >> ((2 * mp[i]) & 63u)) & 3u;
-                        c = c * s_cq[wave][(mp[i] << 2) | xv];
+                        const uint32_t xv = static_cast<uint32_t>(xw >> (m2[i] & 63u)) & 3u;
+                        c = c * s_cq[wave][m2[i] * 2 + xv];
                     }
                     tc = (dist == 0 ? Real(1) : c) * ro;
                 }
+                if (a.calc_mit && dist > 0) tm = mv * ro;
                 if constexpr (PREC == ISSL_PREC_FP32C) {
                     fm = f2_add(fm, tm);
                     fc = f2_add(fc, tc);
@@ -551,7 +559,8 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 uint32_t mp[4];
                 mismatches20(f, mp);
                 hit = first_seen(mp, sl, s_posmask);
-                pk = mp[0] | (mp[1] << 6) | (mp[2] << 12) | (mp[3] << 18) | (dist << 24);
+                // doubled positions (the shift that picks a base), 7 bits each
+                pk = (mp[0] << 1) | (mp[1] << 8) | (mp[2] << 15) | (mp[3] << 22) | (dist << 28);
             }
             head += cnt;
             const uint64_t b = __ballot(hit);
