@@ -28,9 +28,11 @@
 // 32 bits.
 //
 // Precision: ISSL_PREC_FP32C forms each site's MIT/CFD product in FP32 and
-// carries every sum as an unevaluated pair (hi, lo) of FP32 built with
-// error-free TwoSum; FP64 appears only for hi+lo, the 10000/(100+S)
-// transform and the threshold tests.  ISSL_PREC_FP64 uses FP64 throughout.
+// carries each lane's sums as unevaluated pairs (hi, lo) of FP32 built with
+// error-free TwoSum; a lane's pair becomes an exact double, and the 64 lane
+// partials, the tiles and the exit-tile rescan combine in FP64, as do the
+// 10000/(100+S) transform and the threshold tests.  ISSL_PREC_FP64 uses FP64
+// throughout.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,15 +48,10 @@ constexpr int kMaxTiles = 64;        // tiles per guide at most
 constexpr int kTileBlock = 256;      // threads per persistent workgroup
 constexpr int kTileWaves = kTileBlock / 64;
 
+// Sums of one tile.  Both precisions combine lane partials and tiles in
+// FP64; FP32C accumulates each lane's terms as compensated FP32 pairs first.
 struct TilePart {
-    union {
-        struct {
-            float mh, ml, ch, cl;
-        } f;
-        struct {
-            double m, c;
-        } d;
-    };
+    double m, c;
     uint64_t occ;
     uint64_t sites;
 };
@@ -244,6 +241,39 @@ __device__ __forceinline__ bool bexit(int method, double m, double c, double cut
     case ISSL_METHOD_MIT: return m > cut;
     default: return c > cut;
     }
+}
+
+// Value of lane (this lane ^ M): ds_swizzle's bit mode within 32-lane
+// halves for M < 32 (no index arithmetic), ds_bpermute for M = 32.
+template <int M>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+    if constexpr (M < 32) {
+        return static_cast<uint32_t>(
+            __builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x1F | (M << 10)));
+    } else {
+        const int idx = static_cast<int>((__lane_id() ^ 32u) << 2);
+        return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(idx, static_cast<int>(v)));
+    }
+}
+template <int M>
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t v) {
+    return (static_cast<uint64_t>(xor_lane<M>(static_cast<uint32_t>(v >> 32))) << 32) |
+           xor_lane<M>(static_cast<uint32_t>(v));
+}
+template <int M>
+__device__ __forceinline__ double xor_lane_f64(double v) {
+    return __longlong_as_double(static_cast<long long>(
+        xor_lane64<M>(static_cast<uint64_t>(__double_as_longlong(v)))));
+}
+
+// One butterfly step of the tile reduction: every lane adds its partner's
+// partial (a + b and b + a agree, so all lanes hold the same sums after).
+template <int M>
+__device__ __forceinline__ void tile_reduce_step(double &m, double &c, uint64_t &occ, uint32_t &sites) {
+    m += xor_lane_f64<M>(m);
+    c += xor_lane_f64<M>(c);
+    occ += xor_lane64<M>(occ);
+    sites += xor_lane<M>(sites);
 }
 
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
@@ -648,48 +678,32 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         if (base < a1) round(base, std::false_type{});
         while (tail != head) drain(std::min<uint32_t>(64, tail - head));
         if (htail != hhead) score_hits(htail - hhead);
-        // Fixed butterfly: every lane ends with the same, order-determined value.
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            occ_sum += shfl64(occ_sum, lane ^ off);
-            sites += static_cast<uint32_t>(__shfl_xor(static_cast<int>(sites), off));
-            if constexpr (PREC == ISSL_PREC_FP32C) {
-                F2 om{__shfl_xor(fm.s, off), __shfl_xor(fm.c, off)};
-                F2 oc{__shfl_xor(fc.s, off), __shfl_xor(fc.c, off)};
-                fm = f2_merge(fm, om);
-                fc = f2_merge(fc, oc);
-            } else {
-                am += __shfl_xor(am, off);
-                ac += __shfl_xor(ac, off);
-            }
+        // Lane partials as doubles (a compensated pair converts exactly),
+        // combined in a fixed butterfly: every lane ends with the same sums.
+        double dm, dc;
+        if constexpr (PREC == ISSL_PREC_FP32C) {
+            dm = static_cast<double>(fm.s) + static_cast<double>(fm.c);
+            dc = static_cast<double>(fc.s) + static_cast<double>(fc.c);
+        } else {
+            dm = am;
+            dc = ac;
         }
+        tile_reduce_step<32>(dm, dc, occ_sum, sites);
+        tile_reduce_step<16>(dm, dc, occ_sum, sites);
+        tile_reduce_step<8>(dm, dc, occ_sum, sites);
+        tile_reduce_step<4>(dm, dc, occ_sum, sites);
+        tile_reduce_step<2>(dm, dc, occ_sum, sites);
+        tile_reduce_step<1>(dm, dc, occ_sum, sites);
         if (lane == 0) {
             TilePart tp;
-            if constexpr (PREC == ISSL_PREC_FP32C) {
-                tp.f.mh = fm.s;
-                tp.f.ml = fm.c;
-                tp.f.ch = fc.s;
-                tp.f.cl = fc.c;
-            } else {
-                tp.d.m = am;
-                tp.d.c = ac;
-            }
+            tp.m = dm;
+            tp.c = dc;
             tp.occ = occ_sum;
             tp.sites = sites;
             a.ws.part[vc] = tp;
             // A guide with a single tile is complete: its sum is this partial
             // (what the finalizer would form: 0 + partial, exactly).
-            if (mc.K == 1 && a.finish_single) {
-                double smit, scfd;
-                if constexpr (PREC == ISSL_PREC_FP32C) {
-                    smit = f2_value(fm);
-                    scfd = f2_value(fc);
-                } else {
-                    smit = am;
-                    scfd = ac;
-                }
-                write_guide(a.sc, a.out, mc.g, smit, scfd, occ_sum, sites);
-            }
+            if (mc.K == 1 && a.finish_single) write_guide(a.sc, a.out, mc.g, dm, dc, occ_sum, sites);
         }
         if (vn == kNoSlot) break;
         buf ^= 1;
@@ -744,20 +758,15 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     TilePart mine{};
     if (lane < static_cast<int>(K)) mine = a.ws.part[static_cast<uint64_t>(lane) * a.n + g];
 
-    // Sums so far: zero, or the state carried from earlier slice windows.
-    F2 fm{0.f, 0.f}, fc{0.f, 0.f};
+    // Sums so far (FP64): zero, or the state carried from earlier slice
+    // windows.
     double dm = 0, dc = 0;
     uint64_t occ_sum = 0, sites = 0;
     bool exited = false;
     if (a.state_in) {
         const GuideState st = a.state_in[g];
-        if constexpr (PREC == ISSL_PREC_FP32C) {
-            fm = F2{st.f.mh, st.f.ml};
-            fc = F2{st.f.ch, st.f.cl};
-        } else {
-            dm = st.d.m;
-            dc = st.d.c;
-        }
+        dm = st.m;
+        dc = st.c;
         occ_sum = st.occ;
         sites = st.sites & ~kExited;
         exited = (st.sites & kExited) != 0;
@@ -779,18 +788,10 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     for (uint32_t k = 0; k < K; ++k) {
         const uint64_t o = shfl64(mine.occ, k);
         const uint64_t si = shfl64(mine.sites, k);
-        const F2 pm = fm, pc = fc;
         const double qm = dm, qc = dc;
-        if constexpr (PREC == ISSL_PREC_FP32C) {
-            fm = f2_merge(fm, F2{__shfl(mine.f.mh, k), __shfl(mine.f.ml, k)});
-            fc = f2_merge(fc, F2{__shfl(mine.f.ch, k), __shfl(mine.f.cl, k)});
-        } else {
-            dm += __shfl(mine.d.m, k);
-            dc += __shfl(mine.d.c, k);
-        }
-        const double vm = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
-        const double vc = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
-        if (!(a.sc.use_cut && bexit(a.sc.method, vm, vc, a.sc.cut))) {
+        dm += __shfl(mine.m, k);
+        dc += __shfl(mine.c, k);
+        if (!(a.sc.use_cut && bexit(a.sc.method, dm, dc, a.sc.cut))) {
             occ_sum += o;
             sites += si;
             continue;
@@ -798,8 +799,6 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
         // The total meets the exit test within tile k: restore the sums before
         // it and add its hits one at a time, in order, to stop exactly where
         // the reference's loop stops (lib/ISSLScoringModule.cpp:365-401).
-        fm = pm;
-        fc = pc;
         dm = qm;
         dc = qc;
         load_lists();
@@ -840,25 +839,15 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
                 bal &= bal - 1;
                 const Real hm = __shfl(tm, h), hc = __shfl(tc, h);
                 const uint32_t ho = static_cast<uint32_t>(__shfl(static_cast<int>(occ), h));
-                double vm2, vc2;
-                if constexpr (PREC == ISSL_PREC_FP32C) {
-                    fm = f2_add(fm, hm);
-                    fc = f2_add(fc, hc);
-                    vm2 = f2_value(fm);
-                    vc2 = f2_value(fc);
-                } else {
-                    dm += hm;
-                    dc += hc;
-                    vm2 = dm;
-                    vc2 = dc;
-                }
+                dm += static_cast<double>(hm);
+                dc += static_cast<double>(hc);
                 occ_sum += ho;
                 sites += 1;
                 if (a.distinct && lane == h)
                     atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
                                  (static_cast<uint32_t>(a.post[addr]) >> 6),
                              1ULL << (static_cast<uint32_t>(a.post[addr]) & 63));
-                if (bexit(a.sc.method, vm2, vc2, a.sc.cut)) {
+                if (bexit(a.sc.method, dm, dc, a.sc.cut)) {
                     done = true;
                     break;
                 }
@@ -896,23 +885,14 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     if (lane != 0) return;
     if (a.state_out) {
         GuideState st;
-        if constexpr (PREC == ISSL_PREC_FP32C) {
-            st.f.mh = fm.s;
-            st.f.ml = fm.c;
-            st.f.ch = fc.s;
-            st.f.cl = fc.c;
-        } else {
-            st.d.m = dm;
-            st.d.c = dc;
-        }
+        st.m = dm;
+        st.c = dc;
         st.occ = occ_sum;
         st.sites = sites | (exited ? kExited : 0);
         a.state_out[g] = st;
         return;
     }
-    const double smit = (PREC == ISSL_PREC_FP32C) ? f2_value(fm) : dm;
-    const double scfd = (PREC == ISSL_PREC_FP32C) ? f2_value(fc) : dc;
-    write_guide(a.sc, a.out, g, smit, scfd, occ_sum, sites);
+    write_guide(a.sc, a.out, g, dm, dc, occ_sum, sites);
 }
 
 // ------------------------------------------------------------ direct

@@ -213,8 +213,8 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
     bool was_exited = false;
     if (a.state_in) {
         const GuideState st = a.state_in[g];
-        smit = st.d.m;
-        scfd = st.d.c;
+        smit = st.m;
+        scfd = st.c;
         occ_sum = st.occ;
         site_sum = st.sites & ~kExited;
         was_exited = (st.sites & kExited) != 0;
@@ -355,8 +355,8 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
     if (tid != 0) return;
     if (a.state_out) {
         GuideState st;
-        st.d.m = smit;
-        st.d.c = scfd;
+        st.m = smit;
+        st.c = scfd;
         st.occ = occ_sum;
         st.sites = site_sum | ((was_exited || s_exited) ? kExited : 0);
         a.state_out[g] = st;

@@ -104,18 +104,11 @@ struct Workspace {
     HostBuf pinned;
 };
 
-// Running state of one guide between slice windows: the two sums (FP64, or
-// the FP32 pairs of the compensated form), the occurrence and site counts,
-// and in bit 63 of `sites` whether the early exit has been taken.
+// Running state of one guide between slice windows: the two sums in FP64,
+// the occurrence and site counts, and in bit 63 of `sites` whether the early
+// exit has been taken.
 struct GuideState {
-    union {
-        struct {
-            float mh, ml, ch, cl;
-        } f;
-        struct {
-            double m, c;
-        } d;
-    };
+    double m, c;
     uint64_t occ;
     uint64_t sites;
 };
