@@ -11,9 +11,9 @@
 //   * k_pam_scan tests every start position of a chunk (one thread each,
 //     the 23-base windows of a 256-position tile staged in LDS) for the
 //     forward pattern [ACG]N{19}N[AG]G and the reverse pattern
-//     C[CT]NN{19}[TGC], and appends each site's text-order key (base 0 in
-//     the top bits, so numeric order is the reference's string order) with
-//     one atomic per wave;
+//     C[CT]NN{19}[TGC], and stages each site's text-order key (base 0 in
+//     the top bits, so numeric order is the reference's string order) in
+//     LDS; a staged batch takes its place in HBM with one atomic;
 //   * finish sorts the keys with a hipCUB radix sort over their 40 bits and
 //     run-length encodes them: the distinct sites in text order and their
 //     occurrence counts, which is what ISSLCreateIndex.cpp:396-421 derives
@@ -84,16 +84,15 @@ __device__ __forceinline__ uint32_t base_code(uint8_t c) {
     return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : c == 'T' ? 3u : 4u;
 }
 
-__device__ __forceinline__ unsigned long long bcast64(unsigned long long v) {
-    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(v), 0));
-    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(v >> 32), 0));
-    return (static_cast<unsigned long long>(hi) << 32) | lo;
-}
-
 __device__ __forceinline__ uint32_t lanes_below(uint64_t b) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
 }
+
+// Keys staged per workgroup before they go to HBM: one atomic reserves the
+// space for a whole batch (a per-wave atomic on one counter serializes in
+// L2), and the batch is written out coalesced.
+constexpr int kStageKeys = 4096;
 
 // Sites starting at positions [0, npos) of seq[0, nbytes) (windows past
 // nbytes do not match).  keys must hold *count + 2 * npos entries.
@@ -101,8 +100,21 @@ __global__ __launch_bounds__(kScanBlock) void k_pam_scan(const uint8_t *__restri
                                                          uint64_t nbytes, uint64_t npos,
                                                          uint64_t *__restrict__ keys,
                                                          unsigned long long *__restrict__ count) {
+    constexpr int kWaves = kScanBlock / 64;
     __shared__ uint8_t code[kScanBlock + kWin - 1];
-    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ uint64_t stage[kStageKeys];
+    __shared__ uint32_t s_wcnt[kWaves];
+    __shared__ unsigned long long s_gbase;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t fill = 0;  // keys staged (uniform)
+    auto flush = [&]() {
+        if (tid == 0) s_gbase = atomicAdd(count, static_cast<unsigned long long>(fill));
+        __syncthreads();
+        const unsigned long long g = s_gbase;
+        for (uint32_t i = tid; i < fill; i += kScanBlock) keys[g + i] = stage[i];
+        __syncthreads();
+        fill = 0;
+    };
     for (uint64_t base = blockIdx.x * static_cast<uint64_t>(kScanBlock); base < npos;
          base += static_cast<uint64_t>(gridDim.x) * kScanBlock) {
         for (int t = tid; t < kScanBlock + kWin - 1; t += kScanBlock) {
@@ -123,26 +135,32 @@ __global__ __launch_bounds__(kScanBlock) void k_pam_scan(const uint8_t *__restri
                 f = c0 != 3 && (c21 | 2u) == 2u && c22 == 2;
                 // reverse (?=(C[CT][ACGT][ACGT]{19}[TGC])) (ExtractOfftargets.cpp:17)
                 r = c0 == 1 && (c1 & 1u) == 1u && c22 != 0;
-                for (int j = 0; j < kSiteLen; ++j) {
-                    kf = (kf << 2) | code[tid + j];
-                    // reverse complement (lib/util.cpp:50-63): base j of the
-                    // site is the complement of base 22 - j of the window
-                    kr = (kr << 2) | (3u - code[tid + 22 - j]);
+                if (f || r) {
+                    for (int j = 0; j < kSiteLen; ++j) {
+                        kf = (kf << 2) | code[tid + j];
+                        // reverse complement (lib/util.cpp:50-63): base j of the
+                        // site is the complement of base 22 - j of the window
+                        kr = (kr << 2) | (3u - code[tid + 22 - j]);
+                    }
                 }
             }
         }
         const uint64_t bf = __ballot(f), br = __ballot(r);
         const uint32_t nf = static_cast<uint32_t>(__popcll(bf));
-        const uint32_t nr = static_cast<uint32_t>(__popcll(br));
-        if (nf + nr) {  // uniform per wave
-            unsigned long long w = 0;
-            if (lane == 0) w = atomicAdd(count, static_cast<unsigned long long>(nf + nr));
-            w = bcast64(w);
-            if (f) keys[w + lanes_below(bf)] = kf;
-            if (r) keys[w + nf + lanes_below(br)] = kr;
-        }
+        if (lane == 0) s_wcnt[wave] = nf + static_cast<uint32_t>(__popcll(br));
         __syncthreads();
+        uint32_t woff = fill, total = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            woff += w < wave ? s_wcnt[w] : 0u;
+            total += s_wcnt[w];
+        }
+        if (f) stage[woff + lanes_below(bf)] = kf;
+        if (r) stage[woff + nf + lanes_below(br)] = kr;
+        fill += total;
+        __syncthreads();  // code[] and s_wcnt are rewritten next
+        if (fill > static_cast<uint32_t>(kStageKeys - 2 * kScanBlock)) flush();
     }
+    if (fill) flush();
 }
 
 // Text-order key -> signature (position j in bits 2j).
@@ -231,7 +249,7 @@ int submit(issl_extractor *ex, uint64_t nbytes, uint64_t npos) {
     if (npos == 0 || nbytes < kWin) return ISSL_OK;
     if ((rc = ensure_cap(ex, ex->n_keys + 2 * npos)) != ISSL_OK) return rc;
     ISSL_HIP(hipMemcpyAsync(ex->d_seq[b].p, ex->pin[b].p, nbytes, hipMemcpyHostToDevice, ex->stream));
-    hipLaunchKernelGGL(k_pam_scan, dim3(grid_for(npos, kScanBlock, 1u << 16)), dim3(kScanBlock), 0,
+    hipLaunchKernelGGL(k_pam_scan, dim3(grid_for(npos, kScanBlock, 4096)), dim3(kScanBlock), 0,
                        ex->stream, static_cast<const uint8_t *>(ex->d_seq[b].p), nbytes, npos, ex->d_keys,
                        static_cast<unsigned long long *>(ex->d_count.p));
     ISSL_HIP(hipGetLastError());
