@@ -8,7 +8,8 @@ MIT+CFD (method mitAndCfd), score threshold disabled, maxDist 4.
 
 One step = scoring the 10,000 guides once, with the index and the guides
 already resident in HBM (issl_score_device).  `value` is guides/s of the
-BATCHED form (persistent tile kernel, compensated FP32 sums); the
+BATCHED form (persistent tile kernel, FP32 products with compensated FP32
+lane sums combined in FP64); the
 PLAIN form (one guide per workgroup, FP64) and the CPU scorer (the oracle's
 OpenMP restatement of ISSLScoringModule::run on the host cores, same
 workload) are reported beside it.

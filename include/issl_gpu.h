@@ -57,8 +57,9 @@ extern "C" {
 
 /* Precision of the BATCHED form (PLAIN is always FP64).
  *   FP64  : FP64 per-site terms and sums.
- *   FP32C : FP32 per-site products, compensated two-float FP32 sums, FP64
- *           only for the final total, the transform and the threshold test. */
+ *   FP32C : FP32 per-site products, each lane's sums as compensated
+ *           two-float FP32 pairs; lane and tile partials, the transform and
+ *           the threshold tests in FP64. */
 #define ISSL_PREC_FP64   0
 #define ISSL_PREC_FP32C  1
 

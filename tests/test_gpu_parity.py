@@ -5,7 +5,8 @@ Tolerances:
     calls for every method, threshold on and off.
   * BATCHED FP64: scores within 1e-12 relative (summation order differs);
     counts and candidate sets exact.
-  * BATCHED FP32C (FP32 products, compensated FP32 sums): scores within
+  * BATCHED FP32C (FP32 products, compensated FP32 lane sums, FP64
+    combination of lanes and tiles): scores within
     1e-6 relative with the threshold disabled; calls identical with the
     threshold on except guides whose FP64 sum lies within 1e-6 relative of
     the cutoff.
