@@ -152,8 +152,8 @@ __device__ void write_result(const DevResults &o, uint64_t g, const ScoreScalars
 // ------------------------------------------------------------ PLAIN kernel
 
 struct PlainArgs {
-    const uint64_t *sig;
     const uint64_t *post;
+    const uint64_t *inl;            // signature-inline site words, same order as post
     const uint32_t *lstart;
     const SliceMeta *meta;
     const double *mit64;
@@ -176,7 +176,9 @@ constexpr int kPlainPer = 4;  // postings per thread per chunk
 
 // One workgroup per guide.  Each chunk of BLOCK*PER postings of the guide's
 // slice-major sequence is read with coalesced loads (consecutive threads,
-// consecutive entries), the site signatures are gathered, and sites within
+// consecutive entries) of the signature-inline words, which carry each
+// site's signature and occurrences (the posting entry is read only for an
+// occurrence escape or a site id), and sites within
 // 4 mismatches that no earlier slice lists (the reference's first-seen
 // toggle, ISSLScoringModule.cpp:252-258, as a stateless rule: slice s'
 // lists the site iff the fold misses all of s'’s positions) become hits.
@@ -246,22 +248,23 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
     int parity = 0;
 
     for (uint64_t base = 0; base < P; base += CH) {
-        uint64_t ent[PER], site[PER];
+        uint64_t wv[PER], addr[PER], site[PER];
         int sl[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const uint64_t i = base + static_cast<uint64_t>(k) * BLOCK + tid;
             sl[k] = -1;
-            ent[k] = 0;
+            wv[k] = 0;
+            addr[k] = 0;
             if (i < P) {
                 while (i >= s_pre[cur + 1]) ++cur;
                 sl[k] = cur;
-                ent[k] = a.post[s_beg[cur] + (i - s_pre[cur])];
+                addr[k] = s_beg[cur] + (i - s_pre[cur]);
+                wv[k] = a.inl[addr[k]];
             }
         }
 #pragma unroll
-        for (int k = 0; k < PER; ++k)
-            site[k] = sl[k] >= 0 ? (a.sig[static_cast<uint32_t>(ent[k])] & kSigBits) : 0;
+        for (int k = 0; k < PER; ++k) site[k] = wv[k] & kSigBits;
 
         bool hit[PER];
         uint64_t fold[PER];
@@ -310,7 +313,8 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
         for (int k = 0; k < PER; ++k) {
             if (!hit[k]) continue;
             const int dist = __popcll(fold[k]);
-            const uint32_t occ = static_cast<uint32_t>(ent[k] >> 32);
+            uint32_t occ = static_cast<uint32_t>(wv[k] >> 40);
+            if (occ == kOccEscape) occ = static_cast<uint32_t>(a.post[addr[k]] >> 32);
             double tm = 0.0, tc = 0.0;
             if (a.sc.calc_mit && dist > 0) tm = a.mit64[mit_rank(fold[k], dist)] * static_cast<double>(occ);
             if (a.sc.calc_cfd) tc = cfd_site64(q, site[k], fold[k], dist, a.cfd64) * static_cast<double>(occ);
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
                 if (!hit[k] || off[k] >= acc) continue;
-                const uint32_t id = static_cast<uint32_t>(ent[k]);
+                const uint32_t id = static_cast<uint32_t>(a.post[addr[k]]);
                 if (a.distinct)
                     atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (id >> 6),
                              1ULL << (id & 63));
@@ -373,8 +377,8 @@ int launch_plain(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     if (win && cand_ids) return fail(ISSL_ERR_ARG, "candidate lists need a resident index");
     if (!win && !ix->d_post) return fail(ISSL_ERR_ARG, "this operation needs a resident index");
     PlainArgs a;
-    a.sig = ix->d_sig;
     a.post = win ? win->post : ix->d_post;
+    a.inl = win ? win->inl : ix->d_inline;
     a.s0 = win ? win->s0 : 0;
     a.s1 = win ? win->s1 : static_cast<int>(ix->S);
     a.state_in = win ? win->state_in : nullptr;

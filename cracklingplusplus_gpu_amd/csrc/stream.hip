@@ -235,7 +235,7 @@ int score_streamed(issl_index *ix, const uint64_t *d_guides, uint64_t n, const i
                    hipStream_t st) {
     StreamSource &src = *ix->src;
     const int S = static_cast<int>(ix->S);
-    const bool batched = p->form == ISSL_FORM_BATCHED;
+    const bool batched = p->form == ISSL_FORM_BATCHED;  // both forms read the inline words
     // Window size and slot groups: everything resident, or two groups of G
     // slots used alternately (one group when there is a single slot).
     int G, groups;
@@ -270,7 +270,7 @@ int score_streamed(issl_index *ix, const uint64_t *d_guides, uint64_t n, const i
         }
         ISSL_HIP(hipEventRecord(src.ready[grp], src.copy));
         ISSL_HIP(hipStreamWaitEvent(st, src.ready[grp], 0));
-        for (int s = s0; batched && s < s1; ++s) {
+        for (int s = s0; s < s1; ++s) {
             const int j = grp * G + (s - s0);
             if (!src.slot_inline[static_cast<size_t>(j)] &&
                 (rc = build_slot_inline(ix, src, j, st)) != ISSL_OK)
