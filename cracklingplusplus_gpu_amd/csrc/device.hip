@@ -199,11 +199,13 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
     __shared__ uint32_t s_hocc[CH];
     __shared__ uint32_t s_accept;
     __shared__ int s_exited;
+    __shared__ double s_cfd[kCfdTable];
 
     const uint64_t g = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    for (int i = tid; i < kCfdTable; i += BLOCK) s_cfd[i] = a.cfd64[i];
     // Signatures are 40-bit (20 positions); higher bits are ignored.
     const uint64_t q = a.guides[g] & kSigBits;
     const int S = a.n_slices;
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(BLOCK) void k_plain(PlainArgs a) {
             if (occ == kOccEscape) occ = static_cast<uint32_t>(a.post[addr[k]] >> 32);
             double tm = 0.0, tc = 0.0;
             if (a.sc.calc_mit && dist > 0) tm = a.mit64[mit_rank(fold[k], dist)] * static_cast<double>(occ);
-            if (a.sc.calc_cfd) tc = cfd_site64(q, site[k], fold[k], dist, a.cfd64) * static_cast<double>(occ);
+            if (a.sc.calc_cfd) tc = cfd_site64(q, site[k], fold[k], dist, s_cfd) * static_cast<double>(occ);
             s_hmit[off[k]] = tm;
             s_hcfd[off[k]] = tc;
             s_hocc[off[k]] = occ;
