@@ -2,8 +2,8 @@
 // tiles of each guide's slice-major candidate sequence.
 //
 // Pipeline (all on one stream, no memsets):
-//   k_plan     one wave per guide, one lane per slice: slice keys -> list
-//              ranges, their prefix, the posting offsets, the tile length
+//   k_plan     one lane per guide: slice keys -> list ranges, their
+//              prefix, the posting offsets, the tile length
 //              C_g = max(chunk, ceil(P_g/64)) and the tile count K_g
 //   k_tiles    persistent waves.  Tile (g, k) is virtual slot v = k*n + g;
 //              wave w takes slots w, w+W, w+2W, ... and keeps those with
@@ -14,7 +14,8 @@
 //              slice-order first-seen rule and writes one partial per tile,
 //              reduced in a fixed lane order.  The next tile's plan is loaded
 //              while the current tile streams.
-//   k_finalize one wave per guide: adds the tile partials in tile order and,
+//   k_finalize adds the tile partials in tile order (one lane per guide when
+//              no early exit is armed, else one wave per guide) and,
 //              when the early exit is armed, rescans the tile in which the
 //              running total first meets the exit test, hit by hit, to stop
 //              exactly where the reference's loop stops
@@ -297,43 +298,72 @@ struct PlanArgs {
     BatchWs ws;
 };
 
-// One wave per guide, one lane per slice: the slice keys, list ranges and
-// their prefix are formed in parallel (two dependent loads per guide).
-__global__ __launch_bounds__(256) void k_plan(PlanArgs a) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t g = blockIdx.x * 4ULL + (threadIdx.x >> 6);
-    if (g >= a.n) return;  // uniform per wave
-    const uint64_t q = a.guides[g] & kSigBits;
+// One lane per guide.  The slice descriptors are staged in LDS; per group of
+// up to 16 slices every list's bounds are requested, then the prefix is formed
+// in registers and stored: no load waits behind another load of its group,
+// and no lane branches between them (lists outside the window, or of a guide
+// that took the exit, are loaded and then counted as empty).
+constexpr int kPlanBlock = 64;
+constexpr int kPlanGroup = 16;
+
+__global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
+    __shared__ uint64_t s_fold[kMaxSlices];
+    __shared__ uint32_t s_lb[kMaxSlices];
+    const uint64_t g = blockIdx.x * static_cast<uint64_t>(kPlanBlock) + threadIdx.x;
+    const bool in = g < a.n;
+    const uint64_t q = a.guides[in ? g : 0] & kSigBits;   // n >= 1 when launched
+    const int S = a.S;
+    for (int i = threadIdx.x; i < S; i += kPlanBlock) {
+        s_fold[i] = a.meta[i].fold;
+        s_lb[i] = a.meta[i].lbase;
+    }
+    __syncthreads();
+    if (!in) return;
     const bool live = !a.state_in || (a.state_in[g].sites & kExited) == 0;
-    uint32_t b = 0, len = 0;
-    if (lane < a.S && lane >= a.s0 && lane < a.s1 && live) {
-        const uint32_t lb = a.meta[lane].lbase;
-        const uint32_t key = slice_key(q, a.meta[lane].fold);
-        b = a.lstart[lb + key];
-        len = a.lstart[lb + key + 1] - b;
-    }
-    uint64_t incl = len;
+    uint32_t *pre = a.ws.pre + g * (S + 1);
+    uint64_t *delta = a.ws.delta + g * S;
+    uint64_t run = 0;
+    for (int s0 = 0; s0 < S; s0 += kPlanGroup) {
+        uint64_t fold[kPlanGroup];
+        uint32_t lb[kPlanGroup], b[kPlanGroup], e[kPlanGroup];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint64_t v = shfl64(incl, lane >= off ? lane - off : lane);
-        if (lane >= off) incl += v;
+        for (int i = 0; i < kPlanGroup; ++i) {
+            fold[i] = 0;
+            lb[i] = 0;
+            if (s0 + i < S) {
+                fold[i] = s_fold[s0 + i];
+                lb[i] = s_lb[s0 + i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kPlanGroup; ++i) {
+            b[i] = 0;
+            e[i] = 0;
+            if (s0 + i < S) {
+                const uint32_t key = slice_key(q, fold[i]);
+                b[i] = a.lstart[lb[i] + key];
+                e[i] = a.lstart[lb[i] + key + 1];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kPlanGroup; ++i) {
+            const int s = s0 + i;
+            if (s < S) {
+                const bool ok = live && s >= a.s0 && s < a.s1;
+                const uint32_t len = ok ? e[i] - b[i] : 0;
+                pre[s] = static_cast<uint32_t>(run);
+                // posting index of position p in slice s: (s - s0)*N + b + (p - pre[s])
+                delta[s] = len ? static_cast<uint64_t>(s - a.s0) * a.n_sites + b[i] - run : 0;
+                run += len;
+            }
+        }
     }
-    const uint64_t excl = incl - len;
-    const uint64_t run = a.S > 0 ? shfl64(incl, a.S - 1) : 0;
-    if (lane < a.S) {
-        a.ws.pre[g * (a.S + 1) + lane] = static_cast<uint32_t>(excl);
-        // posting index of position p in slice s: (s - s0)*N + b + (p - pre[s])
-        a.ws.delta[g * a.S + lane] =
-            len ? static_cast<uint64_t>(lane - a.s0) * a.n_sites + b - excl : 0;
-    }
-    if (lane == 0) {
-        a.ws.pre[g * (a.S + 1) + a.S] = static_cast<uint32_t>(run);
-        uint64_t c = (run + kMaxTiles - 1) / kMaxTiles;
-        c = (c + 63) / 64 * 64;
-        if (c < a.chunk) c = a.chunk;
-        a.ws.gchunk[g] = static_cast<uint32_t>(c);
-        a.ws.gtiles[g] = static_cast<uint32_t>((run + c - 1) / c);
-    }
+    pre[S] = static_cast<uint32_t>(run);
+    uint64_t c = (run + kMaxTiles - 1) / kMaxTiles;
+    c = (c + 63) / 64 * 64;
+    if (c < a.chunk) c = a.chunk;
+    a.ws.gchunk[g] = static_cast<uint32_t>(c);
+    a.ws.gtiles[g] = static_cast<uint32_t>((run + c - 1) / c);
 }
 
 // Final per-guide results: transform 10000/(100+S), the call and the output
@@ -895,6 +925,46 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     write_guide(a.sc, a.out, g, dm, dc, occ_sum, sites);
 }
 
+// k_finalize without an early exit armed: no rescan can happen, so one lane
+// per guide adds the tile partials in tile order, the same additions the
+// wave form makes.
+__global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
+    const uint64_t g = blockIdx.x * 64ULL + threadIdx.x;
+    if (g >= a.n) return;
+    const uint32_t K = a.ws.gtiles[g];
+    if (K == 0 && a.skip_untiled) return;
+    if (K == 1 && a.finish_single) return;  // finished by the wave that scored its tile
+    double dm = 0, dc = 0;
+    uint64_t occ_sum = 0, sites = 0;
+    bool exited = false;
+    if (a.state_in) {
+        const GuideState st = a.state_in[g];
+        dm = st.m;
+        dc = st.c;
+        occ_sum = st.occ;
+        sites = st.sites & ~kExited;
+        exited = (st.sites & kExited) != 0;
+    }
+#pragma unroll 4
+    for (uint32_t k = 0; k < K; ++k) {
+        const TilePart p = a.ws.part[static_cast<uint64_t>(k) * a.n + g];
+        dm += p.m;
+        dc += p.c;
+        occ_sum += p.occ;
+        sites += p.sites;
+    }
+    if (a.state_out) {
+        GuideState st;
+        st.m = dm;
+        st.c = dc;
+        st.occ = occ_sum;
+        st.sites = sites | (exited ? kExited : 0);
+        a.state_out[g] = st;
+        return;
+    }
+    write_guide(a.sc, a.out, g, dm, dc, occ_sum, sites);
+}
+
 // ------------------------------------------------------------ direct
 
 struct StreamArgs {
@@ -1292,7 +1362,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     } else {
         PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk),
                     win.s0, win.s1, win.state_in, b};
-        hipLaunchKernelGGL(k_plan, dim3(gw), dim3(256), 0, st, pa);
+        hipLaunchKernelGGL(k_plan, dim3(static_cast<uint32_t>((n + kPlanBlock - 1) / kPlanBlock)),
+                           dim3(kPlanBlock), 0, st, pa);
     }
 
     TileArgs ta;
@@ -1339,7 +1410,9 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     fa.state_in = win.state_in;
     fa.state_out = win.state_out;
     fa.ws = b;
-    if (fp32)
+    if (!sc.use_cut)
+        hipLaunchKernelGGL(k_finalize_sum, dim3(static_cast<uint32_t>((n + 63) / 64)), dim3(64), 0, st, fa);
+    else if (fp32)
         hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP32C>), dim3(gw), dim3(256), 0, st, fa);
     else
         hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP64>), dim3(gw), dim3(256), 0, st, fa);

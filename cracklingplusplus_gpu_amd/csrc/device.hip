@@ -1010,14 +1010,25 @@ extern "C" int issl_distinct_reset(issl_index *ix) {
 namespace issl {
 
 int ws_acquire(issl_index *ix, hipStream_t st) {
-    if (ix->ws_stream && ix->ws_stream != st && ix->ws_done) ISSL_HIP(hipStreamWaitEvent(st, ix->ws_done, 0));
+    if (!ix->ws_used || ix->ws_stream == st) return ISSL_OK;
+    if (ix->ws_lazy) {
+        // Everything enqueued on the index's stream so far, the last use of
+        // the workspace included.
+        if (!ix->ws_done) ISSL_HIP(hipEventCreateWithFlags(&ix->ws_done, hipEventDisableTiming));
+        ISSL_HIP(hipEventRecord(ix->ws_done, ix->ws_stream));
+        ix->ws_lazy = false;
+    }
+    ISSL_HIP(hipStreamWaitEvent(st, ix->ws_done, 0));
     return ISSL_OK;
 }
 
 int ws_release(issl_index *ix, hipStream_t st) {
+    ix->ws_used = true;
+    ix->ws_stream = st;
+    ix->ws_lazy = st == ix->stream;
+    if (ix->ws_lazy) return ISSL_OK;
     if (!ix->ws_done) ISSL_HIP(hipEventCreateWithFlags(&ix->ws_done, hipEventDisableTiming));
     ISSL_HIP(hipEventRecord(ix->ws_done, st));
-    ix->ws_stream = st;
     return ISSL_OK;
 }
 

@@ -174,6 +174,11 @@ struct issl_index {
     std::vector<issl::GraphEntry> graphs;   // captured sequences (ISSL_FLAG_GRAPH)
     // The scoring workspace is shared by every call on the index: a launch on
     // another stream than the previous one first waits for that one's work.
+    // After a call on the index's own stream the event is recorded only when
+    // a call on another stream comes (that stream outlives the index's calls;
+    // a caller's stream is recorded at once, as it may be destroyed).
+    bool ws_used = false;
+    bool ws_lazy = false;
     hipStream_t ws_stream = nullptr;
     hipEvent_t ws_done = nullptr;
     // Streamed index (issl_index_open_streamed and issl_index_from_arrays_streamed):

@@ -75,6 +75,22 @@ class DeviceArray:
             pass
 
 
+class Stream:
+    """A HIP stream at the default priority (hipStreamCreate)."""
+
+    def __init__(self):
+        self.handle = c_void_p()
+        _ok(hip().hipStreamCreate(byref(self.handle)), "hipStreamCreate")
+
+    def synchronize(self) -> None:
+        _ok(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def destroy(self) -> None:
+        if self.handle is not None and self.handle.value:
+            _ok(hip().hipStreamDestroy(self.handle), "hipStreamDestroy")
+        self.handle = None
+
+
 def copy_device(dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
     """Device-to-device copy of nbytes (the runtime's own copy kernel)."""
     _ok(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
