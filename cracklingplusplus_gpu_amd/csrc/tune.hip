@@ -114,22 +114,28 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     hipEvent_t e0, e1;
     ISSL_HIP(hipEventCreate(&e0));
     ISSL_HIP(hipEventCreate(&e1));
-    // Time one launch shape: a warm-up, then the best of three launches.
+    // Time one launch shape: two warm-ups, then the best of five trials of
+    // two back-to-back launches (a single short launch between two events
+    // varies by several percent from trial to trial).
+    constexpr int kTrials = 5, kPerTrial = 2;
     auto measure = [&](issl_tuning &t) -> int {
-        int r0 = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st);
-        if (r0 != ISSL_OK) return r0;
+        int r0;
+        for (int r = 0; r < 2; ++r)
+            if ((r0 = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st)) != ISSL_OK)
+                return r0;
         float bestms = 1e30f;
-        for (int r = 0; r < 3; ++r) {
+        for (int r = 0; r < kTrials; ++r) {
             ISSL_HIP(hipEventRecord(e0, st));
-            r0 = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st);
-            if (r0 != ISSL_OK) return r0;
+            for (int k = 0; k < kPerTrial; ++k)
+                if ((r0 = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st)) != ISSL_OK)
+                    return r0;
             ISSL_HIP(hipEventRecord(e1, st));
             ISSL_HIP(hipEventSynchronize(e1));
             float ms = 0;
             ISSL_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (ms < bestms) bestms = ms;
         }
-        t.us_per_guide = 1000.0 * bestms / static_cast<double>(n);
+        t.us_per_guide = 1000.0 * bestms / (static_cast<double>(n) * kPerTrial);
         return ISSL_OK;
     };
     issl_tuning top{};
