@@ -472,6 +472,18 @@ def main() -> int:
         lat.append((time.perf_counter() - t0) * 1e3)
     lat = np.array(lat)
 
+    # ---- the whole step through host buffers (issl_score): guides uploaded,
+    # results copied back, both ways over PCIe; reported beside `value`
+    hres = ix.results(Q)
+    for _ in range(max(1, args.warmup)):
+        ix.score(guides, out=hres, **kw)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ix.score(guides, out=hres, **kw)
+    host_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    host_same = bool(np.array_equal(hres.cfd.view(np.uint64), res_batched["cfd"].view(np.uint64)))
+    log(f"host buffers: {host_ms:.3f} ms/call, {Q / host_ms * 1e3:,.0f} guides/s")
+
     # ---- index construction on the device (sites and occurrences uploaded,
     # lists sorted in HBM) against the host builder above
     t_b = time.perf_counter()
@@ -573,6 +585,11 @@ def main() -> int:
         "precision": {"fp32c_vs_fp64_max_rel": {f: float(rel[f].max()) for f in rel},
                       "fp32c_vs_fp64_mean_rel": {f: float(rel[f].mean()) for f in rel},
                       "counts_identical": same_counts},
+        "host_buffers": {"value": round(Q / host_ms * 1e3, 1), "unit": "guides/s",
+                         "ms_per_call": round(host_ms, 4), "calls": args.steps,
+                         "same_as_device_buffers": host_same,
+                         "mode": "issl_score: guides from host memory, the seven result "
+                                 "columns to host memory (PCIe both ways)"},
         "latency_ms": {"request_guides": 1000, "requests": int(lat.size),
                        "p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),

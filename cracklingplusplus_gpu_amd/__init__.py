@@ -319,13 +319,25 @@ class Index:
                chunk: int = 0, max_dist: int = 4) -> _capi.issl_params:
         return _capi.issl_params(max_dist, method, threshold, form, precision, flags, chunk)
 
+    @staticmethod
+    def results(n: int) -> Results:
+        """Host result buffers for n guides (score's `out`)."""
+        return Results(np.zeros(n), np.zeros(n), np.zeros(n, np.uint64), np.zeros(n, np.uint64),
+                       np.zeros(n, np.uint8), np.zeros(n), np.zeros(n))
+
     def score(self, guides, method: int = METHOD_MIT_AND_CFD, threshold: float = 0.0,
               form: int = FORM_PLAIN, precision: int = PREC_FP64, flags: int = 0,
-              chunk: int = 0, max_dist: int = 4) -> Results:
+              chunk: int = 0, max_dist: int = 4, out: Optional[Results] = None) -> Results:
+        """Blocking scoring through host buffers (issl_score).  `out`: buffers
+        from results(n) to fill instead of new ones."""
         g = np.ascontiguousarray(guides, dtype=np.uint64)
         n = g.shape[0]
-        r = Results(np.zeros(n), np.zeros(n), np.zeros(n, np.uint64), np.zeros(n, np.uint64),
-                    np.zeros(n, np.uint8), np.zeros(n), np.zeros(n))
+        r = out if out is not None else self.results(n)
+        for a, dt in ((r.mit, np.float64), (r.cfd, np.float64), (r.occ_count, np.uint64),
+                      (r.site_count, np.uint64), (r.passed, np.uint8), (r.mit_sum, np.float64),
+                      (r.cfd_sum, np.float64)):
+            if a.shape != (n,) or a.dtype != dt or not a.flags.c_contiguous:
+                raise ValueError("out buffers must be results(n) for this call's n guides")
         res = _capi.issl_results(_ptr(r.mit), _ptr(r.cfd), _ptr(r.occ_count),
                                  _ptr(r.site_count), _ptr(r.passed), _ptr(r.mit_sum),
                                  _ptr(r.cfd_sum))
