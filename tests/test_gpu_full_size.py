@@ -1,0 +1,125 @@
+"""Parity at the benchmark's full size: the 100 M-site, 10-slice index and its
+10,000 guides (genome100m_10x10), on an MI355X.
+
+At this size the oracle scores a sample of guides (its per-guide cost is a
+12.5 MB toggle clear); the rest is checked through properties that do not
+depend on the size:
+
+* PLAIN (FP64) equals the oracle bit for bit on a sample that holds every
+  kind of guide (site, repeat neighbourhood, random), with the threshold off
+  and on;
+* BATCHED FP32C is within 1e-6 relative of PLAIN for all 10,000 guides, with
+  identical site and occurrence counts, and BATCHED FP64 within 1e-12;
+* with the threshold on, BATCHED makes the same calls as PLAIN, except for
+  guides within 1e-9 of the cutoff;
+* a guide's BATCHED result is the same bits alone, in a subset and in the
+  full batch;
+* the site counts equal a brute force over all 100 M sites (the oracle's
+  candidate rule) for sampled guides.
+"""
+import numpy as np
+import pytest
+
+import cracklingplusplus_gpu_amd as issl
+from cracklingplusplus_gpu_amd import workloads
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+FP32C = dict(method=issl.METHOD_MIT_AND_CFD, form=issl.FORM_BATCHED, precision=issl.PREC_FP32C)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.fixture(scope="module")
+def c1(gpu_available):
+    print("[gpu-full] building genome100m_10x10", flush=True)
+    sigs, occ, masks, guides, kinds = workloads.make("genome100m_10x10", threads=16)
+    hx = issl.build_lists(sigs, occ, masks, threads=16)
+    del sigs, occ
+    print("[gpu-full] uploading", flush=True)
+    ix = issl.Index.from_arrays(hx)
+    yield hx, ix, guides, kinds
+    ix.close()
+
+
+def sample(guides, kinds, per_kind, seed=11):
+    rng = np.random.default_rng(seed)
+    pick = []
+    for k in np.unique(kinds):
+        idx = np.nonzero(kinds == k)[0]
+        pick.extend(rng.choice(idx, size=min(per_kind, idx.size), replace=False).tolist())
+    return np.sort(np.array(pick))
+
+
+def test_full_size_plain_bit_exact_vs_oracle(c1):
+    hx, ix, guides, kinds = c1
+    idx = sample(guides, kinds, 96)
+    g = guides[idx]
+    for t in (0.0, 75.0):
+        print(f"[gpu-full] plain vs oracle, {g.size} guides, threshold {t}", flush=True)
+        o = orc.score(hx, g, issl.METHOD_MIT_AND_CFD, t)
+        r = ix.score(g, method=issl.METHOD_MIT_AND_CFD, threshold=t, form=issl.FORM_PLAIN)
+        assert (bits(r.mit) == bits(o["mit"])).all()
+        assert (bits(r.cfd) == bits(o["cfd"])).all()
+        assert (bits(r.mit_sum) == bits(o["mit_sum"])).all()
+        assert (bits(r.cfd_sum) == bits(o["cfd_sum"])).all()
+        assert (r.occ_count == o["occ_count"]).all()
+        assert (r.site_count == o["site_count"]).all()
+        assert (r.passed == o["pass"]).all()
+
+
+def test_full_size_batched_vs_plain(c1):
+    hx, ix, guides, kinds = c1
+    print("[gpu-full] batched vs plain, all guides", flush=True)
+    p = ix.score(guides, method=issl.METHOD_MIT_AND_CFD, threshold=0.0, form=issl.FORM_PLAIN)
+    assert (p.site_count > 0).mean() > 0.5          # the index is not trivially empty
+    b32 = ix.score(guides, threshold=0.0, **FP32C)
+    b64 = ix.score(guides, method=issl.METHOD_MIT_AND_CFD, threshold=0.0,
+                   form=issl.FORM_BATCHED, precision=issl.PREC_FP64)
+    for r, rtol in ((b32, 1e-6), (b64, 1e-12)):
+        assert (r.site_count == p.site_count).all()
+        assert (r.occ_count == p.occ_count).all()
+        for f in ("mit_sum", "cfd_sum", "mit", "cfd"):
+            a, w = getattr(r, f), getattr(p, f)
+            np.testing.assert_allclose(a, w, rtol=rtol, atol=0, err_msg=f)
+
+
+def test_full_size_threshold_calls(c1):
+    hx, ix, guides, kinds = c1
+    t = 75.0
+    cut = (10000.0 - t * 100) / t
+    print("[gpu-full] threshold calls", flush=True)
+    p = ix.score(guides, method=issl.METHOD_MIT_AND_CFD, threshold=t, form=issl.FORM_PLAIN)
+    b = ix.score(guides, threshold=t, **FP32C)
+    # the unthresholded FP64 sums tell which guides sit within 1e-9 of the cutoff
+    full = ix.score(guides, method=issl.METHOD_MIT_AND_CFD, threshold=0.0, form=issl.FORM_PLAIN)
+    near = (np.abs(full.mit_sum - cut) <= 1e-9 * cut) | (np.abs(full.cfd_sum - cut) <= 1e-9 * cut)
+    assert ((b.passed == p.passed) | near).all()
+    assert (p.passed == ord("0")).any() and (p.passed == ord("1")).any()
+
+
+def test_full_size_batch_composition(c1):
+    hx, ix, guides, kinds = c1
+    print("[gpu-full] batch composition", flush=True)
+    whole = ix.score(guides, threshold=0.0, **FP32C)
+    idx = sample(guides, kinds, 300, seed=5)
+    part = ix.score(guides[idx], threshold=0.0, **FP32C)
+    for f in ("mit", "cfd", "mit_sum", "cfd_sum"):
+        assert (bits(getattr(part, f)) == bits(getattr(whole, f)[idx])).all(), f
+    for i in idx[:8]:
+        one = ix.score(guides[i:i + 1], threshold=0.0, **FP32C)
+        assert bits(one.cfd)[0] == bits(whole.cfd)[i]
+        assert bits(one.mit)[0] == bits(whole.mit)[i]
+
+
+def test_full_size_site_counts_vs_brute_force(c1):
+    hx, ix, guides, kinds = c1
+    idx = sample(guides, kinds, 2, seed=3)
+    r = ix.score(guides[idx], threshold=0.0, **FP32C)
+    for j, i in enumerate(idx):
+        print(f"[gpu-full] brute force over 100 M sites, guide {i}", flush=True)
+        ids = orc.candidates_brute(hx, int(guides[i]))
+        assert r.site_count[j] == ids.size
