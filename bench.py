@@ -22,6 +22,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -322,7 +323,7 @@ def main() -> int:
                     help="guides timed on the host CPU scorer")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--latency-requests", type=int, default=200)
-    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "gpurun_out", "issl_tune.txt"))
+    ap.add_argument("--tune-cache", default=os.path.join(tempfile.gettempdir(), "issl_tune.txt"))
     ap.add_argument("--serve-rate", type=float, default=500.0,
                     help="Poisson arrival rate of the 1,000-guide serving leg (req/s)")
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -428,7 +429,7 @@ def main() -> int:
     # (guide, slice); the reference's layout reads an 8 B posting entry and
     # gathers an 8 B signature per posting, reported beside it.
     bytes_inline = 8 * p_total + 16 * S * Q
-    bytes_survey = 16 * p_total + 16 * S * Q
+    bytes_ref_layout = 16 * p_total + 16 * S * Q
     achieved = bytes_inline / (k_avg_ms * 1e-3) / 1e9
     log(f"k_tiles avg {k_avg_ms * 1e3:.1f} us/launch -> {achieved:,.0f} GB/s")
 
@@ -573,14 +574,14 @@ def main() -> int:
                      "traffic": measured_traffic(args.workload), "kernel": "k_tiles",
                      "kernel_us": round(k_avg_ms * 1e3, 2),
                      "bytes_per_launch": bytes_inline,
-                     "reference_layout_bytes_per_launch": bytes_survey,
-                     "reference_layout_GBps": round(bytes_survey / (k_avg_ms * 1e-3) / 1e9, 1)},
+                     "reference_layout_bytes_per_launch": bytes_ref_layout,
+                     "reference_layout_GBps": round(bytes_ref_layout / (k_avg_ms * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
         "plain": {"value": round(Q / (plain_ms * 1e-3), 1), "unit": "guides/s",
                   "ms_per_step": round(plain_ms, 4), "kernel": "k_plain",
                   "kernel_us": round(plain_k_ms * 1e3, 2),
                   "achieved_GBps": round(bytes_inline / (plain_k_ms * 1e-3) / 1e9, 1),
-                  "reference_layout_GBps": round(bytes_survey / (plain_k_ms * 1e-3) / 1e9, 1),
+                  "reference_layout_GBps": round(bytes_ref_layout / (plain_k_ms * 1e-3) / 1e9, 1),
                   "batched_speedup": round(plain_ms / ms_per_step, 3)},
         "precision": {"fp32c_vs_fp64_max_rel": {f: float(rel[f].max()) for f in rel},
                       "fp32c_vs_fp64_mean_rel": {f: float(rel[f].mean()) for f in rel},

@@ -1,6 +1,6 @@
 // ISSLScoringModuleMMF.hpp -- drop-in replacement of the reference's
 // memory-mapped ISSL stage (include/ISSLScoringModuleMMF.hpp:16-30), chosen
-// by the driver when config.offTarget.memoryMapped is set
+// by the Crackling pipeline when config.offTarget.memoryMapped is set
 // (src/CracklingPlusPlus.cpp:73-80).  The .issl file is memory-mapped and
 // its slices stream through HBM slice buffers (issl_index_open_streamed),
 // so the index need not fit in device memory; results are the resident

@@ -1,5 +1,5 @@
 // module_demo.cpp -- drives the drop-in ISSLScoringModule (or, with
-// memoryMapped set, ISSLScoringModuleMMF) the way the Crackling driver does
+// memoryMapped set, ISSLScoringModuleMMF) the way the Crackling pipeline does
 // (src/CracklingPlusPlus.cpp:38-39,73-80): build a cracklingConfig, fill
 // guideResults, call run(), read the result fields.
 //
@@ -56,7 +56,7 @@ int main(int argc, char **argv) {
         guides.push_back(g);
     }
     try {
-        // As the driver does: both modules exist, the flag picks one; a
+        // As the Crackling pipeline does: both modules exist, the flag picks one; a
         // second batch reuses the index opened by the first.
         ISSLScoringModule ISSLScoring(cfg);
         ISSLScoringModuleMMF ISSLScoringMMF(cfg);

@@ -188,7 +188,7 @@ def test_module_built_on_reference_types_matches(tmp_path, method, level, mmf, g
 @pytest.mark.parametrize("slots", [1, 2, 0])
 @pytest.mark.parametrize("method,level", [("mitAndCfd", "medium"), ("cfd", "ultralow")])
 def test_module_mmf_matches_resident(tmp_path, method, level, slots, gpu_available):
-    """The memory-mapped module (the driver's choice when
+    """The memory-mapped module (the Crackling pipeline's choice when
     offTarget.memoryMapped is set) streams the index through HBM slice
     buffers and gives the resident module's results bit for bit."""
     hx = golden_index("tiny4k_5x4")
