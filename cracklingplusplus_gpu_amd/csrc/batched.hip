@@ -4,10 +4,14 @@
 // Pipeline (all on one stream, no memsets):
 //   k_plan     one lane per guide: slice keys -> list ranges, their
 //              prefix, the posting offsets, the tile length
-//              C_g = max(chunk, ceil(P_g/64)) and the tile count K_g
-//   k_tiles    persistent waves.  Tile (g, k) is virtual slot v = k*n + g;
-//              wave w takes slots w, w+W, w+2W, ... and keeps those with
-//              k < K_g, checking 64 slots per load.  For each tile it streams
+//              C_g = max(chunk, ceil(P_g/64)) and the tile count K_g; the
+//              guide's tiles (slot v = k*n + g) are appended to a device
+//              tile queue, tiles of several-tile guides ahead of single ones
+//   k_tiles    persistent waves that take tiles from the queue one at a time
+//              (from one of up to 256 sub-queues, so that no counter takes
+//              many atomics), so waves finish together whatever a tile
+//              costs (candidate-dense tiles of repeat families cost several
+//              times a plain one).  For each tile it streams
 //              the site words (sig | occ<<40: no signature gather), folds and
 //              filters them (dist <= 4), queues the candidates with their
 //              words in an LDS ring, scores them 64 at a time under the
@@ -61,11 +65,38 @@ struct TilePart {
 // (kMaxTiles * n * 32 B) stay modest.
 constexpr uint64_t kMaxBatch = 1ULL << 18;
 
+// Tile queue: the planner appends to kRegions regions with a counter each,
+// the tile kernel takes from up to kSubQueues sub-queues, each counter on its
+// own 64 bytes (atomics on one address serialize at the memory side, ~12 ns
+// apiece).
+constexpr uint32_t kRegions = 32;
+constexpr uint32_t kSubQueues = 256;
+constexpr uint32_t kCtrStride = 16;
+constexpr uint32_t kSubQueueBase = 2 * kRegions;                // taken counters
+constexpr uint32_t kSubQueueCtr = kCtrStride * kSubQueues;      // taken -> finished
+constexpr int kQueueCtrs = static_cast<int>(kSubQueueBase + 2 * kSubQueueCtr);
+// Planner blocks (64 guides each) per region; a region of the first list
+// holds their tiles (at most kMaxTiles per guide), of the second one tile
+// per guide.
+__host__ __device__ inline uint64_t region_blocks(uint64_t n) {
+    return ((n + 63) / 64 + kRegions - 1) / kRegions;
+}
+
 struct BatchWs {
     uint32_t *pre;       // [n][S+1] prefix of the list lengths; [S] = P_g
     uint64_t *delta;     // [n][S]   posting index = position + delta
     uint32_t *gchunk;    // [n] tile length C_g
     uint32_t *gtiles;    // [n] tile count K_g
+    // Tile queue (null: the slot schedule).  Planner block b appends the slots
+    // of its guides with several tiles to region b % kRegions of the first
+    // list and of its single-tile guides to the same region of the second,
+    // counting them in tctr[r] and tctr[kRegions + r] (set back to zero by
+    // k_finalize).  The queue is the first list's regions in order, then the
+    // second's.  The tile kernel splits it into sub-queues, each with a taken
+    // and a finished counter (kSubQueueCtr apart), which its last wave sets
+    // back to zero.
+    uint32_t *tlist;
+    uint32_t *tctr;
     TilePart *part;      // [kMaxTiles][n] partial of tile (g, k) at slot k*n + g
 };
 
@@ -306,6 +337,9 @@ struct PlanArgs {
 constexpr int kPlanBlock = 64;
 constexpr int kPlanGroup = 16;
 
+__device__ uint32_t plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const uint64_t *s_fold,
+                               const uint32_t *s_lb);
+
 __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
     __shared__ uint64_t s_fold[kMaxSlices];
     __shared__ uint32_t s_lb[kMaxSlices];
@@ -318,7 +352,49 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         s_lb[i] = a.meta[i].lbase;
     }
     __syncthreads();
-    if (!in) return;
+    uint32_t K = 0;
+    if (in) K = plan_guide(a, g, q, s_fold, s_lb);
+    if (a.ws.tlist) {
+        // Append the wave's tiles to the queue: one atomic per list per wave,
+        // on the counter of the block's region.
+        const int lane = static_cast<int>(threadIdx.x);
+        const uint32_t ka = K > 1 ? K : 0, kb = K == 1 ? 1 : 0;
+        uint32_t ia = ka, ib = kb;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t va = static_cast<uint32_t>(__shfl(static_cast<int>(ia), lane >= off ? lane - off : lane));
+            const uint32_t vb = static_cast<uint32_t>(__shfl(static_cast<int>(ib), lane >= off ? lane - off : lane));
+            if (lane >= off) {
+                ia += va;
+                ib += vb;
+            }
+        }
+        const uint32_t ta = static_cast<uint32_t>(__shfl(static_cast<int>(ia), 63));
+        const uint32_t tb = static_cast<uint32_t>(__shfl(static_cast<int>(ib), 63));
+        const uint32_t r = blockIdx.x % kRegions;
+        uint32_t base_a = 0, base_b = 0;
+        if (lane == 0) {
+            if (ta) base_a = atomicAdd(a.ws.tctr + r, ta);
+            if (tb) base_b = atomicAdd(a.ws.tctr + kRegions + r, tb);
+        }
+        base_a = static_cast<uint32_t>(__shfl(static_cast<int>(base_a), 0)) + ia - ka;
+        base_b = static_cast<uint32_t>(__shfl(static_cast<int>(base_b), 0)) + ib - kb;
+        const uint64_t rb = region_blocks(a.n) * 64;
+        uint32_t *const la = a.ws.tlist + r * rb * kMaxTiles;
+        uint32_t *const lb = a.ws.tlist + kRegions * rb * kMaxTiles + r * rb;
+        const uint32_t n32 = static_cast<uint32_t>(a.n);
+        // A region holds all its blocks' tiles; the bound only guards the
+        // list against counters that were not zero.
+        for (uint32_t k = 0; k < ka; ++k)
+            if (base_a + k < rb * kMaxTiles) la[base_a + k] = k * n32 + static_cast<uint32_t>(g);
+        if (kb && base_b < rb) lb[base_b] = static_cast<uint32_t>(g);
+    }
+}
+
+// The plan of guide g (k_plan); returns its tile count.
+__device__ uint32_t plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const uint64_t *s_fold,
+                               const uint32_t *s_lb) {
+    const int S = a.S;
     const bool live = !a.state_in || (a.state_in[g].sites & kExited) == 0;
     uint32_t *pre = a.ws.pre + g * (S + 1);
     uint64_t *delta = a.ws.delta + g * S;
@@ -362,8 +438,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
     uint64_t c = (run + kMaxTiles - 1) / kMaxTiles;
     c = (c + 63) / 64 * 64;
     if (c < a.chunk) c = a.chunk;
+    const uint32_t K = static_cast<uint32_t>((run + c - 1) / c);
     a.ws.gchunk[g] = static_cast<uint32_t>(c);
-    a.ws.gtiles[g] = static_cast<uint32_t>((run + c - 1) / c);
+    a.ws.gtiles[g] = K;
+    return K;
 }
 
 // Final per-guide results: transform 10000/(100+S), the call and the output
@@ -418,8 +496,9 @@ constexpr uint32_t kRing = 64 * (kRingChk + 1);
 // Hit ring per wave: fewer than 64 pending plus one check's 64.
 constexpr uint32_t kHitRing = 128;
 
-// Postings in flight per lane: DEPTH.
-template <int PREC, int DEPTH>
+// Postings in flight per lane: DEPTH.  QUEUE: tiles come from the planner's
+// queue (else from the slot schedule).
+template <int PREC, int DEPTH, bool QUEUE>
 __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
     __shared__ Real s_cfd[kCfdEntries];
@@ -454,11 +533,83 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     const uint32_t W = gridDim.x * kTileWaves;
     const uint32_t w = blockIdx.x * kTileWaves + wave;
 
-    // Slots of this wave: w + j*W.  64 of them are checked per load; the real
-    // ones (k < K_g) are taken in increasing order.
+    // Queue schedule: a wave takes the next tile of its sub-queue when it
+    // starts one, so that the waves finish together whatever each tile
+    // costs.  Sub-queue j (of Q) is served by the workgroups b with
+    // b % Q == j and holds queue entries j, j + Q, j + 2Q, ...: its share of
+    // the several-tile guides' tiles first, then of the single tiles.  Slot
+    // schedule (guide streams wrote the plans): the slots of this wave are
+    // w + j*W; 64 of them are checked per load and the real ones (k < K_g)
+    // are taken in increasing order.
+    //
+    // The queue regions' counts, one per lane, and their exclusive prefixes.
+    uint32_t pre_a = 0, pre_b = 0, tot_a = 0, tot_b = 0;
+    uint64_t rb = 0;
+    if constexpr (QUEUE) {
+        rb = region_blocks(n) * 64;
+        // (the bounds only guard the list against counters that were not zero)
+        const uint32_t ca = lane < static_cast<int>(kRegions)
+                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[lane], rb * kMaxTiles)) : 0;
+        const uint32_t cb = lane < static_cast<int>(kRegions)
+                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[kRegions + lane], rb)) : 0;
+        uint32_t ia = ca, ib = cb;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t va = static_cast<uint32_t>(__shfl(static_cast<int>(ia), lane >= off ? lane - off : lane));
+            const uint32_t vb = static_cast<uint32_t>(__shfl(static_cast<int>(ib), lane >= off ? lane - off : lane));
+            if (lane >= off) {
+                ia += va;
+                ib += vb;
+            }
+        }
+        pre_a = ia - ca;
+        pre_b = ib - cb;
+        tot_a = static_cast<uint32_t>(__shfl(static_cast<int>(ia), 63));
+        tot_b = static_cast<uint32_t>(__shfl(static_cast<int>(ib), 63));
+    }
+    // Queue entry c: the region holding it is the last one starting at or
+    // before c.
+    auto entry = [&](uint64_t c) -> uint32_t {
+        if (c < tot_a) {
+            const uint64_t m = __ballot(lane < static_cast<int>(kRegions) && pre_a <= c);
+            const int r = 63 - __builtin_clzll(m);
+            const uint32_t off = static_cast<uint32_t>(c) - static_cast<uint32_t>(__shfl(static_cast<int>(pre_a), r));
+            return a.ws.tlist[r * rb * kMaxTiles + off];
+        }
+        c -= tot_a;
+        if (c < tot_b) {
+            const uint64_t m = __ballot(lane < static_cast<int>(kRegions) && pre_b <= c);
+            const int r = 63 - __builtin_clzll(m);
+            const uint32_t off = static_cast<uint32_t>(c) - static_cast<uint32_t>(__shfl(static_cast<int>(pre_b), r));
+            return a.ws.tlist[kRegions * rb * kMaxTiles + r * rb + off];
+        }
+        return kNoSlot;
+    };
+    const uint32_t nq = std::min<uint32_t>(kSubQueues, gridDim.x);
+    const uint32_t sq = blockIdx.x % nq;
+    const uint32_t sq_waves = (gridDim.x - sq + nq - 1) / nq * kTileWaves;
+    uint32_t *const sq_taken = QUEUE ? a.ws.tctr + kSubQueueBase + sq * kCtrStride : nullptr;
+    uint32_t *const sq_done = QUEUE ? sq_taken + kSubQueueCtr : nullptr;
+    // A wave leaving the kernel: the last one of its sub-queue's waves sets
+    // the sub-queue's counters back to zero for the next dispatch.
+    auto leave = [&]() {
+        if constexpr (QUEUE) {
+            if (lane == 0 && atomicAdd(sq_done, 1u) == sq_waves - 1) {
+                atomicExch(sq_taken, 0u);
+                atomicExch(sq_done, 0u);
+            }
+        }
+    };
     uint32_t j0 = 0, vl = 0;
     uint64_t smask = 0;
     auto next_slot = [&]() -> uint32_t {
+        if constexpr (QUEUE) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(sq_taken, 1u);
+            t = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(t)));
+            const uint32_t v = entry(sq + static_cast<uint64_t>(t) * nq);
+            return v < V ? v : kNoSlot;   // a slot outside the slot space ends the wave
+        }
         while (smask == 0) {
             if (static_cast<uint64_t>(j0) * W + w >= V) return kNoSlot;
             const uint64_t vv = static_cast<uint64_t>(j0 + lane) * W + w;
@@ -508,7 +659,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     };
 
     uint32_t vc = next_slot();
-    if (vc == kNoSlot) return;  // uniform per wave; no workgroup barrier follows
+    if (vc == kNoSlot) {  // uniform per wave; no workgroup barrier follows
+        leave();
+        return;
+    }
     Plan mc, mn;
     plan_load(vc, mc);
     plan_commit(mc, 0);
@@ -735,7 +889,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             // (what the finalizer would form: 0 + partial, exactly).
             if (mc.K == 1 && a.finish_single) write_guide(a.sc, a.out, mc.g, dm, dc, occ_sum, sites);
         }
-        if (vn == kNoSlot) break;
+        if (vn == kNoSlot) {
+            leave();
+            break;
+        }
         buf ^= 1;
         plan_commit(mn, buf);
         vc = vn;
@@ -766,6 +923,17 @@ struct FinArgs {
     BatchWs ws;
 };
 
+// The tile queue's counters back to zero for the next launch sequence (the
+// tile kernel has finished with them).
+__device__ __forceinline__ void reset_queue(const BatchWs &ws) {
+    if (ws.tctr && blockIdx.x == 0 && threadIdx.x < 2 * kRegions) ws.tctr[threadIdx.x] = 0;
+}
+
+__global__ void k_zero_queue(uint32_t *tctr) {
+    for (int i = static_cast<int>(threadIdx.x); i < kQueueCtrs; i += static_cast<int>(blockDim.x))
+        tctr[i] = 0;
+}
+
 template <int PREC>
 __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
@@ -775,6 +943,7 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     __shared__ uint64_t s_pre[W][kMaxSlices + 1];
     __shared__ uint64_t s_delta[W][kMaxSlices];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    reset_queue(a.ws);
     if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
     __syncthreads();
     build_posmask(s_posmask, s_fold, a.S, tid);
@@ -929,6 +1098,7 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
 // per guide adds the tile partials in tile order, the same additions the
 // wave form makes.
 __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
+    reset_queue(a.ws);
     const uint64_t g = blockIdx.x * 64ULL + threadIdx.x;
     if (g >= a.n) return;
     const uint32_t K = a.ws.gtiles[g];
@@ -1240,14 +1410,24 @@ namespace {
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-int carve(Workspace &ws, uint64_t n, int S, BatchWs *b) {
+// fresh: the buffer was (re)allocated, so the queue counters are not zero.
+int carve(Workspace &ws, uint64_t n, int S, BatchWs *b, bool *fresh) {
     const size_t sz_pre = align256(n * (S + 1) * 4);
     const size_t sz_delta = align256(n * std::max(S, 1) * 8);
     const size_t sz_u32 = align256(n * 4);
     const size_t sz_part = align256(n * kMaxTiles * sizeof(TilePart));
-    int rc = ws.tiles.reserve(sz_pre + sz_delta + 2 * sz_u32 + sz_part);
+    const size_t sz_list = align256(kRegions * region_blocks(n) * 64 * (kMaxTiles + 1) * 4);
+    const size_t sz_ctr = align256(kQueueCtrs * 4);
+    const void *old = ws.tiles.p;
+    int rc = ws.tiles.reserve(sz_ctr + sz_pre + sz_delta + 2 * sz_u32 + sz_part + sz_list);
     if (rc != ISSL_OK) return rc;
+    *fresh = ws.tiles.p != old;
     char *p = static_cast<char *>(ws.tiles.p);
+    // The queue counters come first, at the same place for every n: they are
+    // zeroed once and then kept at zero between launch sequences, while the
+    // arrays after them move with n.
+    b->tctr = reinterpret_cast<uint32_t *>(p);
+    p += sz_ctr;
     b->pre = reinterpret_cast<uint32_t *>(p);
     p += sz_pre;
     b->delta = reinterpret_cast<uint64_t *>(p);
@@ -1257,6 +1437,8 @@ int carve(Workspace &ws, uint64_t n, int S, BatchWs *b) {
     b->gtiles = reinterpret_cast<uint32_t *>(p);
     p += sz_u32;
     b->part = reinterpret_cast<TilePart *>(p);
+    p += sz_part;
+    b->tlist = reinterpret_cast<uint32_t *>(p);
     return ISSL_OK;
 }
 
@@ -1265,11 +1447,16 @@ template <int PREC>
 int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st, hipEvent_t e0,
                        hipEvent_t e1) {
     const dim3 g(grid), b(kTileBlock);
-    switch (depth) {
-    case 4: hipExtLaunchKernelGGL((k_tiles<PREC, 4>), g, b, 0, st, e0, e1, 0, ta); break;
-    case 8: hipExtLaunchKernelGGL((k_tiles<PREC, 8>), g, b, 0, st, e0, e1, 0, ta); break;
-    case 16: hipExtLaunchKernelGGL((k_tiles<PREC, 16>), g, b, 0, st, e0, e1, 0, ta); break;
-    default: return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
+    if (depth != 4 && depth != 8 && depth != 16)
+        return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
+    if (ta.ws.tlist) {
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true>), g, b, 0, st, e0, e1, 0, ta);
+    } else {
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, false>), g, b, 0, st, e0, e1, 0, ta);
     }
     return ISSL_OK;
 }
@@ -1316,8 +1503,13 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
                  Workspace &ws, hipStream_t st, const Window &win) {
     const int S = static_cast<int>(ix->S);
     BatchWs b;
-    int rc = carve(ws, n, S, &b);
+    bool fresh = false;
+    int rc = carve(ws, n, S, &b, &fresh);
     if (rc != ISSL_OK) return rc;
+    // The counters are zero unless the buffer is new or an earlier sequence
+    // stopped before its k_finalize was queued.
+    if (fresh || ws.queue_dirty) hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(256), 0, st, b.tctr);
+    ws.queue_dirty = true;
     const bool fp32 = precision == ISSL_PREC_FP32C;
     const void *mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
     const void *cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
@@ -1329,6 +1521,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     const uint32_t gw = static_cast<uint32_t>((n + 3) / 4);
 
     if (direct) {
+        b.tlist = nullptr;   // guide streams write plans without the queue: slot schedule
         StreamArgs da;
         da.guides = d_guides;
         da.n = n;
@@ -1357,6 +1550,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
         if (rc != ISSL_OK) return rc;
         if (!tiles) {
             ISSL_HIP(hipGetLastError());
+            ws.queue_dirty = false;   // no planner ran: the counters are untouched
             return ISSL_OK;
         }
     } else {
@@ -1417,6 +1611,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     else
         hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP64>), dim3(gw), dim3(256), 0, st, fa);
     ISSL_HIP(hipGetLastError());
+    ws.queue_dirty = false;   // k_finalize sets the counters back to zero
     return ISSL_OK;
 }
 

@@ -102,6 +102,9 @@ struct Workspace {
     DevBuf guides, out, tiles, scratch;
     DevBuf state;      // carried guide state of a streamed index (two copies)
     HostBuf pinned;
+    // The tile queue's counters in `tiles` may be nonzero: a launch sequence
+    // stopped before its finalizer was queued.
+    bool queue_dirty = false;
 };
 
 // Running state of one guide between slice windows: the two sums in FP64,

@@ -69,3 +69,23 @@ def test_streams_share_the_workspace(gpu_available, precision, threshold):
     finally:
         for s in (s1, s2, s3):
             s.destroy()
+
+
+@pytest.mark.parametrize("precision", [issl.PREC_FP32C, issl.PREC_FP64])
+def test_workspace_reuse_across_batch_sizes(gpu_available, precision):
+    """One index scores batches that shrink and grow again: the scoring
+    workspace is reused, and each result equals the same guides scored on a
+    fresh index (the tile queue's counters stay valid for every size)."""
+    print(f"[gpu-workspace] batch sizes, precision {precision}", flush=True)
+    hx = golden_index("mid50k_10x10")
+    g = load_golden("mid50k_10x10")["guides"]
+    kw = dict(threshold=0.0, form=issl.FORM_BATCHED, precision=precision)
+    sizes = [len(g), 37, 1, len(g) // 2, 5, len(g)]
+    with issl.Index.from_arrays(hx) as ix:
+        got = [ix.score(g[:m], **kw) for m in sizes]
+    for m, r in zip(sizes, got):
+        with issl.Index.from_arrays(hx) as fresh:
+            w = fresh.score(g[:m], **kw)
+        assert (bits(r.mit) == bits(w.mit)).all(), m
+        assert (bits(r.cfd) == bits(w.cfd)).all(), m
+        assert (r.site_count == w.site_count).all(), m
