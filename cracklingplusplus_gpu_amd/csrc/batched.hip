@@ -6,7 +6,8 @@
 //              prefix, the posting offsets, the tile length
 //              C_g = max(chunk, ceil(P_g/64)) and the tile count K_g; the
 //              guide's tiles (slot v = k*n + g) are appended to a device
-//              tile queue, tiles of several-tile guides ahead of single ones
+//              tile queue, tiles of several-tile guides and long single
+//              tiles ahead of the other single ones
 //   k_tiles    persistent waves that take tiles from the queue one at a time
 //              (from one of up to 256 sub-queues, so that no counter takes
 //              many atomics), so waves finish together whatever a tile
@@ -88,8 +89,9 @@ struct BatchWs {
     uint32_t *gchunk;    // [n] tile length C_g
     uint32_t *gtiles;    // [n] tile count K_g
     // Tile queue (null: the slot schedule).  Planner block b appends the slots
-    // of its guides with several tiles to region b % kRegions of the first
-    // list and of its single-tile guides to the same region of the second,
+    // of its guides with several tiles or a single tile longer than half a
+    // chunk to region b % kRegions of the first list and of its other
+    // single-tile guides to the same region of the second,
     // counting them in tctr[r] and tctr[kRegions + r] (set back to zero by
     // k_finalize).  The queue is the first list's regions in order, then the
     // second's.  The tile kernel splits it into sub-queues, each with a taken
@@ -337,8 +339,13 @@ struct PlanArgs {
 constexpr int kPlanBlock = 64;
 constexpr int kPlanGroup = 16;
 
-__device__ uint32_t plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const uint64_t *s_fold,
-                               const uint32_t *s_lb);
+// A guide's tile count and posting count.
+struct TileCount {
+    uint32_t K, P;
+};
+
+__device__ TileCount plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const uint64_t *s_fold,
+                                const uint32_t *s_lb);
 
 __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
     __shared__ uint64_t s_fold[kMaxSlices];
@@ -352,13 +359,18 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         s_lb[i] = a.meta[i].lbase;
     }
     __syncthreads();
-    uint32_t K = 0;
-    if (in) K = plan_guide(a, g, q, s_fold, s_lb);
+    TileCount tc{0, 0};
+    if (in) tc = plan_guide(a, g, q, s_fold, s_lb);
+    const uint32_t K = tc.K;
     if (a.ws.tlist) {
         // Append the wave's tiles to the queue: one atomic per list per wave,
         // on the counter of the block's region.
         const int lane = static_cast<int>(threadIdx.x);
-        const uint32_t ka = K > 1 ? K : 0, kb = K == 1 ? 1 : 0;
+        // The first list takes the tiles likely to cost most (several-tile
+        // guides, and single tiles longer than half a chunk), so that they
+        // start early and the cheap ones fill the end.
+        const bool first = K > 1 || (K == 1 && tc.P > a.chunk / 2);
+        const uint32_t ka = first ? K : 0, kb = (!first && K == 1) ? 1 : 0;
         uint32_t ia = ka, ib = kb;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -391,9 +403,9 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
     }
 }
 
-// The plan of guide g (k_plan); returns its tile count.
-__device__ uint32_t plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const uint64_t *s_fold,
-                               const uint32_t *s_lb) {
+// The plan of guide g (k_plan); returns its tile and posting counts.
+__device__ TileCount plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const uint64_t *s_fold,
+                                const uint32_t *s_lb) {
     const int S = a.S;
     const bool live = !a.state_in || (a.state_in[g].sites & kExited) == 0;
     uint32_t *pre = a.ws.pre + g * (S + 1);
@@ -441,7 +453,7 @@ __device__ uint32_t plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const 
     const uint32_t K = static_cast<uint32_t>((run + c - 1) / c);
     a.ws.gchunk[g] = static_cast<uint32_t>(c);
     a.ws.gtiles[g] = K;
-    return K;
+    return TileCount{K, static_cast<uint32_t>(run)};
 }
 
 // Final per-guide results: transform 10000/(100+S), the call and the output
@@ -537,7 +549,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     // starts one, so that the waves finish together whatever each tile
     // costs.  Sub-queue j (of Q) is served by the workgroups b with
     // b % Q == j and holds queue entries j, j + Q, j + 2Q, ...: its share of
-    // the several-tile guides' tiles first, then of the single tiles.  Slot
+    // the first list's tiles first, then of the second's.  Slot
     // schedule (guide streams wrote the plans): the slots of this wave are
     // w + j*W; 64 of them are checked per load and the real ones (k < K_g)
     // are taken in increasing order.
