@@ -424,16 +424,50 @@ def main() -> int:
     kms, nl = ix.timing_read()
     ix.timing_enable(False)
     k_avg_ms = kms / max(nl, 1)
-    # Algorithmic bytes per k_tiles dispatch.  This layout streams one 8 B site
-    # word (signature | occurrences) per posting plus two 8 B list bounds per
-    # (guide, slice); the reference's layout reads an 8 B posting entry and
-    # gathers an 8 B signature per posting, reported beside it.
+    res_built, res_used, _ = ix.residual()
+    # Algorithmic bytes per k_tiles dispatch.  The list-residual layout streams
+    # one 4 B site word per posting (the signature-inline layout 8 B), plus two
+    # 8 B list bounds per (guide, slice); the reference's layout reads an 8 B
+    # posting entry and gathers an 8 B signature per posting, reported beside it.
+    word = 4 if res_used else 8
+    bytes_layout = word * p_total + 16 * S * Q
     bytes_inline = 8 * p_total + 16 * S * Q
     bytes_ref_layout = 16 * p_total + 16 * S * Q
-    achieved = bytes_inline / (k_avg_ms * 1e-3) / 1e9
+    achieved = bytes_layout / (k_avg_ms * 1e-3) / 1e9
     log(f"k_tiles avg {k_avg_ms * 1e3:.1f} us/launch -> {achieved:,.0f} GB/s")
 
     res_batched = {k: v.to_host() for k, v in d_out.items()}
+
+    # ---- the same step with the other site-word layout at the same launch
+    # shape (the autotuner picks the layout with the shape); results must be
+    # the same bits
+    other_leg = None
+    if res_built:
+        ix.set_residual(not res_used)
+        for _ in range(max(1, args.warmup)):
+            ix.score_device(d_guides, Q, d_out, **kw)
+        hiprt.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ix.score_device(d_guides, Q, d_out, **kw)
+        hiprt.synchronize()
+        ms_i = (time.perf_counter() - t0) * 1e3 / args.steps
+        ix.timing_enable(True, repeat=TIMING_REPEAT)
+        for _ in range(args.steps):
+            ix.score_device(d_guides, Q, d_out, **kw)
+        kms_i, nl_i = ix.timing_read()
+        ix.timing_enable(False)
+        k_i = kms_i / max(nl_i, 1)
+        same_i = all(np.array_equal(d_out[k].to_host().view(np.uint8), res_batched[k].view(np.uint8))
+                     for k in d_out)
+        word_o = 8 if res_used else 4
+        other_leg = {"site_words": "signature-inline, 8 B" if res_used else "list-residual, 4 B",
+                     "value": round(Q / (ms_i * 1e-3), 1), "ms_per_step": round(ms_i, 4),
+                     "kernel_us": round(k_i * 1e3, 2),
+                     "layout_GBps": round((word_o * p_total + 16 * S * Q) / (k_i * 1e-3) / 1e9, 1),
+                     "same_bits": bool(same_i)}
+        log(f"other layout: {ms_i:.3f} ms/step, k_tiles {k_i * 1e3:.1f} us, same bits {same_i}")
+        ix.set_residual(res_used)
 
     # ---- plain form, same workload
     kwp = dict(kw, form=issl.FORM_PLAIN, precision=issl.PREC_FP64)
@@ -565,6 +599,7 @@ def main() -> int:
                    "max_dist": 4, "form": "batched", "precision": "fp32c",
                    "launch": "hipGraph replay" if args.graph else "eager",
                    "postings_per_step": p_total,
+                   "site_words": "list-residual, 4 B" if res_used else "signature-inline, 8 B",
                    "tuning": {"chunk": tn.chunk, "depth": tn.depth,
                               "grid_per_cu": tn.grid_per_cu,
                               "stream_depth": tn.direct_depth,
@@ -573,10 +608,11 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.workload), "kernel": "k_tiles",
                      "kernel_us": round(k_avg_ms * 1e3, 2),
-                     "bytes_per_launch": bytes_inline,
+                     "bytes_per_launch": bytes_layout,
                      "reference_layout_bytes_per_launch": bytes_ref_layout,
                      "reference_layout_GBps": round(bytes_ref_layout / (k_avg_ms * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
+        "other_layout": other_leg,
         "plain": {"value": round(Q / (plain_ms * 1e-3), 1), "unit": "guides/s",
                   "ms_per_step": round(plain_ms, 4), "kernel": "k_plain",
                   "kernel_us": round(plain_k_ms * 1e3, 2),

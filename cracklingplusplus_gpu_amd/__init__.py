@@ -300,6 +300,18 @@ class Index:
         check(lib().issl_index_get_info(self._h, byref(out)))
         return out
 
+    def set_residual(self, use: bool) -> None:
+        """Stream the list-residual words (4 B per posting) in the BATCHED tile
+        kernel when the index has them (the default), or the signature-inline
+        words (8 B).  Results are the same bits either way."""
+        check(lib().issl_index_set_residual(self._h, 1 if use else 0))
+
+    def residual(self) -> tuple:
+        """(built, in use, occurrence bits per residual word)."""
+        b, u, o = _capi.c_int32(), _capi.c_int32(), _capi.c_uint32()
+        check(lib().issl_index_get_residual(self._h, byref(b), byref(u), byref(o)))
+        return bool(b.value), bool(u.value), int(o.value)
+
     def validate(self) -> int:
         bad = c_uint64()
         check(lib().issl_index_validate(self._h, byref(bad)))

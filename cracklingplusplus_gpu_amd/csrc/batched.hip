@@ -88,6 +88,9 @@ struct BatchWs {
     uint64_t *delta;     // [n][S]   posting index = position + delta
     uint32_t *gchunk;    // [n] tile length C_g
     uint32_t *gtiles;    // [n] tile count K_g
+    // [n][S] the guide's residual codes for the list of slice s (list-residual
+    // layout), with s in bits 24..29; null: the signature-inline layout.
+    uint32_t *gres;
     // Tile queue (null: the slot schedule).  Planner block b appends the slots
     // of its guides with several tiles or a single tile longer than half a
     // chunk to region b % kRegions of the first list and of its other
@@ -433,6 +436,14 @@ __device__ TileCount plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const
                 e[i] = a.lstart[lb[i] + key + 1];
             }
         }
+        // (formed while the list bounds are in flight)
+        if (a.ws.gres) {
+#pragma unroll
+            for (int i = 0; i < kPlanGroup; ++i)
+                if (s0 + i < S)
+                    a.ws.gres[g * S + s0 + i] =
+                        residual_code(q, fold[i]) | (static_cast<uint32_t>(s0 + i) << 24);
+        }
 #pragma unroll
         for (int i = 0; i < kPlanGroup; ++i) {
             const int s = s0 + i;
@@ -486,6 +497,8 @@ __device__ __forceinline__ void write_guide(const ScoreScalars &sc, const DevRes
 struct TileArgs {
     const uint64_t *guides;
     const uint64_t *site;   // signature-inline words
+    const uint32_t *res;    // list-residual words (RES kernels)
+    uint32_t res_shift;     // their first occurrence bit
     const uint64_t *post;   // postings (escape occ, ids)
     const SliceMeta *meta;
     const void *mit;        // float or double by precision
@@ -509,35 +522,76 @@ constexpr uint32_t kRing = 64 * (kRingChk + 1);
 constexpr uint32_t kHitRing = 128;
 
 // Postings in flight per lane: DEPTH.  QUEUE: tiles come from the planner's
-// queue (else from the slot schedule).
-template <int PREC, int DEPTH, bool QUEUE>
+// queue (else from the slot schedule).  RES: the kernel streams the
+// list-residual words (4 B per posting) instead of the signature-inline words
+// (8 B); candidates, hits, their order and every term are the same.
+//
+// In RES form a mismatch is carried as a CFD-table index 4p + x (position p,
+// guide ^ site = x there), 7 bits, kAbsent7 for an absent one: s_cq[kAbsent7]
+// is 1.0 (p = 31 lies past the sequence) and the MIT rank terms are read by
+// the same index (0 at kAbsent7).
+constexpr uint32_t kAbsent7 = 127;
+template <int PREC, int DEPTH, bool QUEUE, bool RES>
 __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
+    // Site word as streamed and ringed; the hit ring holds the word (or, in
+    // RES form, the clipped occurrence count).
+    using Word = typename std::conditional<RES, uint32_t, uint64_t>::type;
     __shared__ Real s_cfd[kCfdEntries];
     __shared__ uint64_t s_fold[kMaxSlices];
     __shared__ uint64_t s_posmask[kNoPos + 1];
-    __shared__ uint32_t s_rank[3][kRankTerms];
+    __shared__ uint32_t s_rank[3][RES ? 128 : kRankTerms];
+    // RES: slice s's residual positions, ascending (the code field's order).
+    __shared__ uint8_t s_rabs[RES ? kMaxSlices : 1][kResPositions];
     // The current tile's CFD factors by (position, guide ^ site there).
     __shared__ Real s_cq[kTileWaves][(kNoPos + 1) * 4];
     // Tile plans, double-buffered: the next tile's plan lands in the other half.
     __shared__ uint32_t s_pre[kTileWaves][2][kMaxSlices + 1];
     __shared__ uint64_t s_delta[kTileWaves][2][kMaxSlices];
+    // RES: the guide's residual codes per slice | slice << 24.
+    __shared__ uint32_t s_gres[RES ? kTileWaves : 1][2][kMaxSlices];
     // Candidate ring: tile-relative position | slice << kRelBits, and the site word.
     __shared__ uint32_t s_rpos[kTileWaves][kRing];
-    __shared__ uint64_t s_rword[kTileWaves][kRing];
+    __shared__ Word s_rword[kTileWaves][kRing];
     // Hit ring: first-seen candidates with packed mismatch positions and distance.
     __shared__ uint32_t s_hpos[kTileWaves][kHitRing];
-    __shared__ uint64_t s_hword[kTileWaves][kHitRing];
+    __shared__ Word s_hword[kTileWaves][kHitRing];
     __shared__ uint32_t s_hmis[kTileWaves][kHitRing];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Real *cfdg = static_cast<const Real *>(a.cfd);
     const Real *mitg = static_cast<const Real *>(a.mit);
     for (int i = tid; i < kCfdEntries; i += kTileBlock) s_cfd[i] = cfdg[i];
     if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
-    build_rank_terms(s_rank, tid, kTileBlock);
+    if constexpr (RES) {
+        // MIT rank terms by CFD-table index 4p + x (the offsets of mit_rank).
+        for (int i = tid; i < 3 * 128; i += kTileBlock) {
+            const uint32_t k = static_cast<uint32_t>(i) / 128;
+            const uint32_t p = (static_cast<uint32_t>(i) % 128) >> 2;
+            uint32_t v = 0;
+            if (p < 20) {
+                if (k == 0) v = 20u + p * (p - 1) / 2;
+                else if (k == 1) v = 190u + p * (p - 1) * (p - 2) / 6;
+                else v = 1140u + p * (p - 1) * (p - 2) * (p - 3) / 24;
+            }
+            s_rank[k][i % 128] = v;
+        }
+        if (tid < a.S) {
+            const uint64_t f = a.meta[tid].fold;
+            uint32_t j = 0;
+            for (uint32_t p = 0; p < static_cast<uint32_t>(kResPositions); ++p)
+                if (!((f >> (2 * p)) & 1ULL)) s_rabs[tid][j++] = static_cast<uint8_t>(p);
+            for (; j < static_cast<uint32_t>(kResPositions); ++j) s_rabs[tid][j] = static_cast<uint8_t>(kNoPos);
+        }
+    } else {
+        build_rank_terms(s_rank, tid, kTileBlock);
+    }
     __syncthreads();
     build_posmask(s_posmask, s_fold, a.S, tid);
     __syncthreads();
+    // RES: the code field's fold bits (bit 2j per residual position j) and the
+    // occurrence escape.
+    const uint32_t res_fold = RES ? (0x55555555u & ((1u << a.res_shift) - 1u)) : 0u;
+    const uint32_t res_esc = RES ? (0xFFFFFFFFu >> a.res_shift) : 0u;
     const Real cfd_pam = s_cfd[320 + 10];   // NGG
     const int S = a.S;
     const uint32_t n = a.n;
@@ -645,6 +699,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         uint32_t g, k, P, C, K;
         uint32_t pre;
         uint64_t dl;
+        uint32_t gr;   // RES: the guide's residual codes for slice `lane` | lane << 24
     };
     auto plan_load = [&](uint32_t v, Plan &m) {
         m.g = v % n;
@@ -656,15 +711,18 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         m.K = a.ws.gtiles[m.g];
         m.pre = 0;
         m.dl = 0;
+        m.gr = 0;
         if (lane < S) {
             m.pre = a.ws.pre[row + lane];
             m.dl = a.ws.delta[static_cast<size_t>(m.g) * S + lane];
+            if constexpr (RES) m.gr = a.ws.gres[static_cast<size_t>(m.g) * S + lane];
         }
     };
     auto plan_commit = [&](const Plan &m, int b) {
         if (lane < S) {
             s_pre[wave][b][lane] = m.pre;
             s_delta[wave][b][lane] = m.dl;
+            if constexpr (RES) s_gres[wave][b][lane] = m.gr;
         }
         if (lane == 0) s_pre[wave][b][S] = m.P;  // S may be 64: no lane S
         __builtin_amdgcn_wave_barrier();
@@ -716,14 +774,22 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             if (static_cast<uint32_t>(lane) < cnt) {
                 const uint32_t e = (hhead + lane) & (kHitRing - 1);
                 const uint32_t ent = s_hpos[wave][e];
-                const uint64_t wv = s_hword[wave][e];
+                const Word wv = s_hword[wave][e];
                 const uint32_t pk = s_hmis[wave][e];
+                // 2p (or, RES, 4p + x) of each mismatch
                 const uint32_t m2[4] = {pk & 127u, (pk >> 7) & 127u, (pk >> 14) & 127u,
-                                        (pk >> 21) & 127u};   // 2p
+                                        (pk >> 21) & 127u};
                 const int dist = static_cast<int>(pk >> 28);
-                const uint64_t sg = wv & kSigBits;
-                uint32_t occ = static_cast<uint32_t>(wv >> 40);
-                if (occ == kOccEscape || a.mark_distinct) {
+                uint32_t occ;
+                bool esc;
+                if constexpr (RES) {
+                    occ = wv;
+                    esc = occ == res_esc;
+                } else {
+                    occ = static_cast<uint32_t>(wv >> 40);
+                    esc = occ == kOccEscape;
+                }
+                if (esc || a.mark_distinct) {
                     const int sl = static_cast<int>(ent >> kRelBits);
                     const uint64_t addr =
                         a0 + (ent & ((1u << kRelBits) - 1)) + s_delta[wave][buf][sl];
@@ -744,16 +810,26 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 // The MIT load goes first so the CFD work covers its latency;
                 // an exact match (dist 0, all positions kNoPos) reads a
                 // valid entry that is then not used.
-                if (a.calc_mit) mv = mitg[(m2[0] >> 1) + s_rank[0][m2[1]] + s_rank[1][m2[2]] + s_rank[2][m2[3]]];
+                if (a.calc_mit) {
+                    if constexpr (RES)
+                        mv = mitg[(m2[0] >> 2) + s_rank[0][m2[1]] + s_rank[1][m2[2]] + s_rank[2][m2[3]]];
+                    else
+                        mv = mitg[(m2[0] >> 1) + s_rank[0][m2[1]] + s_rank[1][m2[2]] + s_rank[2][m2[3]]];
+                }
                 if (a.calc_cfd) {
                     // PAM factor, then one factor per mismatch in position
                     // order (1 for absent ones): the order of cfd_product.
-                    const uint64_t xw = q ^ sg;
                     Real c = cfd_pam;
+                    if constexpr (RES) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const uint32_t xv = static_cast<uint32_t>(xw >> (m2[i] & 63u)) & 3u;
-                        c = c * s_cq[wave][m2[i] * 2 + xv];
+                        for (int i = 0; i < 4; ++i) c = c * s_cq[wave][m2[i]];
+                    } else {
+                        const uint64_t xw = q ^ (wv & kSigBits);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const uint32_t xv = static_cast<uint32_t>(xw >> (m2[i] & 63u)) & 3u;
+                            c = c * s_cq[wave][m2[i] * 2 + xv];
+                        }
                     }
                     tc = (dist == 0 ? Real(1) : c) * ro;
                 }
@@ -773,20 +849,41 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             __builtin_amdgcn_wave_barrier();
             bool hit = false;
             uint32_t ent = 0, pk = 0;
-            uint64_t wv = 0;
+            Word wv = 0;
             if (static_cast<uint32_t>(lane) < cnt) {
                 const uint32_t e = (head + lane) & (kRing - 1);
                 ent = s_rpos[wave][e];
                 wv = s_rword[wave][e];
                 const int sl = static_cast<int>(ent >> kRelBits);
-                const uint32_t f = fold20(static_cast<uint32_t>(q ^ wv),
-                                          static_cast<uint32_t>((q ^ wv) >> 32));
-                const uint32_t dist = static_cast<uint32_t>(__popc(f));
-                uint32_t mp[4];
-                mismatches20(f, mp);
-                hit = first_seen(mp, sl, s_posmask);
-                // doubled positions (the shift that picks a base), 7 bits each
-                pk = (mp[0] << 1) | (mp[1] << 8) | (mp[2] << 15) | (mp[3] << 22) | (dist << 28);
+                if constexpr (RES) {
+                    // Residual fold: bit 2j for each differing residual
+                    // position j; positions ascend with j.  The ring holds
+                    // x = word ^ (guide codes | slice << 24).
+                    const uint32_t x = wv;
+                    uint32_t f = ((x >> 1) | x) & res_fold;
+                    const uint32_t dist = static_cast<uint32_t>(__popc(f));
+                    uint32_t mp[4], ix[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t b2 = f ? static_cast<uint32_t>(__builtin_ctz(f)) : 0u;
+                        mp[i] = f ? static_cast<uint32_t>(s_rabs[sl][b2 >> 1]) : kNoPos;
+                        ix[i] = f ? ((mp[i] << 2) | ((x >> b2) & 3u)) : kAbsent7;
+                        f &= f - 1;
+                    }
+                    hit = first_seen(mp, sl, s_posmask);
+                    pk = ix[0] | (ix[1] << 7) | (ix[2] << 14) | (ix[3] << 21) | (dist << 28);
+                    // the hit ring keeps the occurrence field
+                    wv = (x ^ (static_cast<uint32_t>(sl) << 24)) >> a.res_shift;
+                } else {
+                    const uint32_t f = fold20(static_cast<uint32_t>(q ^ wv),
+                                              static_cast<uint32_t>((q ^ wv) >> 32));
+                    const uint32_t dist = static_cast<uint32_t>(__popc(f));
+                    uint32_t mp[4];
+                    mismatches20(f, mp);
+                    hit = first_seen(mp, sl, s_posmask);
+                    // doubled positions (the shift that picks a base), 7 bits each
+                    pk = (mp[0] << 1) | (mp[1] << 8) | (mp[2] << 15) | (mp[3] << 22) | (dist << 28);
+                }
             }
             head += cnt;
             const uint64_t b = __ballot(hit);
@@ -822,14 +919,18 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         // bits above them and the bit carried across the halves by the shift.
         const uint32_t ql = static_cast<uint32_t>(q), qh = static_cast<uint32_t>(q >> 32);
         const uint64_t *site_p = a.site + dl;   // this lane's current slice, by position
+        // RES: the same for the residual words, and the guide's residual codes
+        // for the current slice | slice << 24.
+        const uint32_t *res_p = RES ? a.res + dl : nullptr;
+        uint32_t gsl = RES ? s_gres[wave][buf][cur] : 0u;
         // One round: DEPTH loads of 64 consecutive postings, then the filter.
         // FULL rounds lie inside the tile; in the last round lanes past the
         // tile read its last posting again (every lane loads without a
         // branch) and fail the filter.
         auto round = [&](uint32_t base, auto full_c) {
             constexpr bool FULL = decltype(full_c)::value;
-            uint64_t wv[DEPTH];
-            uint32_t sl[DEPTH];
+            Word wv[DEPTH];
+            uint32_t sl[DEPTH];   // slice (RES: gsl)
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
                 const uint32_t pos = base + 64 * d + lane;
@@ -839,20 +940,40 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                         ++cur;
                         nb = s_pre[wave][buf][cur + 1];
                     } while (pc >= nb);
-                    site_p = a.site + s_delta[wave][buf][cur];
+                    if constexpr (RES) {
+                        res_p = a.res + s_delta[wave][buf][cur];
+                        gsl = s_gres[wave][buf][cur];
+                    } else {
+                        site_p = a.site + s_delta[wave][buf][cur];
+                    }
                 }
-                sl[d] = static_cast<uint32_t>(cur);
-                wv[d] = site_p[pc];
+                if constexpr (RES) {
+                    sl[d] = gsl;
+                    wv[d] = res_p[pc];
+                } else {
+                    sl[d] = static_cast<uint32_t>(cur);
+                    wv[d] = site_p[pc];
+                }
             }
 #pragma unroll
             for (int d = 0; d < DEPTH; ++d) {
-                // All ones in the low half: 16 differing positions, no candidate.
-                const uint32_t kill = (FULL || base + 64 * d + lane < a1) ? 0u : ~0u;
-                const uint32_t xl = (ql ^ static_cast<uint32_t>(wv[d])) | kill;
-                const uint32_t xh = qh ^ static_cast<uint32_t>(wv[d] >> 32);
-                const uint32_t fl = ((xl >> 1) | xl) & 0x55555555u;
-                const uint32_t fh = ((xh >> 1) | xh) & 0x55u;
-                const bool cand = __popc(fl) + __popc(fh) <= 4;
+                bool cand;
+                if constexpr (RES) {
+                    // 8 added to the count of a lane past the tile: no candidate.
+                    // The ring keeps word ^ (guide codes | slice << 24).
+                    const uint32_t kill = (FULL || base + 64 * d + lane < a1) ? 0u : 8u;
+                    wv[d] ^= sl[d];
+                    const uint32_t f = ((wv[d] >> 1) | wv[d]) & res_fold;
+                    cand = __popc(f) + kill <= 4;
+                } else {
+                    // All ones in the low half: 16 differing positions, no candidate.
+                    const uint32_t kill = (FULL || base + 64 * d + lane < a1) ? 0u : ~0u;
+                    const uint32_t xl = (ql ^ static_cast<uint32_t>(wv[d])) | kill;
+                    const uint32_t xh = qh ^ static_cast<uint32_t>(wv[d] >> 32);
+                    const uint32_t fl = ((xl >> 1) | xl) & 0x55555555u;
+                    const uint32_t fh = ((xh >> 1) | xh) & 0x55u;
+                    cand = __popc(fl) + __popc(fh) <= 4;
+                }
                 const uint64_t b = __ballot(cand);
                 if (b) {
                     const uint32_t r = __builtin_amdgcn_mbcnt_hi(
@@ -860,7 +981,8 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
                     if (cand) {
                         const uint32_t e = (tail + r) & (kRing - 1);
-                        s_rpos[wave][e] = (base - a0 + 64 * d + lane) | (sl[d] << kRelBits);
+                        const uint32_t slice = RES ? (sl[d] >> 24) : sl[d];
+                        s_rpos[wave][e] = (base - a0 + 64 * d + lane) | (slice << kRelBits);
                         s_rword[wave][e] = wv[d];
                     }
                     tail += static_cast<uint32_t>(__popcll(b));
@@ -1423,15 +1545,16 @@ namespace {
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // fresh: the buffer was (re)allocated, so the queue counters are not zero.
-int carve(Workspace &ws, uint64_t n, int S, BatchWs *b, bool *fresh) {
+int carve(Workspace &ws, uint64_t n, int S, bool res, BatchWs *b, bool *fresh) {
     const size_t sz_pre = align256(n * (S + 1) * 4);
+    const size_t sz_gres = res ? align256(n * std::max(S, 1) * 4) : 0;
     const size_t sz_delta = align256(n * std::max(S, 1) * 8);
     const size_t sz_u32 = align256(n * 4);
     const size_t sz_part = align256(n * kMaxTiles * sizeof(TilePart));
     const size_t sz_list = align256(kRegions * region_blocks(n) * 64 * (kMaxTiles + 1) * 4);
     const size_t sz_ctr = align256(kQueueCtrs * 4);
     const void *old = ws.tiles.p;
-    int rc = ws.tiles.reserve(sz_ctr + sz_pre + sz_delta + 2 * sz_u32 + sz_part + sz_list);
+    int rc = ws.tiles.reserve(sz_ctr + sz_pre + sz_delta + 2 * sz_u32 + sz_part + sz_list + sz_gres);
     if (rc != ISSL_OK) return rc;
     *fresh = ws.tiles.p != old;
     char *p = static_cast<char *>(ws.tiles.p);
@@ -1451,24 +1574,31 @@ int carve(Workspace &ws, uint64_t n, int S, BatchWs *b, bool *fresh) {
     b->part = reinterpret_cast<TilePart *>(p);
     p += sz_part;
     b->tlist = reinterpret_cast<uint32_t *>(p);
+    p += sz_list;
+    b->gres = res ? reinterpret_cast<uint32_t *>(p) : nullptr;
     return ISSL_OK;
 }
 
 // e0/e1: timing events attached to the dispatch (null when timing is off).
+// The RES kernels need the tile queue and the planner's guide residuals.
 template <int PREC>
 int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st, hipEvent_t e0,
                        hipEvent_t e1) {
     const dim3 g(grid), b(kTileBlock);
     if (depth != 4 && depth != 8 && depth != 16)
         return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
-    if (ta.ws.tlist) {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true>), g, b, 0, st, e0, e1, 0, ta);
+    if (ta.res && ta.ws.tlist && ta.ws.gres) {
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true, true>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true, true>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true, true>), g, b, 0, st, e0, e1, 0, ta);
+    } else if (ta.ws.tlist) {
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true, false>), g, b, 0, st, e0, e1, 0, ta);
     } else {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, false>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, false>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, false, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, false, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, false, false>), g, b, 0, st, e0, e1, 0, ta);
     }
     return ISSL_OK;
 }
@@ -1514,9 +1644,15 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
                  int precision, const issl_tuning &tn, const DevResults &out, uint64_t *distinct,
                  Workspace &ws, hipStream_t st, const Window &win) {
     const int S = static_cast<int>(ix->S);
+    const bool carry = win.state_in || win.state_out;
+    // Guide streams (k_stream) read the resident index only.
+    const bool direct = tn.heavy_kpost != 0 && !carry && win.s0 == 0 && win.s1 == S;
+    // The tile kernel streams the list-residual words when the window has
+    // them and the planner's queue feeds it.
+    const bool res = win.res != nullptr && !direct;
     BatchWs b;
     bool fresh = false;
-    int rc = carve(ws, n, S, &b, &fresh);
+    int rc = carve(ws, n, S, res, &b, &fresh);
     if (rc != ISSL_OK) return rc;
     // The counters are zero unless the buffer is new or an earlier sequence
     // stopped before its k_finalize was queued.
@@ -1525,9 +1661,6 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     const bool fp32 = precision == ISSL_PREC_FP32C;
     const void *mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
     const void *cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
-    const bool carry = win.state_in || win.state_out;
-    // Guide streams (k_stream) read the resident index only.
-    const bool direct = tn.heavy_kpost != 0 && !carry && win.s0 == 0 && win.s1 == S;
     const int finish_single = (!carry && !sc.use_cut) ? 1 : 0;
     const bool tiles = !direct || tn.heavy_kpost != kHeavyNever;
     const uint32_t gw = static_cast<uint32_t>((n + 3) / 4);
@@ -1575,6 +1708,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     TileArgs ta;
     ta.guides = d_guides;
     ta.site = win.inl;
+    ta.res = res ? win.res : nullptr;
+    ta.res_shift = win.res_shift;
     ta.post = win.post;
     ta.meta = ix->d_meta;
     ta.mit = mit;
@@ -1661,6 +1796,8 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
         w.s1 = static_cast<int>(ix->S);
         w.post = ix->d_post;
         w.inl = ix->d_inline;
+        w.res = ix->res_use ? ix->d_res : nullptr;
+        w.res_shift = ix->res_shift;
     }
     for (uint64_t i = 0; i < n; i += kMaxBatch) {
         const uint64_t m = std::min<uint64_t>(kMaxBatch, n - i);

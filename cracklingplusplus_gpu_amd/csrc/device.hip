@@ -5,6 +5,8 @@
 //   d_sig    u64[N]        signatures, id order (the .issl sig block)
 //   d_post   u64[S*N]      posting entries occ<<32 | id, the .issl order
 //   d_inline u64[S*N]      same order, sig | occ<<40 (BATCHED form)
+//   d_res    u32[S*N]      same order, list-residual words (the tile kernel's
+//                          stream; built when every slice has >= 8 positions)
 //   d_lstart u32[sum(4^w+1)] per slice, start of each key's list within the
 //                          slice's N entries, plus an end sentinel
 // A guide's candidate sequence is the concatenation, slice by slice, of the
@@ -422,6 +424,19 @@ __global__ void k_inline(const uint64_t *__restrict__ post, const uint64_t *__re
     }
 }
 
+// List-residual words of one slice: the codes at the positions outside the
+// slice's fold mask, ascending, then min(occ, escape) from bit `shift`.
+__global__ void k_residual(const uint64_t *__restrict__ post, const uint64_t *__restrict__ sig,
+                           uint32_t *__restrict__ out, uint64_t n, uint64_t fold, uint32_t shift) {
+    const uint32_t esc = 0xFFFFFFFFu >> shift;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t e = post[i];
+        const uint32_t occ = static_cast<uint32_t>(e >> 32);
+        out[i] = residual_code(sig[static_cast<uint32_t>(e)], fold) | (std::min(occ, esc) << shift);
+    }
+}
+
 // Index check: each entry of slice s sits in the list of its own key, its id
 // is < N, and every id occurs exactly once in the slice.
 __global__ void k_validate(const uint64_t *__restrict__ post, const uint64_t *__restrict__ sig,
@@ -468,8 +483,28 @@ int build_inline_layout(issl_index *ix) {
     if (total == 0) return ISSL_OK;
     hipLaunchKernelGGL(k_inline, dim3(4096), dim3(256), 0, ix->stream, ix->d_post, ix->d_sig,
                        ix->d_inline, total);
+    if (ix->d_res)
+        for (uint64_t s = 0; s < ix->S; ++s)
+            hipLaunchKernelGGL(k_residual, dim3(2048), dim3(256), 0, ix->stream, ix->d_post + s * ix->n,
+                               ix->d_sig, ix->d_res + s * ix->n, ix->n, ix->meta_h[s].fold,
+                               ix->res_shift);
     ISSL_HIP(hipGetLastError());
     return ISSL_OK;
+}
+
+// Whether the list-residual words can be built: every slice within positions
+// 0..19 and at least kResMinWidth wide; *shift = 2 (20 - narrowest width).
+bool residual_layout_fits(const issl_index *ix, uint32_t *shift) {
+    if (ix->S == 0) return false;
+    uint32_t min_w = kResPositions;
+    for (const SliceMeta &m : ix->meta_h) {
+        for (uint32_t j = 0; j < m.width; ++j)
+            if (m.pos[j] >= kResPositions) return false;
+        min_w = std::min(min_w, m.width);
+    }
+    if (min_w < static_cast<uint32_t>(kResMinWidth)) return false;
+    *shift = 2 * (kResPositions - min_w);
+    return true;
 }
 
 }  // namespace issl
@@ -485,8 +520,8 @@ void free_index(issl_index *ix) {
     (void)hipSetDevice(ix->device);
     if (ix->stream) (void)hipStreamSynchronize(ix->stream);
     stream_free(ix);
-    void *ptrs[] = {ix->d_sig, ix->d_post, ix->d_inline, ix->d_lstart, ix->d_meta, ix->d_mit64,
-                    ix->d_mit32, ix->d_cfd64, ix->d_cfd32, ix->d_distinct};
+    void *ptrs[] = {ix->d_sig, ix->d_post, ix->d_inline, ix->d_res, ix->d_lstart, ix->d_meta,
+                    ix->d_mit64, ix->d_mit32, ix->d_cfd64, ix->d_cfd32, ix->d_distinct};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ix->ws.guides.release();
@@ -598,9 +633,11 @@ extern "C" int issl_device_memory(int device, uint64_t *free_bytes, uint64_t *to
 }
 
 extern "C" uint64_t issl_index_bytes_needed(uint64_t n_sites, uint64_t n_slices, uint64_t n_lists) {
-    // signatures, postings, signature-inline words, list starts, slice meta,
-    // MIT (f64 + f32), CFD (f64 + f32), distinct bitmap
-    return n_sites * 8 + 2 * n_slices * n_sites * 8 + (n_lists + n_slices) * 4 +
+    // signatures, postings, signature-inline words, list-residual words (not
+    // built when a slice is narrower than 8 positions), list starts, slice
+    // meta, MIT (f64 + f32), CFD (f64 + f32), distinct bitmap
+    return n_sites * 8 + 2 * n_slices * n_sites * 8 + n_slices * n_sites * 4 +
+           (n_lists + n_slices) * 4 +
            n_slices * sizeof(SliceMeta) + kMitEntries * 12 + kCfdTable * 12 +
            (n_sites / 64 + 1) * 8;
 }
@@ -667,6 +704,9 @@ int alloc_index(issl_index *ix, uint64_t lstart_count, bool postings) {
     if (postings) {
         if ((rc = dev_alloc(ix, &ix->d_post, n_slices * n_sites)) != ISSL_OK) return rc;
         if ((rc = dev_alloc(ix, &ix->d_inline, n_slices * n_sites)) != ISSL_OK) return rc;
+        if (residual_layout_fits(ix, &ix->res_shift) &&
+            (rc = dev_alloc(ix, &ix->d_res, n_slices * n_sites)) != ISSL_OK)
+            return rc;
     }
     if ((rc = dev_alloc(ix, &ix->d_lstart, lstart_count)) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix, &ix->d_meta, std::max<uint64_t>(n_slices, 1))) != ISSL_OK) return rc;
@@ -956,6 +996,23 @@ extern "C" int issl_index_get_info(const issl_index *ix, issl_index_info *out) {
     out->device_bytes = ix->device_bytes;
     out->device = ix->device;
     out->max_width = ix->max_w;
+    return ok();
+}
+
+extern "C" int issl_index_set_residual(issl_index *ix, int32_t use) {
+    if (!ix) return fail(ISSL_ERR_ARG, "null index");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    ix->res_use = use != 0;
+    return ok();
+}
+
+extern "C" int issl_index_get_residual(const issl_index *ix, int32_t *built, int32_t *in_use,
+                                       uint32_t *occ_bits) {
+    if (!ix) return fail(ISSL_ERR_ARG, "null index");
+    const bool b = ix->d_res != nullptr;
+    if (built) *built = b ? 1 : 0;
+    if (in_use) *in_use = (b && ix->res_use) ? 1 : 0;
+    if (occ_bits) *occ_bits = b ? 32u - ix->res_shift : 0u;
     return ok();
 }
 

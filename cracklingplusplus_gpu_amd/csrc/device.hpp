@@ -54,6 +54,32 @@ constexpr int32_t kHeavyNever = ISSL_HEAVY_NEVER;
 constexpr uint64_t kSigBits = (1ULL << 40) - 1;
 constexpr uint32_t kOccEscape = 0xFFFFFF;
 
+// List-residual site word (BATCHED tile kernel): every site in the list of
+// slice s for key k holds k's bases at the slice's positions, which are the
+// guide's own when the guide selected that list, so only the other 20 - w_s
+// positions can differ.  Their 2-bit codes, ascending by position, fill bits
+// 0 .. 2 r_s - 1 (r_s = 20 - w_s); bits res_shift .. 31 hold
+// min(occ, escape), escape = 2^(32 - res_shift) - 1 meaning "read occ from
+// the posting"; res_shift = 2 max_s r_s.  Built when every slice lies within
+// positions 0..19 and has at least kResMinWidth positions (at least 8
+// occurrence bits).
+constexpr int kResMinWidth = 8;
+constexpr int kResPositions = 20;
+
+// The codes of `sig` at the positions 0..19 outside `fold` (bit 2j per slice
+// position j), ascending, packed from bit 0: a list-residual word's code
+// field, and a guide's residual for the lists it selects.
+__host__ __device__ __forceinline__ uint32_t residual_code(uint64_t sig, uint64_t fold) {
+    uint32_t r = 0, j = 0;
+#pragma unroll
+    for (int p = 0; p < kResPositions; ++p) {
+        const uint32_t keep = static_cast<uint32_t>(~(fold >> (2 * p))) & 1u;
+        r |= (static_cast<uint32_t>(sig >> (2 * p)) & 3u & (0u - keep)) << (2 * j);
+        j += keep;
+    }
+    return r;
+}
+
 // Device CFD table: 320 position factors, 16 PAM factors, then 1.0.
 constexpr int kCfdTable = 337;
 
@@ -126,6 +152,8 @@ struct Window {
     int s0 = 0, s1 = 0;
     const uint64_t *post = nullptr;
     const uint64_t *inl = nullptr;
+    const uint32_t *res = nullptr;   // list-residual words (resident index only), or null
+    uint32_t res_shift = 0;
     const GuideState *state_in = nullptr;
     GuideState *state_out = nullptr;
 };
@@ -157,6 +185,9 @@ struct issl_index {
     uint64_t *d_sig = nullptr;      // [n] signatures
     uint64_t *d_post = nullptr;     // [S*n] occ<<32 | id, .issl order
     uint64_t *d_inline = nullptr;   // [S*n] sig | occ<<40 (same order)
+    uint32_t *d_res = nullptr;      // [S*n] list-residual words (same order), or null
+    uint32_t res_shift = 0;         // first occurrence bit of a residual word
+    bool res_use = true;            // BATCHED streams d_res when it is built
     uint32_t *d_lstart = nullptr;   // per slice: 4^w + 1 list starts
     issl::SliceMeta *d_meta = nullptr;
     double *d_mit64 = nullptr;      // [6195] by combinadic rank

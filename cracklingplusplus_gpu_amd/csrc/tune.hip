@@ -32,12 +32,14 @@ std::string shape_key(const issl_index *ix) {
     for (char &c : arch)
         if (c == ' ' || c == ':') c = '_';
     std::ostringstream k;
-    k << "t2/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
+    k << "t3/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
     for (const SliceMeta &m : ix->meta_h) k << m.width << ',';
     return k.str();
 }
 
-bool cache_lookup(const char *path, const std::string &key, issl_tuning *t) {
+// A cache line: key, the tuning's fields, then the site-word layout (1:
+// list-residual words).
+bool cache_lookup(const char *path, const std::string &key, issl_tuning *t, int *res) {
     if (!path) return false;
     std::ifstream in(path);
     std::string line;
@@ -46,10 +48,12 @@ bool cache_lookup(const char *path, const std::string &key, issl_tuning *t) {
         std::istringstream ls(line);
         std::string k;
         issl_tuning v{};
+        int r = 0;
         if (ls >> k >> v.chunk >> v.block >> v.depth >> v.grid_per_cu >> v.direct_depth >>
-                v.heavy_kpost >> v.us_per_guide &&
+                v.heavy_kpost >> v.us_per_guide >> r &&
             k == key) {
             *t = v;
+            *res = r;
             found = true;  // the last entry for a key wins
         }
     }
@@ -86,9 +90,12 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     if (ix->src) return fail(ISSL_ERR_ARG, "autotune needs a resident index");
     const std::string key = shape_key(ix);
     issl_tuning cached{};
-    if (cache_lookup(cache_path, key, &cached)) {
+    int cached_res = 0;
+    if (cache_lookup(cache_path, key, &cached, &cached_res)) {
         int rc = issl_set_tuning(ix, &cached);
         if (rc == ISSL_OK) {
+            std::lock_guard<std::mutex> lk(ix->mu);
+            ix->res_use = cached_res != 0;
             *best = ix->tuning;
             return ok();
         }
@@ -146,19 +153,31 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
         if (t.us_per_guide < top.us_per_guide) top = t;
         return ISSL_OK;
     };
-    // (1) every guide on the persistent tile kernel: tile shape and occupancy
+    // (1) every guide on the persistent tile kernel: tile shape and occupancy,
+    // for each site-word layout the index holds (the list-residual words move
+    // half the bytes; they pay where the streams are long enough to be
+    // bandwidth-bound, and the signature-inline words where a shorter kernel
+    // is bound by latency and issue)
     const int chunks[] = {1024, 2048, 4096};
     const int depths[] = {4, 8, 16};
     const int grids[] = {4, 6, 8};
     issl_tuning tiles_best{};
     tiles_best.us_per_guide = 1e300;
-    for (int c : chunks)
-        for (int d : depths)
-            for (int gpc : grids) {
-                issl_tuning t{c, 256, d, gpc, 8, 0, 0};
-                if ((rc = measure(t)) != ISSL_OK) return rc;
-                if (t.us_per_guide < tiles_best.us_per_guide) tiles_best = t;
-            }
+    bool res_best = false;
+    for (int layout = ix->d_res ? 1 : 0; layout >= 0; --layout) {
+        ix->res_use = layout != 0;
+        for (int c : chunks)
+            for (int d : depths)
+                for (int gpc : grids) {
+                    issl_tuning t{c, 256, d, gpc, 8, 0, 0};
+                    if ((rc = measure(t)) != ISSL_OK) return rc;
+                    if (t.us_per_guide < tiles_best.us_per_guide) {
+                        tiles_best = t;
+                        res_best = ix->res_use;
+                    }
+                }
+    }
+    ix->res_use = res_best;
     if (tiles_best.us_per_guide < top.us_per_guide) top = tiles_best;
     // (2) one wave per guide, optionally sending long guides to the tile kernel
     const int ddepths[] = {4, 8, 16};
@@ -181,9 +200,9 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%s %d %d %d %d %d %d %.6g\n", key.c_str(), top.chunk, top.block,
+            fprintf(f, "%s %d %d %d %d %d %d %.6g %d\n", key.c_str(), top.chunk, top.block,
                     top.depth, top.grid_per_cu, top.direct_depth, top.heavy_kpost,
-                    top.us_per_guide);
+                    top.us_per_guide, ix->res_use ? 1 : 0);
             fclose(f);
         }
     }

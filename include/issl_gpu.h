@@ -179,6 +179,23 @@ int issl_index_close(issl_index *ix);
 
 int issl_index_get_info(const issl_index *ix, issl_index_info *out);
 
+/* Site-word layout of the BATCHED tile kernel.  No reference counterpart (the
+ * reference reads a posting and then gathers the site's signature,
+ * lib/ISSLScoringModule.cpp:241-292).  Every site in the list that a guide's
+ * slice key selects holds the guide's bases at that slice's positions, so
+ * only the other 20 - w positions can differ: the list-residual layout keeps
+ * those positions' codes and a clipped occurrence count in 4 bytes per
+ * posting (the signature-inline layout takes 8).  It is built when every
+ * slice has at least 8 positions, all below 20.  A new index uses it;
+ * issl_autotune times both layouts and keeps the faster (the 4-byte words pay
+ * where the streams are long enough to be bandwidth-bound); set overrides
+ * either.  Results are bit-identical either way.  get: *built, *in_use and
+ * the occurrence bits a residual word holds (counts at or above 2^bits - 1
+ * are read from the posting); each output may be NULL. */
+int issl_index_set_residual(issl_index *ix, int32_t use);
+int issl_index_get_residual(const issl_index *ix, int32_t *built, int32_t *in_use,
+                            uint32_t *occ_bits);
+
 /* Check on the device that every posting entry sits in the list of its own
  * key and that each slice lists every site once; *bad_entries receives the
  * number of violations.  (The reference trusts the file.) */
@@ -238,8 +255,9 @@ typedef struct {
 
 #define ISSL_HEAVY_NEVER 0x7FFFFFFF
 
-/* Time the BATCHED kernel's launch shapes on a sample of guides for this
- * index and keep the fastest (also written to the cache file when path is
+/* Time the BATCHED kernel's launch shapes, and the site-word layouts the
+ * index holds (issl_index_set_residual), on a sample of guides for this index
+ * and keep the fastest (also written to the cache file when path is
  * non-NULL).  No reference counterpart. */
 int issl_autotune(issl_index *ix, const uint64_t *sample_guides, uint64_t n,
                   const char *cache_path, issl_tuning *best);
