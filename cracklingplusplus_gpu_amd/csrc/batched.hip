@@ -2,8 +2,8 @@
 // tiles of each guide's slice-major candidate sequence.
 //
 // Pipeline (all on one stream, no memsets):
-//   k_plan     one lane per guide: slice keys -> list ranges, their
-//              prefix, the posting offsets, the tile length
+//   k_plan     one lane per (guide, slice): slice keys -> list ranges,
+//              their prefix, the posting offsets, the tile length
 //              C_g = max(chunk, ceil(P_g/64)) and the tile count K_g; the
 //              guide's tiles (slot v = k*n + g) are appended to a device
 //              tile queue, tiles of several-tile guides and long single
@@ -73,14 +73,19 @@ constexpr uint64_t kMaxBatch = 1ULL << 18;
 constexpr uint32_t kRegions = 32;
 constexpr uint32_t kSubQueues = 256;
 constexpr uint32_t kCtrStride = 16;
-constexpr uint32_t kSubQueueBase = 2 * kRegions;                // taken counters
+// Region counter i (i < 2 kRegions: first list's regions, then the second's)
+// sits at tctr[i * kCtrStride].
+constexpr uint32_t kSubQueueBase = 2 * kRegions * kCtrStride;   // taken counters
 constexpr uint32_t kSubQueueCtr = kCtrStride * kSubQueues;      // taken -> finished
 constexpr int kQueueCtrs = static_cast<int>(kSubQueueBase + 2 * kSubQueueCtr);
-// Planner blocks (64 guides each) per region; a region of the first list
-// holds their tiles (at most kMaxTiles per guide), of the second one tile
-// per guide.
-__host__ __device__ inline uint64_t region_blocks(uint64_t n) {
-    return ((n + 63) / 64 + kRegions - 1) / kRegions;
+// Guides a region takes at most: planner block b (kPlanWaves waves of 64 / S
+// guides each) appends to region b % kRegions; a region of the first list
+// holds their tiles (at most kMaxTiles per guide), of the second one tile per
+// guide.
+constexpr uint64_t kPlanWavesPerBlock = 4;
+__host__ __device__ inline uint64_t region_guides(uint64_t n, int S) {
+    const uint64_t G = kPlanWavesPerBlock * (64u / static_cast<uint64_t>(S > 0 ? S : 1));
+    return ((n + G - 1) / G + kRegions - 1) / kRegions * G;
 }
 
 struct BatchWs {
@@ -95,7 +100,7 @@ struct BatchWs {
     // of its guides with several tiles or a single tile longer than half a
     // chunk to region b % kRegions of the first list and of its other
     // single-tile guides to the same region of the second,
-    // counting them in tctr[r] and tctr[kRegions + r] (set back to zero by
+    // counting them in tctr[r * kCtrStride] and tctr[(kRegions + r) * kCtrStride] (set back to zero by
     // k_finalize).  The queue is the first list's regions in order, then the
     // second's.  The tile kernel splits it into sub-queues, each with a taken
     // and a finished counter (kSubQueueCtr apart), which its last wave sets
@@ -334,46 +339,78 @@ struct PlanArgs {
     BatchWs ws;
 };
 
-// One lane per guide.  The slice descriptors are staged in LDS; per group of
-// up to 16 slices every list's bounds are requested, then the prefix is formed
-// in registers and stored: no load waits behind another load of its group,
-// and no lane branches between them (lists outside the window, or of a guide
-// that took the exit, are loaded and then counted as empty).
-constexpr int kPlanBlock = 64;
-constexpr int kPlanGroup = 16;
+// One lane per (guide, slice): a wave plans G = 64 / S guides, each on S
+// consecutive lanes, and a workgroup kPlanWaves waves.  Every lane forms its
+// slice key and requests its list's bounds (and, for the list-residual words,
+// forms the guide's residual codes while they are in flight); the prefix over
+// a guide's slices is a shuffle scan within its lanes.  Lists outside the
+// window, or of a guide that took the exit, are loaded and then counted as
+// empty.
+constexpr int kPlanWaves = static_cast<int>(kPlanWavesPerBlock);
+constexpr int kPlanBlock = 64 * kPlanWaves;
 
-// A guide's tile count and posting count.
-struct TileCount {
-    uint32_t K, P;
-};
-
-__device__ TileCount plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const uint64_t *s_fold,
-                                const uint32_t *s_lb);
+__host__ __device__ inline uint32_t plan_guides_per_wave(int S) {
+    return 64u / static_cast<uint32_t>(S > 0 ? S : 1);
+}
 
 __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
-    __shared__ uint64_t s_fold[kMaxSlices];
-    __shared__ uint32_t s_lb[kMaxSlices];
-    const uint64_t g = blockIdx.x * static_cast<uint64_t>(kPlanBlock) + threadIdx.x;
-    const bool in = g < a.n;
-    const uint64_t q = a.guides[in ? g : 0] & kSigBits;   // n >= 1 when launched
+    __shared__ uint32_t s_tot[kPlanWaves][2];
+    __shared__ uint32_t s_base[2];
     const int S = a.S;
-    for (int i = threadIdx.x; i < S; i += kPlanBlock) {
-        s_fold[i] = a.meta[i].fold;
-        s_lb[i] = a.meta[i].lbase;
+    const int SS = S > 0 ? S : 1;
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const int G = static_cast<int>(plan_guides_per_wave(S));
+    const int gi = lane / SS, s = lane - gi * SS;
+    const uint64_t g = (blockIdx.x * static_cast<uint64_t>(kPlanWaves) + wave) * G + gi;
+    const bool in = gi < G && g < a.n;
+    const bool sl_in = in && s < S;
+    const uint64_t q = a.guides[in ? g : 0] & kSigBits;   // n >= 1 when launched
+    uint64_t fold = 0;
+    uint32_t b = 0, e = 0;
+    if (sl_in) {
+        fold = a.meta[s].fold;
+        const uint32_t lb = a.meta[s].lbase;
+        const uint32_t key = slice_key(q, fold);
+        b = a.lstart[lb + key];
+        e = a.lstart[lb + key + 1];
     }
-    __syncthreads();
-    TileCount tc{0, 0};
-    if (in) tc = plan_guide(a, g, q, s_fold, s_lb);
-    const uint32_t K = tc.K;
+    // (formed while the list bounds are in flight)
+    const uint32_t gr = a.ws.gres && sl_in ? residual_code(q, fold) | (static_cast<uint32_t>(s) << 24) : 0u;
+    const bool live = in && (!a.state_in || (a.state_in[g].sites & kExited) == 0);
+    const uint32_t len = (sl_in && live && s >= a.s0 && s < a.s1) ? e - b : 0u;
+    // Inclusive scan of the lengths over the guide's lanes (lane - off is the
+    // same guide's lane exactly when s >= off).
+    uint32_t x = len;
+    for (int off = 1; off < SS; off <<= 1) {
+        const uint32_t v = static_cast<uint32_t>(__shfl_up(static_cast<int>(x), off));
+        if (s >= off) x += v;
+    }
+    const uint32_t run = x - len;
+    const uint32_t P = static_cast<uint32_t>(__shfl(static_cast<int>(x), gi * SS + SS - 1));
+    if (sl_in) {
+        a.ws.pre[g * (S + 1) + s] = run;
+        // posting index of position p in slice s: (s - s0)*N + b + (p - pre[s])
+        a.ws.delta[g * S + s] = len ? static_cast<uint64_t>(s - a.s0) * a.n_sites + b - run : 0;
+        if (a.ws.gres) a.ws.gres[g * S + s] = gr;
+    }
+    uint64_t c = (static_cast<uint64_t>(P) + kMaxTiles - 1) / kMaxTiles;
+    c = (c + 63) / 64 * 64;
+    if (c < a.chunk) c = a.chunk;
+    const uint32_t K = static_cast<uint32_t>((P + c - 1) / c);
+    const bool lead = in && s == 0;
+    if (lead) {
+        a.ws.pre[g * (S + 1) + S] = P;
+        a.ws.gchunk[g] = static_cast<uint32_t>(c);
+        a.ws.gtiles[g] = K;
+    }
     if (a.ws.tlist) {
-        // Append the wave's tiles to the queue: one atomic per list per wave,
-        // on the counter of the block's region.
-        const int lane = static_cast<int>(threadIdx.x);
-        // The first list takes the tiles likely to cost most (several-tile
-        // guides, and single tiles longer than half a chunk), so that they
-        // start early and the cheap ones fill the end.
-        const bool first = K > 1 || (K == 1 && tc.P > a.chunk / 2);
-        const uint32_t ka = first ? K : 0, kb = (!first && K == 1) ? 1 : 0;
+        // Append the workgroup's tiles to the queue: one atomic per list per
+        // workgroup, on the counter of its region.  The first list takes the
+        // tiles likely to cost most (several-tile guides, and single tiles
+        // longer than half a chunk), so that they start early and the cheap
+        // ones fill the end.
+        const bool first = K > 1 || (K == 1 && P > a.chunk / 2);
+        const uint32_t ka = (lead && first) ? K : 0, kb = (lead && !first && K == 1) ? 1 : 0;
         uint32_t ia = ka, ib = kb;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -384,87 +421,37 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
                 ib += vb;
             }
         }
-        const uint32_t ta = static_cast<uint32_t>(__shfl(static_cast<int>(ia), 63));
-        const uint32_t tb = static_cast<uint32_t>(__shfl(static_cast<int>(ib), 63));
-        const uint32_t r = blockIdx.x % kRegions;
-        uint32_t base_a = 0, base_b = 0;
-        if (lane == 0) {
-            if (ta) base_a = atomicAdd(a.ws.tctr + r, ta);
-            if (tb) base_b = atomicAdd(a.ws.tctr + kRegions + r, tb);
+        if (lane == 63) {
+            s_tot[wave][0] = ia;
+            s_tot[wave][1] = ib;
         }
-        base_a = static_cast<uint32_t>(__shfl(static_cast<int>(base_a), 0)) + ia - ka;
-        base_b = static_cast<uint32_t>(__shfl(static_cast<int>(base_b), 0)) + ib - kb;
-        const uint64_t rb = region_blocks(a.n) * 64;
-        uint32_t *const la = a.ws.tlist + r * rb * kMaxTiles;
-        uint32_t *const lb = a.ws.tlist + kRegions * rb * kMaxTiles + r * rb;
+        __syncthreads();
+        const uint32_t r = blockIdx.x % kRegions;
+        if (threadIdx.x == 0) {
+            uint32_t ta = 0, tb = 0;
+            for (int w = 0; w < kPlanWaves; ++w) {
+                ta += s_tot[w][0];
+                tb += s_tot[w][1];
+            }
+            s_base[0] = ta ? atomicAdd(a.ws.tctr + r * kCtrStride, ta) : 0u;
+            s_base[1] = tb ? atomicAdd(a.ws.tctr + (kRegions + r) * kCtrStride, tb) : 0u;
+        }
+        __syncthreads();
+        uint32_t base_a = s_base[0] + ia - ka, base_b = s_base[1] + ib - kb;
+        for (int w = 0; w < wave; ++w) {
+            base_a += s_tot[w][0];
+            base_b += s_tot[w][1];
+        }
+        const uint64_t rg = region_guides(a.n, S);
+        uint32_t *const la = a.ws.tlist + r * rg * kMaxTiles;
+        uint32_t *const lb = a.ws.tlist + kRegions * rg * kMaxTiles + r * rg;
         const uint32_t n32 = static_cast<uint32_t>(a.n);
         // A region holds all its blocks' tiles; the bound only guards the
         // list against counters that were not zero.
         for (uint32_t k = 0; k < ka; ++k)
-            if (base_a + k < rb * kMaxTiles) la[base_a + k] = k * n32 + static_cast<uint32_t>(g);
-        if (kb && base_b < rb) lb[base_b] = static_cast<uint32_t>(g);
+            if (base_a + k < rg * kMaxTiles) la[base_a + k] = k * n32 + static_cast<uint32_t>(g);
+        if (kb && base_b < rg) lb[base_b] = static_cast<uint32_t>(g);
     }
-}
-
-// The plan of guide g (k_plan); returns its tile and posting counts.
-__device__ TileCount plan_guide(const PlanArgs &a, uint64_t g, uint64_t q, const uint64_t *s_fold,
-                                const uint32_t *s_lb) {
-    const int S = a.S;
-    const bool live = !a.state_in || (a.state_in[g].sites & kExited) == 0;
-    uint32_t *pre = a.ws.pre + g * (S + 1);
-    uint64_t *delta = a.ws.delta + g * S;
-    uint64_t run = 0;
-    for (int s0 = 0; s0 < S; s0 += kPlanGroup) {
-        uint64_t fold[kPlanGroup];
-        uint32_t lb[kPlanGroup], b[kPlanGroup], e[kPlanGroup];
-#pragma unroll
-        for (int i = 0; i < kPlanGroup; ++i) {
-            fold[i] = 0;
-            lb[i] = 0;
-            if (s0 + i < S) {
-                fold[i] = s_fold[s0 + i];
-                lb[i] = s_lb[s0 + i];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < kPlanGroup; ++i) {
-            b[i] = 0;
-            e[i] = 0;
-            if (s0 + i < S) {
-                const uint32_t key = slice_key(q, fold[i]);
-                b[i] = a.lstart[lb[i] + key];
-                e[i] = a.lstart[lb[i] + key + 1];
-            }
-        }
-        // (formed while the list bounds are in flight)
-        if (a.ws.gres) {
-#pragma unroll
-            for (int i = 0; i < kPlanGroup; ++i)
-                if (s0 + i < S)
-                    a.ws.gres[g * S + s0 + i] =
-                        residual_code(q, fold[i]) | (static_cast<uint32_t>(s0 + i) << 24);
-        }
-#pragma unroll
-        for (int i = 0; i < kPlanGroup; ++i) {
-            const int s = s0 + i;
-            if (s < S) {
-                const bool ok = live && s >= a.s0 && s < a.s1;
-                const uint32_t len = ok ? e[i] - b[i] : 0;
-                pre[s] = static_cast<uint32_t>(run);
-                // posting index of position p in slice s: (s - s0)*N + b + (p - pre[s])
-                delta[s] = len ? static_cast<uint64_t>(s - a.s0) * a.n_sites + b[i] - run : 0;
-                run += len;
-            }
-        }
-    }
-    pre[S] = static_cast<uint32_t>(run);
-    uint64_t c = (run + kMaxTiles - 1) / kMaxTiles;
-    c = (c + 63) / 64 * 64;
-    if (c < a.chunk) c = a.chunk;
-    const uint32_t K = static_cast<uint32_t>((run + c - 1) / c);
-    a.ws.gchunk[g] = static_cast<uint32_t>(c);
-    a.ws.gtiles[g] = K;
-    return TileCount{K, static_cast<uint32_t>(run)};
 }
 
 // Final per-guide results: transform 10000/(100+S), the call and the output
@@ -612,12 +599,12 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     uint32_t pre_a = 0, pre_b = 0, tot_a = 0, tot_b = 0;
     uint64_t rb = 0;
     if constexpr (QUEUE) {
-        rb = region_blocks(n) * 64;
+        rb = region_guides(n, a.S);
         // (the bounds only guard the list against counters that were not zero)
         const uint32_t ca = lane < static_cast<int>(kRegions)
-                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[lane], rb * kMaxTiles)) : 0;
+                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[lane * kCtrStride], rb * kMaxTiles)) : 0;
         const uint32_t cb = lane < static_cast<int>(kRegions)
-                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[kRegions + lane], rb)) : 0;
+                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[(kRegions + lane) * kCtrStride], rb)) : 0;
         uint32_t ia = ca, ib = cb;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -1060,7 +1047,7 @@ struct FinArgs {
 // The tile queue's counters back to zero for the next launch sequence (the
 // tile kernel has finished with them).
 __device__ __forceinline__ void reset_queue(const BatchWs &ws) {
-    if (ws.tctr && blockIdx.x == 0 && threadIdx.x < 2 * kRegions) ws.tctr[threadIdx.x] = 0;
+    if (ws.tctr && blockIdx.x == 0 && threadIdx.x < 2 * kRegions) ws.tctr[threadIdx.x * kCtrStride] = 0;
 }
 
 __global__ void k_zero_queue(uint32_t *tctr) {
@@ -1551,7 +1538,7 @@ int carve(Workspace &ws, uint64_t n, int S, bool res, BatchWs *b, bool *fresh) {
     const size_t sz_delta = align256(n * std::max(S, 1) * 8);
     const size_t sz_u32 = align256(n * 4);
     const size_t sz_part = align256(n * kMaxTiles * sizeof(TilePart));
-    const size_t sz_list = align256(kRegions * region_blocks(n) * 64 * (kMaxTiles + 1) * 4);
+    const size_t sz_list = align256(kRegions * region_guides(n, S) * (kMaxTiles + 1) * 4);
     const size_t sz_ctr = align256(kQueueCtrs * 4);
     const void *old = ws.tiles.p;
     int rc = ws.tiles.reserve(sz_ctr + sz_pre + sz_delta + 2 * sz_u32 + sz_part + sz_list + sz_gres);
@@ -1701,7 +1688,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     } else {
         PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk),
                     win.s0, win.s1, win.state_in, b};
-        hipLaunchKernelGGL(k_plan, dim3(static_cast<uint32_t>((n + kPlanBlock - 1) / kPlanBlock)),
+        const uint64_t G = kPlanWaves * plan_guides_per_wave(S);
+        hipLaunchKernelGGL(k_plan, dim3(static_cast<uint32_t>((n + G - 1) / G)),
                            dim3(kPlanBlock), 0, st, pa);
     }
 
