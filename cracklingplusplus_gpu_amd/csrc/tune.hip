@@ -161,9 +161,12 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     const int chunks[] = {1024, 2048, 4096};
     const int depths[] = {4, 8, 16};
     const int grids[] = {4, 6, 8};
-    issl_tuning tiles_best{};
-    tiles_best.us_per_guide = 1e300;
-    bool res_best = false;
+    // The list-residual words are kept only when they beat the signature-inline
+    // words (which the other kernels read too) by more than kResMargin, above
+    // the trial-to-trial spread of one to two percent.
+    constexpr double kResMargin = 0.97;
+    issl_tuning layout_best[2];
+    layout_best[0].us_per_guide = layout_best[1].us_per_guide = 1e300;
     for (int layout = ix->d_res ? 1 : 0; layout >= 0; --layout) {
         ix->res_use = layout != 0;
         for (int c : chunks)
@@ -171,13 +174,11 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
                 for (int gpc : grids) {
                     issl_tuning t{c, 256, d, gpc, 8, 0, 0};
                     if ((rc = measure(t)) != ISSL_OK) return rc;
-                    if (t.us_per_guide < tiles_best.us_per_guide) {
-                        tiles_best = t;
-                        res_best = ix->res_use;
-                    }
+                    if (t.us_per_guide < layout_best[layout].us_per_guide) layout_best[layout] = t;
                 }
     }
-    ix->res_use = res_best;
+    ix->res_use = layout_best[1].us_per_guide < kResMargin * layout_best[0].us_per_guide;
+    const issl_tuning tiles_best = layout_best[ix->res_use ? 1 : 0];
     if (tiles_best.us_per_guide < top.us_per_guide) top = tiles_best;
     // (2) one wave per guide, optionally sending long guides to the tile kernel
     const int ddepths[] = {4, 8, 16};
