@@ -261,3 +261,32 @@ def test_timing_repeat_keeps_results(cases, form):
         assert (bits(r.cfd) == bits(ref.cfd)).all()
         assert (bits(r.mit) == bits(ref.mit)).all()
         assert (r.site_count == ref.site_count).all()
+
+
+@pytest.mark.parametrize("name", ["mid50k_10x10", "small200k_8x8", "uniform1m_5x4"])
+def test_one_tile_launches(cases, name):
+    """A chunk that covers the longest candidate sequence gives every guide one
+    tile (finished by the wave that scores it), short chunks give many; the
+    shapes and the early exit alternate, so each launch sequence must leave
+    the tile queue's counters usable for the next."""
+    hx, ix, gd = cases[name]
+    g = gd["guides"]
+    o0 = orc.score(hx, g, issl.METHOD_MIT_AND_CFD, 0.0)
+    o75 = orc.score(hx, g, issl.METHOD_MIT_AND_CFD, 75.0)
+    try:
+        for step, (chunk, t) in enumerate([(1 << 20, 0.0), (1 << 20, 0.0), (1024, 0.0), (1 << 20, 0.0),
+                                           (1 << 20, 75.0), (1 << 20, 0.0), (64, 0.0), (1 << 20, 0.0)]):
+            ix.set_tuning(chunk=chunk, depth=8, grid_per_cu=4, heavy_kpost=0)
+            r = ix.score(g, method=issl.METHOD_MIT_AND_CFD, threshold=t, form=issl.FORM_BATCHED,
+                         precision=issl.PREC_FP32C)
+            o = o0 if t == 0.0 else o75
+            what = f"{name} step {step} chunk {chunk} t {t}"
+            if t == 0.0:
+                assert (r.site_count == o["site_count"]).all(), what
+                assert (r.occ_count == o["occ_count"]).all(), what
+                np.testing.assert_allclose(r.cfd, o["cfd"], rtol=1e-6, err_msg=what)
+                np.testing.assert_allclose(r.mit, o["mit"], rtol=1e-6, err_msg=what)
+            key = f"m{issl.METHOD_MIT_AND_CFD}_t{int(t)}"
+            assert (r.passed == gd[key + "_pass"]).all(), what
+    finally:
+        ix.set_tuning()
