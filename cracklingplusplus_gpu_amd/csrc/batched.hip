@@ -529,7 +529,12 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __shared__ uint64_t s_posmask[kNoPos + 1];
     __shared__ uint32_t s_rank[3][RES ? 128 : kRankTerms];
     // RES: slice s's residual positions, ascending (the code field's order).
-    __shared__ uint8_t s_rabs[RES ? kMaxSlices : 1][kResPositions];
+    // (at most kResMaxCodes: the layout is built when every slice has
+    // kResMinWidth or more positions)
+    __shared__ uint8_t s_rabs[RES ? kMaxSlices : 1][kResMaxCodes];
+    // RES: the slices holding each of those positions (posmask by residual
+    // index), read beside the position so stage A needs one LDS round trip.
+    __shared__ uint64_t s_rpm[RES ? kMaxSlices : 1][kResMaxCodes];
     // The current tile's CFD factors by (position, guide ^ site there).
     __shared__ Real s_cq[kTileWaves][(kNoPos + 1) * 4];
     // Tile plans, double-buffered: the next tile's plan lands in the other half.
@@ -566,8 +571,9 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             const uint64_t f = a.meta[tid].fold;
             uint32_t j = 0;
             for (uint32_t p = 0; p < static_cast<uint32_t>(kResPositions); ++p)
-                if (!((f >> (2 * p)) & 1ULL)) s_rabs[tid][j++] = static_cast<uint8_t>(p);
-            for (; j < static_cast<uint32_t>(kResPositions); ++j) s_rabs[tid][j] = static_cast<uint8_t>(kNoPos);
+                if (!((f >> (2 * p)) & 1ULL) && j < static_cast<uint32_t>(kResMaxCodes))
+                    s_rabs[tid][j++] = static_cast<uint8_t>(p);
+            for (; j < static_cast<uint32_t>(kResMaxCodes); ++j) s_rabs[tid][j] = static_cast<uint8_t>(kNoPos);
         }
     } else {
         build_rank_terms(s_rank, tid, kTileBlock);
@@ -575,6 +581,13 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __syncthreads();
     build_posmask(s_posmask, s_fold, a.S, tid);
     __syncthreads();
+    if constexpr (RES) {
+        for (int i = tid; i < a.S * kResMaxCodes; i += kTileBlock) {
+            const uint32_t p = s_rabs[i / kResMaxCodes][i % kResMaxCodes];
+            s_rpm[i / kResMaxCodes][i % kResMaxCodes] = p < kNoPos ? s_posmask[p] : 0ULL;
+        }
+        __syncthreads();
+    }
     // RES: the code field's fold bits (bit 2j per residual position j) and the
     // occurrence escape.
     const uint32_t res_fold = RES ? (0x55555555u & ((1u << a.res_shift) - 1u)) : 0u;
@@ -849,15 +862,20 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     const uint32_t x = wv;
                     uint32_t f = ((x >> 1) | x) & res_fold;
                     const uint32_t dist = static_cast<uint32_t>(__popc(f));
-                    uint32_t mp[4], ix[4];
+                    uint32_t ix[4];
+                    uint64_t nm = 0;   // slices holding a mismatch position
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const uint32_t b2 = f ? static_cast<uint32_t>(__builtin_ctz(f)) : 0u;
-                        mp[i] = f ? static_cast<uint32_t>(s_rabs[sl][b2 >> 1]) : kNoPos;
-                        ix[i] = f ? ((mp[i] << 2) | ((x >> b2) & 3u)) : kAbsent7;
+                        const uint32_t p = s_rabs[sl][b2 >> 1];
+                        const uint64_t pm = s_rpm[sl][b2 >> 1];
+                        ix[i] = f ? ((p << 2) | ((x >> b2) & 3u)) : kAbsent7;
+                        nm |= f ? pm : 0ULL;
                         f &= f - 1;
                     }
-                    hit = first_seen(mp, sl, s_posmask);
+                    // the first-seen rule (first_seen) on those slices
+                    const uint64_t earlier = sl ? (~0ULL >> (64 - sl)) : 0ULL;
+                    hit = (earlier & ~nm) == 0;
                     pk = ix[0] | (ix[1] << 7) | (ix[2] << 14) | (ix[3] << 21) | (dist << 28);
                     // the hit ring keeps the occurrence field
                     wv = (x ^ (static_cast<uint32_t>(sl) << 24)) >> a.res_shift;

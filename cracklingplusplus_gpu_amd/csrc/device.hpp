@@ -65,6 +65,7 @@ constexpr uint32_t kOccEscape = 0xFFFFFF;
 // occurrence bits).
 constexpr int kResMinWidth = 8;
 constexpr int kResPositions = 20;
+constexpr int kResMaxCodes = kResPositions - kResMinWidth;   // residual positions per slice
 
 // The codes of `sig` at the positions 0..19 outside `fold` (bit 2j per slice
 // position j), ascending, packed from bit 0: a list-residual word's code

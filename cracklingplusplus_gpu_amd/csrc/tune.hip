@@ -21,7 +21,9 @@ using namespace issl;
 
 namespace {
 
-std::string shape_key(const issl_index *ix) {
+// The pick depends on the index's shape and on the batch: the sample's size
+// enters the key by its power of two.
+std::string shape_key(const issl_index *ix, uint64_t n_sample) {
     hipDeviceProp_t prop;
     std::string arch = "unknown";
     int cus = 0;
@@ -32,8 +34,11 @@ std::string shape_key(const issl_index *ix) {
     for (char &c : arch)
         if (c == ' ' || c == ':') c = '_';
     std::ostringstream k;
-    k << "t3/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
+    k << "t4/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
     for (const SliceMeta &m : ix->meta_h) k << m.width << ',';
+    int q = 0;
+    while ((n_sample >> q) > 1) ++q;
+    k << "/q" << q;
     return k.str();
 }
 
@@ -88,7 +93,7 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     if (!ix || !best) return fail(ISSL_ERR_ARG, "null argument");
     if (n == 0 || !sample) return fail(ISSL_ERR_ARG, "autotune needs sample guides");
     if (ix->src) return fail(ISSL_ERR_ARG, "autotune needs a resident index");
-    const std::string key = shape_key(ix);
+    const std::string key = shape_key(ix, n);
     issl_tuning cached{};
     int cached_res = 0;
     if (cache_lookup(cache_path, key, &cached, &cached_res)) {
