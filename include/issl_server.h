@@ -100,7 +100,9 @@ int issl_server_get_stats(issl_server *srv, int reset, issl_server_stats *out);
 /* Free and total HBM of a device (hipMemGetInfo). */
 int issl_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 
-/* HBM an index of this shape holds once resident (scoring workspace aside). */
+/* HBM an index of this shape holds once resident (scoring workspace aside):
+ * exact when every slice has 8 or more positions, else an upper bound (the
+ * 4-byte list-residual words are then not built). */
 uint64_t issl_index_bytes_needed(uint64_t n_sites, uint64_t n_slices, uint64_t n_lists);
 
 #ifdef __cplusplus
