@@ -189,7 +189,10 @@ int issl_index_get_info(const issl_index *ix, issl_index_info *out);
  * slice has at least 8 positions, all below 20.  A new index uses it;
  * issl_autotune times both layouts and keeps the faster (the 4-byte words pay
  * where the streams are long enough to be bandwidth-bound); set overrides
- * either.  Results are bit-identical either way.  get: *built, *in_use and
+ * either.  Results are bit-identical either way for an index whose sites each
+ * sit in the list of their own key (issl_index_validate); for one that does
+ * not, only the signature-inline words (use = 0) take the distance on the
+ * whole signature as the reference does.  get: *built, *in_use and
  * the occurrence bits a residual word holds (counts at or above 2^bits - 1
  * are read from the posting); each output may be NULL. */
 int issl_index_set_residual(issl_index *ix, int32_t use);
