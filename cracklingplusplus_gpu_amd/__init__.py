@@ -302,8 +302,9 @@ class Index:
 
     def set_residual(self, use: bool) -> None:
         """Stream the list-residual words (4 B per posting) in the BATCHED tile
-        kernel when the index has them (the default), or the signature-inline
-        words (8 B).  Results are the same bits either way."""
+        kernel when the index has them, or the signature-inline words (8 B, a
+        new index's choice; the autotuner picks the faster).  Results are the
+        same bits either way."""
         check(lib().issl_index_set_residual(self._h, 1 if use else 0))
 
     def residual(self) -> tuple:

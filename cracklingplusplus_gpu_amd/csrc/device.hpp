@@ -188,7 +188,7 @@ struct issl_index {
     uint64_t *d_inline = nullptr;   // [S*n] sig | occ<<40 (same order)
     uint32_t *d_res = nullptr;      // [S*n] list-residual words (same order), or null
     uint32_t res_shift = 0;         // first occurrence bit of a residual word
-    bool res_use = true;            // BATCHED streams d_res when it is built
+    bool res_use = false;           // BATCHED streams d_res (when built): autotuner or caller
     uint32_t *d_lstart = nullptr;   // per slice: 4^w + 1 list starts
     issl::SliceMeta *d_meta = nullptr;
     double *d_mit64 = nullptr;      // [6195] by combinadic rank

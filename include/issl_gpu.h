@@ -186,8 +186,9 @@ int issl_index_get_info(const issl_index *ix, issl_index_info *out);
  * only the other 20 - w positions can differ: the list-residual layout keeps
  * those positions' codes and a clipped occurrence count in 4 bytes per
  * posting (the signature-inline layout takes 8).  It is built when every
- * slice has at least 8 positions, all below 20.  A new index uses it;
- * issl_autotune times both layouts and keeps the faster (the 4-byte words pay
+ * slice has at least 8 positions, all below 20.  A new index streams the
+ * signature-inline words; issl_autotune times both layouts and keeps the
+ * faster (the 4-byte words pay
  * where the streams are long enough to be bandwidth-bound); set overrides
  * either.  Results are bit-identical either way for an index whose sites each
  * sit in the list of their own key (issl_index_validate); for one that does

@@ -60,7 +60,7 @@ def both_layouts(ix, g, **kw):
         ix.set_residual(False)
         inl = ix.score(g, form=issl.FORM_BATCHED, **kw)
     finally:
-        ix.set_residual(True)
+        ix.set_residual(False)
     return res, inl
 
 
@@ -69,11 +69,11 @@ def test_residual_built_where_it_fits(cases, name):
     _, ix, _ = cases[name]
     built, used, occ_bits = ix.residual()
     assert (built, occ_bits) == EXPECT[name]
-    assert used == built
-    ix.set_residual(False)
-    assert ix.residual()[1] is False
+    assert used is False          # a new index streams the signature-inline words
     ix.set_residual(True)
     assert ix.residual()[1] == built
+    ix.set_residual(False)
+    assert ix.residual()[1] is False
 
 
 @pytest.mark.parametrize("precision", [issl.PREC_FP64, issl.PREC_FP32C])
@@ -141,4 +141,4 @@ def test_residual_occurrence_escape(gpu_available, name):
             ix.distinct_reset()
             ix.score(g, form=issl.FORM_BATCHED, flags=issl.FLAG_DISTINCT)
             assert ix.distinct_count() == want
-        ix.set_residual(True)
+        ix.set_residual(False)
