@@ -1670,9 +1670,14 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     const bool tiles = !direct || tn.heavy_kpost != kHeavyNever;
     const uint32_t gw = static_cast<uint32_t>((n + 3) / 4);
 
+    StreamArgs da{};
+    uint32_t sg = 0;
+    auto stream_launch = [&](hipEvent_t e0, hipEvent_t e1) {
+        return fp32 ? launch_direct_depth<ISSL_PREC_FP32C>(da, tn.direct_depth, sg, st, e0, e1)
+                    : launch_direct_depth<ISSL_PREC_FP64>(da, tn.direct_depth, sg, st, e0, e1);
+    };
     if (direct) {
         b.tlist = nullptr;   // guide streams write plans without the queue: slot schedule
-        StreamArgs da;
         da.guides = d_guides;
         da.n = n;
         da.site = win.inl;   // slice s at s * N: the window is the whole index
@@ -1692,17 +1697,14 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
         // Persistent guide streams: grid_per_cu workgroups per CU, at most one wave per guide.
         const uint64_t sgrid = std::max<uint64_t>(
             1, std::min<uint64_t>(gw, static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu));
-        const uint32_t sg = static_cast<uint32_t>(sgrid);
-        rc = timed_dispatch(ix, st, [&](hipEvent_t e0, hipEvent_t e1) {
-            return fp32 ? launch_direct_depth<ISSL_PREC_FP32C>(da, tn.direct_depth, sg, st, e0, e1)
-                        : launch_direct_depth<ISSL_PREC_FP64>(da, tn.direct_depth, sg, st, e0, e1);
-        });
-        if (rc != ISSL_OK) return rc;
+        sg = static_cast<uint32_t>(sgrid);
         if (!tiles) {
+            if ((rc = timed_dispatch(ix, st, stream_launch)) != ISSL_OK) return rc;
             ISSL_HIP(hipGetLastError());
             ws.queue_dirty = false;   // no planner ran: the counters are untouched
             return ISSL_OK;
         }
+        // (launched below with the tile kernel, the two timed together)
     } else {
         PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk),
                     win.s0, win.s1, win.state_in, b};
@@ -1737,7 +1739,15 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
         return fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st, e0, e1)
                     : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st, e0, e1);
     };
-    rc = direct ? tiles_launch(nullptr, nullptr) : timed_dispatch(ix, st, tiles_launch);
+    if (direct) {
+        // Long guides go to the tile kernel: one event pair spans both kernels.
+        rc = timed_dispatch(ix, st, [&](hipEvent_t e0, hipEvent_t e1) {
+            const int r0 = stream_launch(e0, nullptr);
+            return r0 != ISSL_OK ? r0 : tiles_launch(nullptr, e1);
+        });
+    } else {
+        rc = timed_dispatch(ix, st, tiles_launch);
+    }
     if (rc != ISSL_OK) return rc;
 
     FinArgs fa;

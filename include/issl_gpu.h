@@ -274,7 +274,8 @@ int issl_set_tuning(issl_index *ix, const issl_tuning *in);
  * phase timers, ISSLScoreOfftargets.cpp:185-188,296,507-510).  While enabled,
  * a pair of HIP events is attached to the dispatch of the dominant kernel of
  * every scoring launch on this index (k_plain for PLAIN, k_tiles or k_stream
- * for BATCHED; hipExtLaunchKernelGGL), so they time that kernel alone.
+ * for BATCHED, or k_stream and k_tiles together when long guides go to the
+ * tile kernel; hipExtLaunchKernelGGL), so they time that kernel alone.
  * enable = N > 1 dispatches that kernel N times back to back (it is
  * idempotent; results are unchanged) between one recorded event pair, which
  * spreads the events' own overhead over N dispatches.  Enabling clears
