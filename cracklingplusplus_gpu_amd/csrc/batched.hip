@@ -96,15 +96,15 @@ struct BatchWs {
     // [n][S] the guide's residual codes for the list of slice s (list-residual
     // layout), with s in bits 24..29; null: the signature-inline layout.
     uint32_t *gres;
-    // Tile queue (null: the slot schedule).  Planner block b appends the slots
-    // of its guides with several tiles or a single tile longer than half a
-    // chunk to region b % kRegions of the first list and of its other
-    // single-tile guides to the same region of the second,
-    // counting them in tctr[r * kCtrStride] and tctr[(kRegions + r) * kCtrStride] (set back to zero by
-    // k_finalize).  The queue is the first list's regions in order, then the
-    // second's.  The tile kernel splits it into sub-queues, each with a taken
-    // and a finished counter (kSubQueueCtr apart), which its last wave sets
-    // back to zero.
+    // Tile queue (null: the slot schedule).  Planner workgroup b appends the
+    // slots of its guides with several tiles or a single tile longer than half
+    // a chunk to region b % kRegions of the first list and of its other
+    // single-tile guides to the same region of the second, counting them in
+    // tctr[r * kCtrStride] and tctr[(kRegions + r) * kCtrStride] (set back to
+    // zero by k_finalize).  The queue is the first list's regions in order,
+    // then the second's.  The tile kernel splits it into sub-queues, each with
+    // a taken and a finished counter (kSubQueueCtr apart), which its last wave
+    // sets back to zero.
     uint32_t *tlist;
     uint32_t *tctr;
     TilePart *part;      // [kMaxTiles][n] partial of tile (g, k) at slot k*n + g
