@@ -177,12 +177,14 @@ def score_tables():
 
 
 def synth(cfg: dict, threads: int = 0):
-    """Generate a seeded workload: (sigs, occ, masks, guides, kinds)."""
+    """Generate a seeded workload: (sigs, occ, masks, guides, kinds).  With
+    guide_batches = B > 1, guides holds B disjoint batches of the stated mix,
+    one after another (batch 0 is the single-batch workload)."""
     c = _capi.issl_synth_cfg()
     for key, val in cfg.items():
         setattr(c, key, val)
     q = (c.n_guides_site + c.n_guides_unique + c.n_guides_planted + c.n_guides_repeat
-         + c.n_guides_random)
+         + c.n_guides_random) * max(1, c.guide_batches)
     sigs = np.zeros(c.n_sites, dtype=np.uint64)
     occ = np.zeros(c.n_sites, dtype=np.uint32)
     masks = np.zeros(c.n_slices, dtype=np.uint64)

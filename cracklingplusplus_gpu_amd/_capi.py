@@ -64,7 +64,7 @@ class issl_synth_cfg(Structure):
                 ("n_repeat_seeds", c_uint64), ("repeat_sub_p", c_double),
                 ("occ_p_unique", c_double), ("occ_p_repeat", c_double),
                 ("n_slices", c_uint64), ("slice_width", c_uint64),
-                ("contiguous_masks", c_int32), ("pad", c_int32),
+                ("contiguous_masks", c_int32), ("guide_batches", c_int32),
                 ("n_guides_site", c_uint64), ("n_guides_unique", c_uint64),
                 ("n_guides_planted", c_uint64), ("n_guides_repeat", c_uint64),
                 ("n_guides_random", c_uint64), ("seed", c_uint64)]

@@ -38,6 +38,8 @@ int mask_positions(uint64_t mask, uint64_t seq_len, uint8_t *pos_out);
 // Number of posting entries whose id is >= n_sites (host, OpenMP).  The
 // device kernels gather sig[id], so an index is only uploaded when this is 0.
 uint64_t count_bad_ids(const uint64_t *entries, uint64_t count, uint64_t n_sites);
+// Signatures with bits at or above bit 40 (more than 20 positions).
+uint64_t count_wide_sigs(const uint64_t *sigs, uint64_t n);
 
 const char *last_error_cstr();
 

@@ -756,10 +756,20 @@ int build_lstart(issl_index *ix, const std::vector<const uint64_t *> &sizes,
     return ISSL_OK;
 }
 
-int check_index_args(uint64_t n_sites, uint64_t n_slices, int device) {
+int check_index_args(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices, int device) {
     if (n_sites >= (1ULL << 32)) return fail(ISSL_ERR_FORMAT, "ids are 32-bit: N must be < 2^32");
     if (n_slices > static_cast<uint64_t>(kMaxSlices))
         return fail(ISSL_ERR_ARG, "at most 64 slices are supported");
+    // The scorer encodes guides with 20 positions and takes the distance on
+    // the whole XOR of guide and site signatures
+    // (lib/ISSLScoringModule.cpp:219,286-290; ISSLScoreOfftargets.cpp:360-366),
+    // so a site with more positions differs from every guide wherever those
+    // extra codes are not A, and its MIT lookup fails (ISSLScoringModule.cpp:296).
+    // The kernels take the distance on 20 positions; an index of any other
+    // length is refused rather than scored differently.
+    if (seq_len != 20)
+        return fail(ISSL_ERR_FORMAT, "index sequence length is " + std::to_string(seq_len) +
+                                         ": the scorer takes 20-position signatures only");
     const int ndev = issl_device_count();
     if (ndev <= 0) return fail(ISSL_ERR_DEVICE, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(ISSL_ERR_ARG, "device ordinal out of range");
@@ -774,11 +784,14 @@ extern "C" int issl_index_from_arrays(uint64_t n_sites, uint64_t seq_len, uint64
                                       int device, issl_index **out) {
     if (!out) return fail(ISSL_ERR_ARG, "null output handle");
     *out = nullptr;
-    int rc = check_index_args(n_sites, n_slices, device);
-    if (rc != ISSL_OK) return rc;
     if ((n_sites && !sigs) || (n_slices && (!slice_masks || !list_sizes)) ||
         (n_sites && n_slices && !entries))
         return fail(ISSL_ERR_ARG, "null index array");
+    // The file's own consistency first, then the device.
+    if (seq_len == 20 && n_sites < (1ULL << 32) && count_wide_sigs(sigs, n_sites) != 0)
+        return fail(ISSL_ERR_FORMAT, "signatures hold codes beyond position 19");
+    int rc = check_index_args(n_sites, seq_len, n_slices, device);
+    if (rc != ISSL_OK) return rc;
     std::unique_ptr<issl_index, void (*)(issl_index *)> ix(new (std::nothrow) issl_index, free_index);
     if (!ix) return fail(ISSL_ERR_NOMEM, "host allocation failed");
     ix->device = device;
@@ -868,12 +881,14 @@ int index_build(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices, const uin
                 issl_index **out) {
     if (!out) return fail(ISSL_ERR_ARG, "null output handle");
     *out = nullptr;
-    int rc = check_index_args(n_sites, n_slices, device);
-    if (rc != ISSL_OK) return rc;
     if ((n_sites && (!sigs || !occ)) || (n_slices && !slice_masks))
         return fail(ISSL_ERR_ARG, "null index array");
     if (n_sites > 0x7FFFFFFFULL)
         return fail(ISSL_ERR_ARG, "device build takes at most 2^31 - 1 sites (build on the host)");
+    if (!on_device && seq_len == 20 && count_wide_sigs(sigs, n_sites) != 0)
+        return fail(ISSL_ERR_FORMAT, "signatures hold codes beyond position 19");
+    int rc = check_index_args(n_sites, seq_len, n_slices, device);
+    if (rc != ISSL_OK) return rc;
     std::unique_ptr<issl_index, void (*)(issl_index *)> ix(new (std::nothrow) issl_index, free_index);
     if (!ix) return fail(ISSL_ERR_NOMEM, "host allocation failed");
     ix->device = device;

@@ -291,10 +291,11 @@ struct DeviceGuard {
 };
 
 // Index construction steps shared by the resident and streamed forms
-// (device.hip): argument checks, slice metadata from the masks, list starts
+// (device.hip): argument checks (the index's own shape first: N, slice
+// count and the 20-position sequence length; then the device), slice metadata from the masks, list starts
 // from the list sizes, and the device arrays (with or without the S*N
 // posting arrays), plus a staged host-to-device copy and the destructor.
-int check_index_args(uint64_t n_sites, uint64_t n_slices, int device);
+int check_index_args(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices, int device);
 int init_meta(issl_index *ix, uint64_t n_sites, uint64_t seq_len, uint64_t n_slices,
               const uint64_t *slice_masks, uint64_t *lstart_count);
 int build_lstart(issl_index *ix, const std::vector<const uint64_t *> &sizes,

@@ -47,6 +47,13 @@ uint64_t issl::count_bad_ids(const uint64_t *entries, uint64_t count, uint64_t n
     return bad;
 }
 
+uint64_t issl::count_wide_sigs(const uint64_t *sigs, uint64_t n) {
+    uint64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) bad += (sigs[i] >> 40) != 0 ? 1 : 0;
+    return bad;
+}
+
 extern "C" uint64_t issl_list_count(const uint64_t *slice_masks, uint64_t n_slices,
                                     uint64_t seq_len) {
     uint64_t total = 0;

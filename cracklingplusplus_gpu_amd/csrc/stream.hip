@@ -307,7 +307,11 @@ int make_streamed(uint64_t n_sites, uint64_t seq_len, uint64_t n_slices, const u
                   const uint64_t *slice_masks, const std::vector<const uint64_t *> &sizes,
                   const std::vector<const uint64_t *> &entries, int device, uint64_t slot_bytes,
                   void *map, size_t map_len, issl_index **out) {
-    int rc = check_index_args(n_sites, n_slices, device);
+    // The index's own consistency first, then the device.
+    int rc = ISSL_OK;
+    if (seq_len == 20 && n_sites < (1ULL << 32) && count_wide_sigs(sigs, n_sites) != 0)
+        rc = fail(ISSL_ERR_FORMAT, "signatures hold codes beyond position 19");
+    if (rc == ISSL_OK) rc = check_index_args(n_sites, seq_len, n_slices, device);
     if (rc != ISSL_OK) {
         if (map) munmap(map, map_len);
         return rc;

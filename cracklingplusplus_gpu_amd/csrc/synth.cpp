@@ -170,6 +170,8 @@ extern "C" int issl_synth(const issl_synth_cfg *cfg, uint64_t *sigs, uint32_t *o
     const uint64_t Q = cfg->n_guides_site + cfg->n_guides_unique + cfg->n_guides_planted +
                        cfg->n_guides_repeat + cfg->n_guides_random;
     if (Q && !guides) return fail(ISSL_ERR_ARG, "issl_synth: null guides");
+    if (cfg->guide_batches < 0 || cfg->guide_batches > 4096)
+        return fail(ISSL_ERR_ARG, "issl_synth: guide_batches out of range");
     if (cfg->n_slices && !masks) return fail(ISSL_ERR_ARG, "issl_synth: null masks");
     if (N == 0 || N >= (1ULL << 32)) return fail(ISSL_ERR_ARG, "issl_synth: n_sites out of range");
     if (cfg->n_guides_planted > N) return fail(ISSL_ERR_ARG, "issl_synth: too many planted guides");
@@ -273,41 +275,47 @@ extern "C" int issl_synth(const issl_synth_cfg *cfg, uint64_t *sigs, uint32_t *o
         masks[s] = m;
     }
 
-    // Guides.
-    std::vector<uint8_t> kind(Q);
+    // Guides: batch b draws items b*count + i of each guide stream and is
+    // shuffled on its own, so batches are disjoint draws of the same mix and
+    // batch 0 is the single-batch workload.
+    const uint64_t B = cfg->guide_batches > 0 ? static_cast<uint64_t>(cfg->guide_batches) : 1;
+    std::vector<uint8_t> kind(B * Q);
     uint64_t g = 0;
-    for (uint64_t i = 0; i < cfg->n_guides_site; ++i, ++g) {
-        Rng r(seed, kStreamGuideSite, i);
-        guides[g] = sigs[r.below(N)];
-        kind[g] = 0;
+    for (uint64_t b = 0; b < B; ++b) {
+        const uint64_t g0 = g;
+        for (uint64_t i = 0; i < cfg->n_guides_site; ++i, ++g) {
+            Rng r(seed, kStreamGuideSite, b * cfg->n_guides_site + i);
+            guides[g] = sigs[r.below(N)];
+            kind[g] = 0;
+        }
+        for (uint64_t i = 0; i < cfg->n_guides_unique; ++i, ++g) {
+            Rng r(seed, kStreamGuideSite, (1ULL << 40) + b * cfg->n_guides_unique + i);
+            uint64_t id = r.below(N);
+            for (int t = 0; t < 1000 && (pool[id] & 1ULL); ++t) id = r.below(N);
+            guides[g] = sigs[id];
+            kind[g] = 1;
+        }
+        for (uint64_t i = 0; i < P; ++i, ++g) {
+            guides[g] = planted[i];
+            kind[g] = 2;
+        }
+        for (uint64_t i = 0; i < cfg->n_guides_repeat; ++i, ++g) {
+            Rng r(seed, kStreamGuideRepeat, b * cfg->n_guides_repeat + i);
+            guides[g] = nseeds ? substitute(rseed[r.below(nseeds)], tsub, r) : base.sig(r);
+            kind[g] = 3;
+        }
+        for (uint64_t i = 0; i < cfg->n_guides_random; ++i, ++g) {
+            Rng r(seed, kStreamGuideRandom, b * cfg->n_guides_random + i);
+            guides[g] = base.sig(r);
+            kind[g] = 4;
+        }
+        Rng sh(seed, kStreamShuffle, b);
+        for (uint64_t i = Q; i > 1; --i) {
+            const uint64_t j = sh.below(i);
+            std::swap(guides[g0 + i - 1], guides[g0 + j]);
+            std::swap(kind[g0 + i - 1], kind[g0 + j]);
+        }
     }
-    for (uint64_t i = 0; i < cfg->n_guides_unique; ++i, ++g) {
-        Rng r(seed, kStreamGuideSite, (1ULL << 40) + i);
-        uint64_t id = r.below(N);
-        for (int t = 0; t < 1000 && (pool[id] & 1ULL); ++t) id = r.below(N);
-        guides[g] = sigs[id];
-        kind[g] = 1;
-    }
-    for (uint64_t i = 0; i < P; ++i, ++g) {
-        guides[g] = planted[i];
-        kind[g] = 2;
-    }
-    for (uint64_t i = 0; i < cfg->n_guides_repeat; ++i, ++g) {
-        Rng r(seed, kStreamGuideRepeat, i);
-        guides[g] = nseeds ? substitute(rseed[r.below(nseeds)], tsub, r) : base.sig(r);
-        kind[g] = 3;
-    }
-    for (uint64_t i = 0; i < cfg->n_guides_random; ++i, ++g) {
-        Rng r(seed, kStreamGuideRandom, i);
-        guides[g] = base.sig(r);
-        kind[g] = 4;
-    }
-    Rng sh(seed, kStreamShuffle, 0);
-    for (uint64_t i = Q; i > 1; --i) {
-        const uint64_t j = sh.below(i);
-        std::swap(guides[i - 1], guides[j]);
-        std::swap(kind[i - 1], kind[j]);
-    }
-    if (kinds) std::memcpy(kinds, kind.data(), Q);
+    if (kinds) std::memcpy(kinds, kind.data(), B * Q);
     return ok();
 }

@@ -20,7 +20,7 @@ from __future__ import annotations
 from . import synth
 
 BASE = dict(gc=0.5, repeat_frac=0.0, n_repeat_seeds=0, repeat_sub_p=0.1, occ_p_unique=0.8,
-            occ_p_repeat=0.5, contiguous_masks=1, pad=0, n_guides_site=0, n_guides_unique=0,
+            occ_p_repeat=0.5, contiguous_masks=1, guide_batches=0, n_guides_site=0, n_guides_unique=0,
             n_guides_planted=0, n_guides_repeat=0, n_guides_random=0)
 
 WORKLOADS = {

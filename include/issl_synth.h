@@ -33,7 +33,11 @@ typedef struct {
     uint64_t n_slices;         /* slice masks */
     uint64_t slice_width;      /* positions per mask */
     int32_t  contiguous_masks; /* 1: windows 0..w-1, w..2w-1, ...; 0: seeded draws */
-    int32_t  pad;
+    int32_t  guide_batches;    /* disjoint guide batches of the stated mix (0 = 1): batch b
+                                  draws items b*count .. b*count + count - 1 of each guide
+                                  stream and is shuffled on its own; batch 0 is the single
+                                  batch.  Planted guides are the same in every batch (their
+                                  variants are sites). */
     uint64_t n_guides_site;    /* guides copied from the site set */
     uint64_t n_guides_unique;  /* guides copied from non-repeat sites */
     uint64_t n_guides_planted; /* random guides; a 1-4 substitution variant of
@@ -45,8 +49,9 @@ typedef struct {
 
 /* Generate one workload.  Buffers: sigs[n_sites] and occ[n_sites] (ids in
  * sorted-text order, all distinct), masks[n_slices],
- * guides[sum of n_guides_*] (shuffled), kinds[same] (nullable; 0 site,
- * 1 unique, 2 planted, 3 repeat, 4 random). */
+ * guides[max(1, guide_batches) * sum of n_guides_*] (batch after batch, each
+ * shuffled), kinds[same] (nullable; 0 site, 1 unique, 2 planted, 3 repeat,
+ * 4 random). */
 int issl_synth(const issl_synth_cfg *cfg, uint64_t *sigs, uint32_t *occ,
                uint64_t *masks, uint64_t *guides, uint8_t *kinds, int threads);
 

@@ -158,3 +158,48 @@ def test_scoring_calls_fail_loudly_without_gpu():
     with pytest.raises(issl.IsslError) as e:
         issl.Index.from_arrays(ix)
     assert e.value.code == _capi.ISSL_ERR_DEVICE
+
+
+def _index21(tmp_path):
+    """A 21-position index (ISSLCreateIndex accepts seqLength up to 21,
+    ISSLCreateIndex.cpp:296), five contiguous 4-position slices."""
+    rng = np.random.default_rng(21)
+    sigs = np.unique(rng.integers(0, 1 << 42, size=3000, dtype=np.uint64))
+    occ = np.ones(sigs.shape[0], dtype=np.uint32)
+    masks = np.array([0xF << (4 * s) for s in range(5)], dtype=np.uint64)
+    return issl.build_lists(sigs, occ, masks, seq_len=21), occ
+
+
+def test_21_position_index_is_refused(tmp_path):
+    """The reference takes the distance on the whole 64-bit XOR
+    (lib/ISSLScoringModule.cpp:286-290, ISSLScoreOfftargets.cpp:360-366), so a
+    21-position index scores differently from a 20-position distance; every
+    loader refuses it with ISSL_ERR_FORMAT before looking for a device."""
+    hx, occ = _index21(tmp_path)
+    assert hx.seq_len == 21 and int(hx.sigs.max()) >= (1 << 40)
+    p = str(tmp_path / "i21.issl")
+    issl.write_index(p, hx)
+    calls = [lambda: issl.Index.from_arrays(hx),
+             lambda: issl.Index.open(p),
+             lambda: issl.Index.from_arrays_streamed(hx),
+             lambda: issl.Index.open_streamed(p),
+             lambda: issl.Index.build(hx.sigs, occ, hx.masks, seq_len=21)]
+    for call in calls:
+        with pytest.raises(issl.IsslError) as e:
+            call()
+        assert e.value.code == _capi.ISSL_ERR_FORMAT, e.value
+        assert "20" in str(e.value)
+
+
+def test_wide_signatures_in_20_position_index_are_refused():
+    ix = issl.read_index(os.path.join(GOLDEN, "tiny4k_5x4.issl"))
+    sigs = ix.sigs.copy()
+    sigs[7] |= np.uint64(1 << 41)
+    bad = issl.IndexArrays(sigs, ix.masks, ix.list_sizes, ix.entries, 20)
+    for call in (lambda: issl.Index.from_arrays(bad),
+                 lambda: issl.Index.from_arrays_streamed(bad),
+                 lambda: issl.Index.build(sigs, ix.occurrences(), ix.masks)):
+        with pytest.raises(issl.IsslError) as e:
+            call()
+        assert e.value.code == _capi.ISSL_ERR_FORMAT, e.value
+        assert "position 19" in str(e.value)
