@@ -435,6 +435,7 @@ class Request:
         """Block until done; raise IsslError if the request failed."""
         st = ctypes.c_int32()
         check(lib().issl_server_wait(self.server._h, self.ticket, timeout, byref(st)))
+        self.server._inflight.pop(self.ticket, None)   # the server writes nothing more
         self.status = int(st.value)
         if self.status != _capi.ISSL_OK:
             raise _capi.IsslError(self.status, lib().issl_last_error().decode())
@@ -446,6 +447,7 @@ class Request:
         done, st = ctypes.c_int32(), ctypes.c_int32()
         check(lib().issl_server_poll(self.server._h, self.ticket, byref(done), byref(st)))
         if done.value:
+            self.server._inflight.pop(self.ticket, None)
             self.status = int(st.value)
         return bool(done.value)
 
@@ -521,12 +523,16 @@ class Server:
         h = c_void_p()
         check(lib().issl_server_create(byref(cfg), byref(h)))
         self._h = h
+        # Result buffers of requests not yet released (ticket -> Results): the
+        # server writes into them until the request completes, whether or not
+        # the caller still holds its Request.
         self._inflight = {}
 
     def close(self) -> None:
         if self._h is not None and self._h.value:
             check(lib().issl_server_destroy(self._h))
         self._h = None
+        self._inflight.clear()
 
     def __enter__(self):
         return self
@@ -563,6 +569,7 @@ class Server:
         t = c_uint64()
         check(lib().issl_server_submit(self._h, index_id, _ptr(g), n, byref(p), byref(res),
                                        byref(t)))
+        self._inflight[int(t.value)] = r
         return Request(self, int(t.value), r)
 
     def stats(self, reset: bool = False) -> _capi.issl_server_stats:

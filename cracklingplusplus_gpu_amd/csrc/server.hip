@@ -14,6 +14,10 @@
 //   completion : retires slots in launch order, scatters each slot's results
 //                into the requests' host buffers and completes a request when
 //                all its guides are back.
+// A request is complete only when none of its pieces is still in a slot, also
+// when it fails (a failed piece, index removal, server shutdown): the slots
+// still holding its other pieces write into its buffers when they retire, so
+// the caller may release those buffers only after the request completes.
 // Each guide's result depends only on the guide, so how requests are merged
 // or split never changes a result.
 #include <hip/hip_runtime.h>
@@ -52,6 +56,7 @@ struct Request {
     Clock::time_point t_submit;
     uint64_t launched = 0;      // guides handed to slots so far
     uint64_t returned = 0;      // guides whose results are back
+    uint64_t pending = 0;       // pieces in slots not yet retired
     bool done = false;
     int32_t status = ISSL_OK;
     std::string error;
@@ -120,19 +125,33 @@ struct issl_server {
 
     void launch_loop();
     void complete_loop();
-    void finish(const std::shared_ptr<Request> &r, int rc, const std::string &msg);
+    void fail_request(const std::shared_ptr<Request> &r, int rc, const std::string &msg);
+    void settle(const std::shared_ptr<Request> &r);
     int fill(Slot *s, issl_index *ix, const issl_params &p, const ScoreScalars &sc,
              const issl_tuning &tn);
 };
 
-// Mark a request complete (the caller holds mu).
-void issl_server::finish(const std::shared_ptr<Request> &r, int rc, const std::string &msg) {
+// Record a request's failure (the first one wins) and complete it if no slot
+// still holds a piece of it (the caller holds mu and has taken it off the
+// queue).
+void issl_server::fail_request(const std::shared_ptr<Request> &r, int rc, const std::string &msg) {
     if (r->done) return;
+    if (r->status == ISSL_OK) {
+        r->status = rc;
+        r->error = msg;
+    }
+    settle(r);
+}
+
+// Complete a request once no slot holds a piece of it and it has failed or
+// all its guides are back (the caller holds mu).  Waiters are woken by the
+// caller.
+void issl_server::settle(const std::shared_ptr<Request> &r) {
+    if (r->done || r->pending) return;
+    if (r->status == ISSL_OK && r->returned < r->guides.size()) return;
     r->done = true;
-    r->status = rc;
-    r->error = msg;
     ++done;
-    if (rc != ISSL_OK) ++failed;
+    if (r->status != ISSL_OK) ++failed;
     guides += r->guides.size();
     lat_ms.push_back(
         std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count());
@@ -195,6 +214,7 @@ void issl_server::launch_loop() {
             const uint64_t take = std::min<uint64_t>(r->guides.size() - r->launched, cfg.max_batch - s->n);
             s->pieces.push_back(Piece{r, r->launched, take});
             r->launched += take;
+            ++r->pending;
             s->n += take;
             if (r->launched == r->guides.size())
                 it = queue.erase(it);
@@ -273,14 +293,15 @@ void issl_server::complete_loop() {
         lk.lock();
         for (const Piece &pc : s->pieces) {
             const std::shared_ptr<Request> &r = pc.req;
+            --r->pending;
             if (rc != ISSL_OK) {
                 // the rest of a failed request is not launched
                 auto q = std::find(queue.begin(), queue.end(), r);
                 if (q != queue.end()) queue.erase(q);
-                finish(r, rc, s->error);
+                fail_request(r, rc, s->error);
             } else {
                 r->returned += pc.n;
-                if (r->returned == r->guides.size()) finish(r, ISSL_OK, std::string());
+                settle(r);
             }
         }
         s->pieces.clear();
@@ -324,7 +345,7 @@ extern "C" int issl_server_destroy(issl_server *srv) {
     {
         std::lock_guard<std::mutex> lk(srv->mu);
         for (const auto &r : srv->queue)
-            srv->finish(r, ISSL_ERR_STATE, "server destroyed before the request ran");
+            srv->fail_request(r, ISSL_ERR_STATE, "server destroyed before the request ran");
         srv->queue.clear();
         srv->stop = true;
     }
@@ -421,7 +442,7 @@ extern "C" int issl_server_remove_index(issl_server *srv, int32_t index_id) {
         if (it == srv->indexes.end()) return fail(ISSL_ERR_ARG, "unknown index id");
         for (auto q = srv->queue.begin(); q != srv->queue.end();) {
             if ((*q)->index_id == index_id) {
-                srv->finish(*q, ISSL_ERR_STATE, "index was removed");
+                srv->fail_request(*q, ISSL_ERR_STATE, "index was removed");
                 q = srv->queue.erase(q);
             } else {
                 ++q;
@@ -457,7 +478,7 @@ extern "C" int issl_server_submit(issl_server *srv, int32_t index_id, const uint
     srv->tickets[r->ticket] = r;
     *ticket = r->ticket;
     if (n == 0) {
-        srv->finish(r, ISSL_OK, std::string());
+        srv->settle(r);
         srv->cv_done.notify_all();
         return ok();
     }

@@ -25,7 +25,11 @@
  *     p50/p99 over the requests completed since the last reset.
  *
  * The caller's result buffers (issl_results, host memory) must stay valid
- * until the request completes; guides are copied at submit.
+ * until the request completes, i.e. until issl_server_wait or issl_server_poll
+ * reports it done; guides are copied at submit.  A request completes only
+ * when no launch still holds any of its guides, also when it fails (a failed
+ * launch, index removal, server shutdown): launches holding its other guides
+ * still write their results into its buffers when they finish.
  */
 #ifndef ISSL_SERVER_H
 #define ISSL_SERVER_H
@@ -62,8 +66,9 @@ typedef struct {
 
 int issl_server_create(const issl_server_config *cfg, issl_server **out);
 
-/* Fails every request still queued (ISSL_ERR_STATE), waits for the launch in
- * flight, closes the indexes and frees the server. */
+/* Fails every request still queued (ISSL_ERR_STATE), waits for the launches
+ * in flight (completing the requests they hold), closes the indexes and frees
+ * the server. */
 int issl_server_destroy(issl_server *srv);
 
 /* Admit an index from a .issl file or from host arrays (the layout of
@@ -75,8 +80,9 @@ int issl_server_add_index_arrays(issl_server *srv, uint64_t n_sites, uint64_t se
                                  const uint64_t *slice_masks, const uint64_t *list_sizes,
                                  const uint64_t *entries, int32_t *index_id);
 
-/* Remove an index: queued requests for it fail with ISSL_ERR_STATE; a launch
- * in flight on it completes first. */
+/* Remove an index: queued requests for it fail with ISSL_ERR_STATE (each
+ * completes once no launch holds any of its guides); the launches in flight
+ * on it complete first. */
 int issl_server_remove_index(issl_server *srv, int32_t index_id);
 
 /* Queue a request.  Parameters are checked here, so a bad request fails at

@@ -158,3 +158,37 @@ def test_large_requests_split_over_slots_match_direct(gpu_available):
         st = srv.stats()
         assert st.launches >= 6 and st.requests_failed == 0
         assert st.peak_inflight >= 2      # launches overlapped on the device
+
+
+def test_failed_split_request_completes_after_its_launches(gpu_available):
+    """A request split over many launches whose index is removed while some of
+    them are in flight fails, but completes only once no launch holds any of
+    its guides: after wait() its buffers are never written again, and the
+    guides that did come back carry the direct scoring's results."""
+    import time
+    hx = golden_index("mid50k_10x10")
+    g = np.resize(load_golden("mid50k_10x10")["guides"], 200_000)
+    with issl.Index.from_arrays(hx) as ix, issl.Server(max_batch=256, slots=3) as srv:
+        iid = srv.add_index(hx)
+        req = srv.submit(iid, g, precision=issl.PREC_FP64)   # 782 launches
+        res = req.results
+        del req                                # the server keeps the buffers alive
+        srv.remove_index(iid)                  # fails the rest; in-flight launches finish
+        ticket = next(iter(srv._inflight))
+        r = issl.Request(srv, ticket, res)
+        with pytest.raises(issl.IsslError) as e:
+            r.wait(timeout=60.0)
+        assert e.value.code == issl._capi.ISSL_ERR_STATE
+        assert ticket not in srv._inflight
+        snap = {k: getattr(res, k).copy() for k in ("mit", "cfd", "occ_count", "passed")}
+        time.sleep(0.2)
+        for k, v in snap.items():
+            assert np.array_equal(getattr(res, k), v), k
+        back = res.mit != 0                    # a scored guide has 10000/(100+S) > 0
+        assert int(back.sum()) < g.shape[0]
+        if back.any():
+            want = ix.score(g[back], form=issl.FORM_BATCHED, precision=issl.PREC_FP64)
+            assert (bits(res.mit[back]) == bits(want.mit)).all()
+            assert (bits(res.cfd[back]) == bits(want.cfd)).all()
+        st = srv.stats()
+        assert st.requests_failed == 1
