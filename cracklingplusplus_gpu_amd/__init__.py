@@ -373,11 +373,15 @@ class Index:
         check(lib().issl_candidates(self._h, _ptr(g), n, _ptr(off), _ptr(ids), total))
         return off, ids[:total]
 
-    def autotune(self, sample, cache_path: Optional[str] = None) -> _capi.issl_tuning:
+    def autotune(self, sample, cache_path: Optional[str] = None,
+                 precision: int = PREC_FP64) -> _capi.issl_tuning:
+        """Time the BATCHED launch shapes and site-word layouts on `sample` at
+        `precision` and keep the fastest (issl_autotune_precision)."""
         g = np.ascontiguousarray(sample, dtype=np.uint64)
         t = _capi.issl_tuning()
-        check(lib().issl_autotune(self._h, _ptr(g), g.shape[0],
-                                  cache_path.encode() if cache_path else None, byref(t)))
+        check(lib().issl_autotune_precision(self._h, _ptr(g), g.shape[0], precision,
+                                            cache_path.encode() if cache_path else None,
+                                            byref(t)))
         return t
 
     def tuning(self) -> _capi.issl_tuning:

@@ -2,10 +2,12 @@
 //
 // Candidates: tile length (chunk), postings in flight per lane (depth),
 // persistent workgroups per CU, and the one-wave-per-guide kernel with or
-// without hand-off of long guides to the tile kernel.  Each is timed with HIP events on a sample of
-// the caller's guides against the resident index; the fastest is kept on the
-// index and, when a cache path is given, appended to a text cache keyed by
-// the index shape and the device, so later processes skip the sweep.
+// without hand-off of long guides to the tile kernel.  Each is timed with HIP
+// events on a sample of the caller's guides against the resident index, at
+// the precision the caller will score in; the fastest is kept on the index
+// and, when a cache path is given, appended to a text cache keyed by the
+// index shape, the sample size, the precision and the device, so later
+// processes skip the sweep.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -23,7 +25,7 @@ namespace {
 
 // The pick depends on the index's shape and on the batch: the sample's size
 // enters the key by its power of two.
-std::string shape_key(const issl_index *ix, uint64_t n_sample) {
+std::string shape_key(const issl_index *ix, uint64_t n_sample, int precision) {
     hipDeviceProp_t prop;
     std::string arch = "unknown";
     int cus = 0;
@@ -38,7 +40,7 @@ std::string shape_key(const issl_index *ix, uint64_t n_sample) {
     for (const SliceMeta &m : ix->meta_h) k << m.width << ',';
     int q = 0;
     while ((n_sample >> q) > 1) ++q;
-    k << "/q" << q;
+    k << "/q" << q << "/p" << precision;
     return k.str();
 }
 
@@ -88,12 +90,15 @@ extern "C" int issl_set_tuning(issl_index *ix, const issl_tuning *in) {
     return ok();
 }
 
-extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
-                             const char *cache_path, issl_tuning *best) {
+extern "C" int issl_autotune_precision(issl_index *ix, const uint64_t *sample, uint64_t n,
+                                       int32_t precision, const char *cache_path,
+                                       issl_tuning *best) {
     if (!ix || !best) return fail(ISSL_ERR_ARG, "null argument");
     if (n == 0 || !sample) return fail(ISSL_ERR_ARG, "autotune needs sample guides");
     if (ix->src) return fail(ISSL_ERR_ARG, "autotune needs a resident index");
-    const std::string key = shape_key(ix, n);
+    if (precision != ISSL_PREC_FP64 && precision != ISSL_PREC_FP32C)
+        return fail(ISSL_ERR_ARG, "unknown precision");
+    const std::string key = shape_key(ix, n, precision);
     issl_tuning cached{};
     int cached_res = 0;
     if (cache_lookup(cache_path, key, &cached, &cached_res)) {
@@ -133,13 +138,13 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
     auto measure = [&](issl_tuning &t) -> int {
         int r0;
         for (int r = 0; r < 2; ++r)
-            if ((r0 = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st)) != ISSL_OK)
+            if ((r0 = launch_batched(ix, dg, n, sc, precision, 0, t, o, nullptr, ix->ws, st)) != ISSL_OK)
                 return r0;
         float bestms = 1e30f;
         for (int r = 0; r < kTrials; ++r) {
             ISSL_HIP(hipEventRecord(e0, st));
             for (int k = 0; k < kPerTrial; ++k)
-                if ((r0 = launch_batched(ix, dg, n, sc, ISSL_PREC_FP32C, 0, t, o, nullptr, ix->ws, st)) != ISSL_OK)
+                if ((r0 = launch_batched(ix, dg, n, sc, precision, 0, t, o, nullptr, ix->ws, st)) != ISSL_OK)
                     return r0;
             ISSL_HIP(hipEventRecord(e1, st));
             ISSL_HIP(hipEventSynchronize(e1));
@@ -213,4 +218,9 @@ extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
         }
     }
     return ok();
+}
+
+extern "C" int issl_autotune(issl_index *ix, const uint64_t *sample, uint64_t n,
+                             const char *cache_path, issl_tuning *best) {
+    return issl_autotune_precision(ix, sample, n, ISSL_PREC_FP64, cache_path, best);
 }

@@ -261,8 +261,12 @@ typedef struct {
 
 /* Time the BATCHED kernel's launch shapes, and the site-word layouts the
  * index holds (issl_index_set_residual), on a sample of guides for this index
- * and keep the fastest (also written to the cache file when path is
- * non-NULL).  No reference counterpart. */
+ * at `precision` (ISSL_PREC_*) and keep the fastest (also written to the
+ * cache file when path is non-NULL; the cache is keyed by the index shape,
+ * the sample size's power of two, the precision and the device).
+ * issl_autotune tunes at ISSL_PREC_FP64.  No reference counterpart. */
+int issl_autotune_precision(issl_index *ix, const uint64_t *sample_guides, uint64_t n,
+                            int32_t precision, const char *cache_path, issl_tuning *best);
 int issl_autotune(issl_index *ix, const uint64_t *sample_guides, uint64_t n,
                   const char *cache_path, issl_tuning *best);
 int issl_get_tuning(issl_index *ix, issl_tuning *out);
