@@ -2,19 +2,25 @@
 """Benchmark: ISSL MIT+CFD scoring throughput on one MI355X.
 
 Workload (default): 100,000,000 distinct 20-mer sites at 41 % GC (2 % from
-2,000 repeat seeds), 10 seeded slice masks of 10 positions, 10,000 guides
-(half from the sites, a quarter repeat neighbourhoods, a quarter random),
-MIT+CFD (method mitAndCfd), score threshold disabled, maxDist 4.
+2,000 repeat seeds), 10 seeded slice masks of 10 positions, batches of 10,000
+guides (half from the sites, a quarter repeat neighbourhoods, a quarter
+random), MIT+CFD (method mitAndCfd), score threshold disabled, maxDist 4.
 
-One step = scoring the 10,000 guides once, with the index and the guides
-already resident in HBM (issl_score_device).  `value` is guides/s of the
-BATCHED form (persistent tile kernel, FP32 products with compensated FP32
-lane sums combined in FP64); the
-PLAIN form (one guide per workgroup, FP64) and the CPU scorer (the oracle's
-OpenMP restatement of ISSLScoringModule::run on the host cores, same
-workload) are reported beside it.
+One step = scoring one 10,000-guide batch, with the index and the guides
+already resident in HBM (issl_score_device).  The timed steps rotate over
+--batches disjoint batches of that mix (batch 0 is the workload's own 10,000
+guides), so that the lists the steps stream do not fit the 256 MB Infinity
+Cache together; the autotuner picks the launch shape on a further, held-out
+batch.  `value` is guides/s of the BATCHED form in FP64 (persistent tile
+kernel, FP64 terms and sums: the reference's arithmetic).  Beside it: the
+same steps in compensated FP32 (FP32C), the PLAIN form (one guide per
+workgroup, FP64, bit-exact), the other site-word layout, the CPU scorer (the
+oracle's OpenMP restatement of ISSLScoringModule::run on the host cores),
+100,000 guides in one launch on the same index, and the serving loop with the
+100 M- and 300 M-site indexes resident together.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+    python bench.py --headline-only        (the timed steps and kernel timing only)
 """
 from __future__ import annotations
 
@@ -25,27 +31,23 @@ import sys
 import tempfile
 import time
 
-import numpy as np
+# The CPU scorer's OpenMP threads stay on neighbouring cores; the runtime reads
+# this when it loads, so it is set before any library that links it.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PCIE_PEAK_GBS = 64.0   # host link of the MI355X: PCIe 5.0 x16, one direction
-TIMING_REPEAT = 10   # dominant-kernel dispatches per HIP event pair
+INFINITY_CACHE_BYTES = 256 << 20
 METRIC = "guides/sec at 100M-site index (MIT+CFD); p99 latency; HBM GB/s fraction"
 
 
 def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
-
-
-def host_threads() -> int:
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
 
 
 def cpu_model() -> str:
@@ -58,18 +60,55 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def postings_per_guide(ix, guides: np.ndarray) -> np.ndarray:
-    """P(g): total length of the slice lists each guide visits."""
-    n_lists_before = 0
-    total = np.zeros(guides.shape[0], dtype=np.int64)
+def cpu_info() -> dict:
+    """Host cores this process may use: the affinity mask, capped by a cgroup
+    CPU quota when one is set (threads beyond the quota only time-share)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"threads": threads, "affinity_cpus": aff, "cgroup_cpu_quota": quota,
+            "system_cpus": os.cpu_count(), "model": cpu_model(),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+
+
+def host_threads() -> int:
+    return cpu_info()["threads"]
+
+
+def list_index(ix, guides: np.ndarray) -> np.ndarray:
+    """Global list number (slice-major over all slices' 4^w lists) of every
+    (guide, slice): shape (n, S)."""
+    out = np.zeros((guides.shape[0], ix.n_slices), dtype=np.int64)
+    base = 0
     for s, m in enumerate(ix.masks.tolist()):
         pos = [j for j in range(ix.seq_len) if (m >> j) & 1]
         key = np.zeros(guides.shape[0], dtype=np.int64)
         for j, p in enumerate(pos):
             key |= ((guides >> np.uint64(2 * p)) & np.uint64(3)).astype(np.int64) << (2 * j)
-        total += ix.list_sizes[n_lists_before + key].astype(np.int64)
-        n_lists_before += 1 << (2 * len(pos))
-    return total
+        out[:, s] = base + key
+        base += 1 << (2 * len(pos))
+    return out
+
+
+def postings_per_guide(ix, guides: np.ndarray) -> np.ndarray:
+    """P(g): total length of the slice lists each guide visits."""
+    return ix.list_sizes[list_index(ix, guides)].astype(np.int64).sum(axis=1)
+
+
+def footprint_bytes(ix, batches, word: int) -> int:
+    """Bytes of site words the batches stream together: every list any of
+    them visits, once."""
+    ids = np.unique(np.concatenate([list_index(ix, b).ravel() for b in batches]))
+    return int(ix.list_sizes[ids].astype(np.int64).sum()) * word
 
 
 def copy_calibration(hiprt) -> int:
@@ -85,23 +124,6 @@ def copy_calibration(hiprt) -> int:
     return 0
 
 
-def measured_traffic(workload: str, layout: str):
-    """HBM bytes per k_tiles launch from the committed counter summary.
-
-    The summary holds FETCH_SIZE + WRITE_SIZE per dispatch of k_tiles on the
-    named workload, for each site-word layout ("inline" or "residual"), each
-    scaled by the reading of a 1 GiB copy.  Returns None when no summary
-    covers this workload and layout.
-    """
-    path = os.path.join(ROOT, "profiles", f"k_tiles_{workload}_hbm_traffic.md")
-    key = f"traffic_bytes_per_launch_{layout}:"
-    try:
-        for line in open(path):
-            if line.startswith(key):
-                return int(line.split(":", 1)[1].strip().replace(",", ""))
-    except OSError:
-        pass
-    return None
 
 
 def stream_leg(issl, hiprt, hx, guides, ix, args) -> dict:
@@ -314,13 +336,195 @@ def serving_main(args) -> int:
     return 0
 
 
+
+
+def measured_traffic(kernel_tag: str, workload: str, layout: str):
+    """HBM bytes per dispatch of a kernel from its committed counter summary.
+
+    profiles/<kernel_tag>_<workload>_hbm_traffic.md holds FETCH_SIZE +
+    WRITE_SIZE per dispatch on the named workload, for each site-word layout
+    ("inline" or "residual"), each scaled by the reading of a 1 GiB copy.
+    Returns None when no summary covers them."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_{workload}_hbm_traffic.md")
+    key = f"traffic_bytes_per_launch_{layout}:"
+    try:
+        for line in open(path):
+            if line.startswith(key):
+                return int(line.split(":", 1)[1].strip().replace(",", ""))
+    except OSError:
+        pass
+    return None
+
+
+class Rotation:
+    """Disjoint guide batches resident in HBM, each with its own device result
+    columns; step k scores batch k mod R."""
+
+    def __init__(self, hiprt, batches):
+        self.batches = [np.ascontiguousarray(b) for b in batches]
+        self.Q = self.batches[0].shape[0]
+        self.d_g = [hiprt.DeviceArray.from_host(b) for b in self.batches]
+        self.d_out = [{"mit": hiprt.DeviceArray(self.Q, np.float64),
+                       "cfd": hiprt.DeviceArray(self.Q, np.float64),
+                       "occ_count": hiprt.DeviceArray(self.Q, np.uint64),
+                       "site_count": hiprt.DeviceArray(self.Q, np.uint64),
+                       "pass": hiprt.DeviceArray(self.Q, np.uint8)} for _ in self.batches]
+
+    def __len__(self):
+        return len(self.batches)
+
+    def run(self, ix, steps: int, start: int, **kw) -> None:
+        for k in range(steps):
+            b = (start + k) % len(self)
+            ix.score_device(self.d_g[b], self.Q, self.d_out[b], **kw)
+
+    def results(self):
+        return [{k: v.to_host() for k, v in o.items()} for o in self.d_out]
+
+
+def timed(hiprt, rot, ix, steps, warmup, **kw):
+    """Warm-up steps, then `steps` timed steps between two device
+    synchronizations; ms per step."""
+    rot.run(ix, max(1, warmup), 0, **kw)
+    hiprt.synchronize()
+    t0 = time.perf_counter()
+    rot.run(ix, steps, 0, **kw)
+    hiprt.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def kernel_us(hiprt, rot, ix, steps, **kw):
+    """Average duration of the dominant kernel's dispatches over `steps`
+    rotated steps: a HIP event pair attached to each dispatch on the stream
+    it runs on (one dispatch per pair, no repeats of a batch)."""
+    ix.timing_enable(True, repeat=1)
+    rot.run(ix, steps, 0, **kw)
+    hiprt.synchronize()
+    kms, nl = ix.timing_read()
+    ix.timing_enable(False)
+    return kms * 1e3 / max(nl, 1), nl
+
+
+def max_rel(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b))) if a.size else 0.0
+
+
+def kernel_name(issl, tn):
+    return ("k_tiles" if tn.heavy_kpost == 0 else
+            "k_stream" if tn.heavy_kpost == issl._capi.HEAVY_NEVER else "k_stream+k_tiles")
+
+
+def large_batch_leg(issl, hiprt, ix, hx, args, threads):
+    """100,000 guides in one launch on the same index (30 % repeat
+    neighbourhoods, 70 % unique sites), BATCHED FP64, tuned on a held-out
+    batch of the same mix; PLAIN FP64 on the same launch beside it."""
+    from cracklingplusplus_gpu_amd import workloads
+    name = "genome100m_10x10_q100k"
+    log(f"{name}: generating guides")
+    sigs2, _, masks2, guides2, _ = workloads.make(name, threads=threads, guide_batches=2)
+    if not (np.array_equal(sigs2, hx.sigs) and np.array_equal(masks2, hx.masks)):
+        raise RuntimeError(f"{name} does not share the index of the headline workload")
+    del sigs2
+    Q2 = guides2.shape[0] // 2
+    g_time, g_tune = guides2[:Q2], guides2[Q2:]
+    tn = ix.autotune(g_tune, args.tune_cache, precision=issl.PREC_FP64)
+    rot = Rotation(hiprt, [g_time])
+    kw = dict(method=issl.METHOD_MIT_AND_CFD, threshold=0.0, form=issl.FORM_BATCHED,
+              precision=issl.PREC_FP64)
+    ms = timed(hiprt, rot, ix, args.large_steps, 1, **kw)
+    k_us, _ = kernel_us(hiprt, rot, ix, args.large_steps, **kw)
+    res_b = rot.results()[0]
+    P = postings_per_guide(hx, g_time)
+    S = hx.n_slices
+    bytes_ref = 16 * int(P.sum()) + 16 * S * Q2
+    _, res_used, _ = ix.residual()
+    word = 4 if res_used else 8
+    kwp = dict(kw, form=issl.FORM_PLAIN)
+    ms_p = timed(hiprt, rot, ix, 2, 1, **kwp)
+    res_p = rot.results()[0]
+    out = {"workload": name, "guides_per_launch": Q2, "value": round(Q2 / (ms * 1e-3), 1),
+           "unit": "guides/s", "ms_per_step": round(ms, 4), "precision": "fp64",
+           "kernel": kernel_name(issl, tn), "kernel_us": round(k_us, 2),
+           "site_words": "list-residual, 4 B" if res_used else "signature-inline, 8 B",
+           "postings_per_launch": int(P.sum()),
+           "postings_p50_p99_max": [int(np.percentile(P, 50)), int(np.percentile(P, 99)),
+                                    int(P.max())],
+           "roofline_achieved_GBps": round(bytes_ref / (k_us * 1e-6) / 1e9, 1),
+           "roofline_frac": round(bytes_ref / (k_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+           "layout_GBps": round((word * int(P.sum()) + 16 * S * Q2) / (k_us * 1e-6) / 1e9, 1),
+           "tuning": {"chunk": tn.chunk, "depth": tn.depth, "grid_per_cu": tn.grid_per_cu,
+                      "heavy_kpost": tn.heavy_kpost},
+           "plain": {"ms_per_step": round(ms_p, 4), "value": round(Q2 / (ms_p * 1e-3), 1)},
+           "fp64_vs_plain_max_rel": max(max_rel(res_b["mit"], res_p["mit"]),
+                                        max_rel(res_b["cfd"], res_p["cfd"])),
+           "counts_identical": bool((res_b["occ_count"] == res_p["occ_count"]).all())}
+    log(f"{name}: {ms:.3f} ms/step, {out['value']:,.0f} guides/s, kernel {k_us:.1f} us, "
+        f"plain {ms_p:.3f} ms")
+    return out
+
+
+def serving_c4_leg(issl, hx1, args, threads):
+    """The 100 M- and 300 M-site indexes resident together in one server:
+    Poisson arrivals of log-uniform 1..50,000-guide requests against a random
+    index, and 1,000-guide requests at a rate at which arriving requests queue
+    behind busy slots and are merged into shared launches."""
+    from cracklingplusplus_gpu_amd import workloads
+    srv = issl.Server(max_batch=65536, hbm_reserve=8 << 30)
+    ids, pools = [srv.add_index(hx1)], []
+    g1 = workloads.make("genome100m_10x10", threads=threads, guide_batches=4)[3]
+    pools.append(g1)
+    log("serving: generating genome300m_10x10")
+    sigs, occ, masks, g3, _ = workloads.make("genome300m_10x10", threads=threads)
+    hx3 = issl.build_lists(sigs, occ, masks, threads=threads)
+    del sigs, occ
+    ids.append(srv.add_index(hx3))
+    pools.append(g3)
+    del hx3
+    st = srv.stats()
+    log(f"serving: resident {st.resident_bytes / 1e9:.1f} GB, free {st.hbm_free / 1e9:.1f} GB")
+    rng = np.random.default_rng(0x5E4E)
+    hi = np.log(50000.0)
+
+    def log_uniform(r):
+        return max(1, int(np.exp(r.uniform(0.0, hi))))
+
+    serve_poisson(issl, srv, ids, pools, 50.0, 10, log_uniform, rng)   # warm both indexes
+    legs = []
+    for rate in args.c4_rates:
+        n_req = max(20, int(rate * args.serve_seconds))
+        res = serve_poisson(issl, srv, ids, pools, rate, n_req, log_uniform, rng)
+        res["request_guides"] = "log-uniform 1..50000"
+        log(f"serving C4 {rate} req/s: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms, "
+            f"{res['launches']} launches for {res['requests']} requests")
+        legs.append(res)
+    for rate in args.merge_rates:
+        res = serve_poisson(issl, srv, ids, pools, rate, args.merge_requests, lambda r: 1000, rng)
+        res["request_guides"] = 1000
+        log(f"serving C4 1,000-guide requests at {rate} req/s (achieved {res['achieved_rps']}): "
+            f"p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms, {res['launches']} launches for "
+            f"{res['requests']} requests (mean batch {res['mean_batch_guides']})")
+        legs.append(res)
+    srv.close()
+    return {"workload": "genome100m_10x10 + genome300m_10x10 resident in one server",
+            "resident_GB": round(st.resident_bytes / 1e9, 2), "form": "batched",
+            "precision": "fp32c", "max_batch": 65536, "slots": 3, "legs": legs}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="genome100m_10x10")
-    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--batches", type=int, default=6,
+                    help="disjoint guide batches the timed steps rotate over")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the timed steps and their kernel timing (for profiling)")
+    ap.add_argument("--form", choices=["batched", "plain"], default="batched",
+                    help="form of the timed steps with --headline-only (profiling)")
+    ap.add_argument("--precision", choices=["fp64", "fp32c"], default="fp64",
+                    help="precision of the timed steps with --headline-only (profiling)")
+    ap.add_argument("--plain-steps", type=int, default=6)
     ap.add_argument("--cpu-sample", type=int, default=10000,
                     help="guides timed on the host CPU scorer")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -347,6 +551,13 @@ def main() -> int:
     ap.add_argument("--extract-bases", type=int, default=500_000_000,
                     help="synthetic genome of the extraction leg (0: skip the leg)")
     ap.add_argument("--extract-cpu-bases", type=int, default=50_000_000)
+    ap.add_argument("--large-steps", type=int, default=5,
+                    help="steps of the 100,000-guide leg (0: skip it)")
+    ap.add_argument("--c4", type=int, default=1,
+                    help="1: serving leg with the 100 M- and 300 M-site indexes resident")
+    ap.add_argument("--c4-rates", type=float, nargs="+", default=[50, 500])
+    ap.add_argument("--merge-rates", type=float, nargs="+", default=[20000])
+    ap.add_argument("--merge-requests", type=int, default=4000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -368,11 +579,18 @@ def main() -> int:
         return copy_calibration(hiprt)
     if args.serving:
         return serving_main(args)
-    threads = host_threads()
+    cpu = cpu_info()
+    threads = cpu["threads"]
     t_setup = time.time()
-    log(f"generating workload {args.workload} with {threads} threads")
-    sigs, occ, masks, guides, kinds = workloads.make(args.workload, threads=threads)
-    log(f"sites {sigs.shape[0]:,}  guides {guides.shape[0]:,}; building slice lists")
+    R = max(1, args.batches)
+    log(f"generating workload {args.workload}: {R} timed guide batches and one held out, "
+        f"{threads} threads")
+    sigs, occ, masks, guides_all, kinds = workloads.make(args.workload, threads=threads,
+                                                         guide_batches=R + 1)
+    Q = guides_all.shape[0] // (R + 1)
+    batches = [guides_all[b * Q:(b + 1) * Q] for b in range(R)]
+    g_tune = guides_all[R * Q:]
+    log(f"sites {sigs.shape[0]:,}, {R} x {Q:,} guides; building slice lists")
     t_b = time.perf_counter()
     hx = issl.build_lists(sigs, occ, masks, threads=threads)
     host_build_s = time.perf_counter() - t_b
@@ -380,216 +598,57 @@ def main() -> int:
     ix = issl.Index.from_arrays(hx)
     info = ix.info()
     log(f"index resident: {info.device_bytes / 1e9:.2f} GB of HBM")
-    Q = guides.shape[0]
     S = hx.n_slices
-    P = postings_per_guide(hx, guides)
-    p_total = int(P.sum())
-    log(f"postings per step {p_total:,} (mean {P.mean():.0f}, max {P.max():,})")
+    P = [postings_per_guide(hx, b) for b in batches]
+    p_step = [int(p.sum()) for p in P]
+    P0 = P[0]
+    log(f"postings per step {np.mean(p_step):,.0f} (batch 0: mean {P0.mean():.0f}, "
+        f"p99 {np.percentile(P0, 99):.0f}, max {P0.max():,})")
 
-    d_guides = hiprt.DeviceArray.from_host(guides)
-    d_out = {"mit": hiprt.DeviceArray(Q, np.float64), "cfd": hiprt.DeviceArray(Q, np.float64),
-             "occ_count": hiprt.DeviceArray(Q, np.uint64),
-             "pass": hiprt.DeviceArray(Q, np.uint8)}
-    log("autotuning the batched kernel")
+    log("autotuning the batched kernel (FP64) on the held-out batch")
     os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
-    tn = ix.autotune(guides, args.tune_cache)
+    tn = ix.autotune(g_tune, args.tune_cache, precision=issl.PREC_FP64)
     log(f"tuning: chunk {tn.chunk} depth {tn.depth} grid/CU {tn.grid_per_cu} "
-        f"stream depth {tn.direct_depth} "
-        f"heavy {tn.heavy_kpost} ({tn.us_per_guide * 1e3:.2f} ns/guide)")
+        f"stream depth {tn.direct_depth} heavy {tn.heavy_kpost} "
+        f"({tn.us_per_guide * 1e3:.2f} ns/guide)")
+    res_built, res_used, _ = ix.residual()
+    word = 4 if res_used else 8
+    foot = footprint_bytes(hx, batches, word)
+    log(f"site words streamed by the {R} batches together: {foot / 1e6:.0f} MB "
+        f"(Infinity Cache {INFINITY_CACHE_BYTES >> 20} MB)")
+    rot = Rotation(hiprt, batches)
     setup_s = time.time() - t_setup
 
     kw = dict(method=issl.METHOD_MIT_AND_CFD, threshold=0.0, form=issl.FORM_BATCHED,
-              precision=issl.PREC_FP32C)
-    # With --graph the step's launch sequence is captured in a hipGraph by the
-    # first warm-up call and replayed afterwards (measured slower than eager
-    # launches for this three-kernel sequence, so off by default).
+              precision=issl.PREC_FP64)
+    if args.headline_only:
+        kw.update(form=issl.FORM_PLAIN if args.form == "plain" else issl.FORM_BATCHED,
+                  precision=issl.PREC_FP32C if args.precision == "fp32c" else issl.PREC_FP64)
+    elif (args.form, args.precision) != ("batched", "fp64"):
+        raise SystemExit("--form / --precision apply to --headline-only runs only")
+    # With --graph each batch's launch sequence is captured in a hipGraph by its
+    # first warm-up call and replayed afterwards (slower than eager launches for
+    # this three-kernel sequence, so off by default).
     kw_step = dict(kw, flags=issl.FLAG_GRAPH if args.graph else 0)
 
-    # ---- timed region: batched, inputs resident in HBM
-    for _ in range(max(1, args.warmup)):
-        ix.score_device(d_guides, Q, d_out, **kw_step)
-    hiprt.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ix.score_device(d_guides, Q, d_out, **kw_step)
-    hiprt.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = Q * args.steps / (t1 - t0)
-    log(f"batched: {ms_per_step:.3f} ms/step, {value:,.0f} guides/s")
+    # ---- timed region: BATCHED FP64, inputs resident in HBM, rotating batches
+    warm = max(args.warmup, R if args.graph else 0)
+    ms_per_step = timed(hiprt, rot, ix, args.steps, warm, **kw_step)
+    value = Q / (ms_per_step * 1e-3)
+    log(f"batched fp64: {ms_per_step:.4f} ms/step, {value:,.0f} guides/s")
 
-    # ---- dominant kernel time (k_tiles): HIP events on the index's stream around
-    # TIMING_REPEAT back-to-back dispatches of k_tiles per step (it is idempotent)
-    ix.timing_enable(True, repeat=TIMING_REPEAT)
-    for _ in range(args.steps):
-        ix.score_device(d_guides, Q, d_out, **kw)
-    kms, nl = ix.timing_read()
-    ix.timing_enable(False)
-    k_avg_ms = kms / max(nl, 1)
-    res_built, res_used, _ = ix.residual()
-    # Algorithmic bytes per k_tiles dispatch: the reference's method reads an
-    # 8 B posting entry and gathers an 8 B signature per posting, plus two 8 B
-    # list bounds per (guide, slice).  That figure does not depend on how this
-    # build lays the sites out; the bytes its own layout streams (one 4 B
-    # list-residual or 8 B signature-inline word per posting, plus the bounds)
-    # and the measured HBM traffic are reported beside it.
-    word = 4 if res_used else 8
-    bytes_layout = word * p_total + 16 * S * Q
-    bytes_inline = 8 * p_total + 16 * S * Q
-    bytes_ref_layout = 16 * p_total + 16 * S * Q
-    achieved = bytes_ref_layout / (k_avg_ms * 1e-3) / 1e9
-    # The timed dispatch: the tile kernel, the one-wave-per-guide kernel, or
-    # both together when the tuning sends only long guides to the tile kernel.
-    kernel = ("k_tiles" if tn.heavy_kpost == 0 else
-              "k_stream" if tn.heavy_kpost == issl._capi.HEAVY_NEVER else "k_stream+k_tiles")
-    # (the committed counter summaries cover the tile kernel's dispatches)
-    traffic = (measured_traffic(args.workload, "residual" if res_used else "inline")
+    # ---- dominant kernel time, HIP events on each of K rotated steps' dispatches
+    k_us, n_disp = kernel_us(hiprt, rot, ix, args.steps, **kw)
+    kernel = "k_plain" if kw["form"] == issl.FORM_PLAIN else kernel_name(issl, tn)
+    steps_bytes = [16 * p_step[k % R] + 16 * S * Q for k in range(args.steps)]
+    bytes_ref = int(np.mean(steps_bytes))
+    bytes_layout = int(np.mean([word * p_step[k % R] + 16 * S * Q for k in range(args.steps)]))
+    achieved = bytes_ref / (k_us * 1e-6) / 1e9
+    traffic = (measured_traffic("k_tiles_fp64", args.workload, "residual" if res_used else "inline")
                if kernel == "k_tiles" else None)
-    log(f"{kernel} avg {k_avg_ms * 1e3:.1f} us/launch -> {achieved:,.0f} GB/s")
-
-    res_batched = {k: v.to_host() for k, v in d_out.items()}
-
-    # ---- the same step with the other site-word layout at the same launch
-    # shape (the autotuner picks the layout with the shape); results must be
-    # the same bits
-    other_leg = None
-    if res_built:
-        ix.set_residual(not res_used)
-        for _ in range(max(1, args.warmup)):
-            ix.score_device(d_guides, Q, d_out, **kw)
-        hiprt.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ix.score_device(d_guides, Q, d_out, **kw)
-        hiprt.synchronize()
-        ms_i = (time.perf_counter() - t0) * 1e3 / args.steps
-        ix.timing_enable(True, repeat=TIMING_REPEAT)
-        for _ in range(args.steps):
-            ix.score_device(d_guides, Q, d_out, **kw)
-        kms_i, nl_i = ix.timing_read()
-        ix.timing_enable(False)
-        k_i = kms_i / max(nl_i, 1)
-        same_i = all(np.array_equal(d_out[k].to_host().view(np.uint8), res_batched[k].view(np.uint8))
-                     for k in d_out)
-        word_o = 8 if res_used else 4
-        other_leg = {"site_words": "signature-inline, 8 B" if res_used else "list-residual, 4 B",
-                     "value": round(Q / (ms_i * 1e-3), 1), "ms_per_step": round(ms_i, 4),
-                     "kernel_us": round(k_i * 1e3, 2),
-                     "layout_GBps": round((word_o * p_total + 16 * S * Q) / (k_i * 1e-3) / 1e9, 1),
-                     "same_bits": bool(same_i)}
-        log(f"other layout: {ms_i:.3f} ms/step, k_tiles {k_i * 1e3:.1f} us, same bits {same_i}")
-        ix.set_residual(res_used)
-
-    # ---- plain form, same workload
-    kwp = dict(kw, form=issl.FORM_PLAIN, precision=issl.PREC_FP64)
-    ix.score_device(d_guides, Q, d_out, **kwp)
-    hiprt.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        ix.score_device(d_guides, Q, d_out, **kwp)
-    hiprt.synchronize()
-    t1 = time.perf_counter()
-    plain_ms = (t1 - t0) * 1e3 / args.plain_steps
-    ix.timing_enable(True, repeat=TIMING_REPEAT)
-    for _ in range(args.plain_steps):
-        ix.score_device(d_guides, Q, d_out, **kwp)
-    pkms, pnl = ix.timing_read()
-    ix.timing_enable(False)
-    plain_k_ms = pkms / max(pnl, 1)
-    res_plain = {k: v.to_host() for k, v in d_out.items()}
-    log(f"plain: {plain_ms:.3f} ms/step, kernel {plain_k_ms * 1e3:.1f} us")
-
-    # ---- precision of the batched FP32C form against the FP64 in-order path
-    rel = {}
-    for f in ("mit", "cfd"):
-        a, b = res_batched[f], res_plain[f]
-        rel[f] = np.abs(a - b) / np.abs(b)
-    same_counts = bool((res_batched["occ_count"] == res_plain["occ_count"]).all())
-
-    # ---- request latency: 1,000-guide requests, host buffers, one at a time
-    lat = []
-    rng = np.random.default_rng(7)
-    for i in range(args.latency_requests):
-        lo = int(rng.integers(0, max(1, Q - 1000)))
-        req = guides[lo:lo + 1000]
-        t0 = time.perf_counter()
-        ix.score(req, method=issl.METHOD_MIT_AND_CFD, threshold=0.0,
-                 form=issl.FORM_BATCHED, precision=issl.PREC_FP32C)
-        lat.append((time.perf_counter() - t0) * 1e3)
-    lat = np.array(lat)
-
-    # ---- the whole step through host buffers (issl_score): guides uploaded,
-    # results copied back, both ways over PCIe; reported beside `value`
-    hres = ix.results(Q)
-    for _ in range(max(1, args.warmup)):
-        ix.score(guides, out=hres, **kw)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ix.score(guides, out=hres, **kw)
-    host_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    host_same = bool(np.array_equal(hres.cfd.view(np.uint64), res_batched["cfd"].view(np.uint64)))
-    log(f"host buffers: {host_ms:.3f} ms/call, {Q / host_ms * 1e3:,.0f} guides/s")
-
-    # ---- index construction on the device (sites and occurrences uploaded,
-    # lists sorted in HBM) against the host builder above
-    t_b = time.perf_counter()
-    with issl.Index.build(sigs, occ, masks) as dix:
-        device_build_s = time.perf_counter() - t_b
-        dsizes, _ = dix.export(entries=False)
-    del occ
-    build = {"host_lists_s": round(host_build_s, 3), "host_threads": threads,
-             "device_build_s": round(device_build_s, 3),
-             "list_sizes_identical": bool(np.array_equal(dsizes, hx.list_sizes))}
-    log(f"index build: host lists {host_build_s:.2f} s ({threads} threads), device "
-        f"(upload + sort + postings) {device_build_s:.2f} s, sizes identical "
-        f"{build['list_sizes_identical']}")
-
-    # ---- streamed index: postings in host memory, slices streamed through
-    # args.stream_slots HBM slice buffers on every call
-    streamed = None
-    if args.stream_slots > 0:
-        streamed = stream_leg(issl, hiprt, hx, guides, ix, args)
-
-    # ---- off-target extraction (the step before the index) on a synthetic genome
-    extraction = extract_leg(issl, args, masks) if args.extract_bases > 0 else None
-
-    # ---- serving: Poisson arrivals of 1,000-guide requests through the request loop
-    log(f"serving: {args.serve_requests} requests of 1,000 guides at {args.serve_rate:g} req/s")
-    srv = issl.Server(max_batch=65536)
-    sid = srv.add_index(hx)
-    srv_rng = np.random.default_rng(0x5E4E)
-    serve_poisson(issl, srv, [sid], [guides], args.serve_rate, 20, lambda r: 1000, srv_rng)
-    serving = serve_poisson(issl, srv, [sid], [guides], args.serve_rate, args.serve_requests,
-                            lambda r: 1000, srv_rng)
-    srv.close()
-    log(f"serving: p50 {serving['p50_ms']} ms, p99 {serving['p99_ms']} ms, "
-        f"mean batch {serving['mean_batch_guides']:.0f} guides")
-
-    # ---- CPU baseline: the oracle's OpenMP scorer on the host cores
-    cpu = None
-    if not args.skip_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_lib
-        n_cpu = min(args.cpu_sample, Q)
-        sample = guides[:n_cpu]
-        log(f"CPU scorer: {n_cpu} guides on {threads} threads, repeated for "
-            f">= {args.cpu_seconds:g} s")
-        t0 = time.perf_counter()
-        o = oracle_lib.score(hx, sample, issl.METHOD_MIT_AND_CFD, 0.0, threads=threads)
-        passes = 1
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            oracle_lib.score(hx, sample, issl.METHOD_MIT_AND_CFD, 0.0, threads=threads)
-            passes += 1
-        t1 = time.perf_counter()
-        cpu_rate = n_cpu * passes / (t1 - t0)
-        cpu_match = bool((o["cfd"].view(np.uint64) ==
-                          res_plain["cfd"][:n_cpu].view(np.uint64)).all())
-        cpu = {"value": round(cpu_rate, 1), "unit": "guides/s", "cores": threads,
-               "kind": "port",
-               "sample": f"the step's first {n_cpu} guides scored {passes} times, index "
-                         f"preloaded, scoring loop only ({t1 - t0:.1f} s), {cpu_model()}",
-               "plain_gpu_bit_exact": cpu_match}
-        log(f"CPU: {cpu_rate:,.0f} guides/s (GPU plain bit-exact: {cpu_match})")
+    log(f"{kernel} avg {k_us:.2f} us over {n_disp} dispatches -> {achieved:,.0f} GB/s "
+        f"(frac {achieved / HBM_PEAK_GBS:.3f})")
+    res_fp64 = rot.results()
 
     line = {
         "metric": METRIC,
@@ -602,14 +661,17 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64+f32c",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n_sites": int(hx.n_sites), "n_slices": S,
                    "slice_width": 10 if args.workload.startswith("genome") else None,
-                   "guides_per_step": Q, "method": "mitAndCfd", "threshold": 0,
-                   "max_dist": 4, "form": "batched", "precision": "fp32c",
+                   "guides_per_step": Q, "rotation_batches": R,
+                   "rotation_footprint_bytes": foot,
+                   "tuning_sample": "a further batch of the same mix, not timed",
+                   "method": "mitAndCfd", "threshold": 0, "max_dist": 4, "form": "batched",
+                   "precision": "fp64 (FP64 per-site terms and sums; integer signature work)",
                    "launch": "hipGraph replay" if args.graph else "eager",
-                   "postings_per_step": p_total,
+                   "postings_per_step": int(np.mean(p_step)),
                    "site_words": "list-residual, 4 B" if res_used else "signature-inline, 8 B",
                    "tuning": {"chunk": tn.chunk, "depth": tn.depth,
                               "grid_per_cu": tn.grid_per_cu,
@@ -617,28 +679,164 @@ def main() -> int:
                               "heavy_kpost": tn.heavy_kpost}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": kernel,
-                     "kernel_us": round(k_avg_ms * 1e3, 2),
-                     "bytes_per_launch": bytes_ref_layout,
+                     "traffic": traffic, "kernel": kernel, "kernel_us": round(k_us, 2),
+                     "kernel_dispatches_timed": n_disp,
+                     "bytes_per_launch": bytes_ref,
                      "bytes_basis": "16 B per posting (8 B entry + 8 B signature gather) "
                                     "+ 16 B list bounds per (guide, slice)",
                      "layout_bytes_per_launch": bytes_layout,
-                     "layout_GBps": round(bytes_layout / (k_avg_ms * 1e-3) / 1e9, 1),
-                     "moved_GBps": (round(traffic / (k_avg_ms * 1e-3) / 1e9, 1)
-                                    if traffic else None),
-                     "moved_frac": (round(traffic / (k_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                     "layout_GBps": round(bytes_layout / (k_us * 1e-6) / 1e9, 1),
+                     "moved_GBps": (round(traffic / (k_us * 1e-6) / 1e9, 1) if traffic else None),
+                     "moved_frac": (round(traffic / (k_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
                                     if traffic else None)},
-        "cpu_baseline": cpu,
+    }
+    if args.headline_only:
+        line["config"].update(form=args.form, precision=args.precision)
+        print(json.dumps(line), flush=True)
+        ix.close()
+        return 0
+
+    # ---- compensated FP32 (the north star's precision form), same steps and shape
+    kwc = dict(kw, precision=issl.PREC_FP32C)
+    ms_c = timed(hiprt, rot, ix, args.steps, args.warmup, **kwc)
+    kc_us, _ = kernel_us(hiprt, rot, ix, args.steps, **kwc)
+    res_fp32c = rot.results()
+    log(f"batched fp32c: {ms_c:.4f} ms/step, kernel {kc_us:.2f} us")
+
+    # ---- the other site-word layout at the same shape (FP64); same bits expected
+    other_leg = None
+    if res_built:
+        ix.set_residual(not res_used)
+        ms_o = timed(hiprt, rot, ix, args.steps, args.warmup, **kw)
+        ko_us, _ = kernel_us(hiprt, rot, ix, args.steps, **kw)
+        same_o = all(np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
+                     for a, b in zip(rot.results(), res_fp64) for k in a)
+        word_o = 8 if res_used else 4
+        lb_o = np.mean([word_o * p + 16 * S * Q for p in p_step])
+        other_leg = {"site_words": "signature-inline, 8 B" if res_used else "list-residual, 4 B",
+                     "value": round(Q / (ms_o * 1e-3), 1), "ms_per_step": round(ms_o, 4),
+                     "kernel_us": round(ko_us, 2),
+                     "layout_GBps": round(lb_o / (ko_us * 1e-6) / 1e9, 1),
+                     "same_bits": bool(same_o)}
+        log(f"other layout: {ms_o:.4f} ms/step, kernel {ko_us:.2f} us, same bits {same_o}")
+        ix.set_residual(res_used)
+
+    # ---- PLAIN form (bit-exact FP64), same rotation
+    kwp = dict(kw, form=issl.FORM_PLAIN)
+    plain_ms = timed(hiprt, rot, ix, args.plain_steps, 1, **kwp)
+    plain_k_us, _ = kernel_us(hiprt, rot, ix, args.plain_steps, **kwp)
+    res_plain = rot.results()
+    log(f"plain: {plain_ms:.4f} ms/step, kernel {plain_k_us:.1f} us")
+
+    # ---- precision against the in-order FP64 path (PLAIN), every batch
+    cat = lambda rs, f: np.concatenate([r[f] for r in rs])  # noqa: E731
+    prec = {}
+    for name, rs in (("fp64", res_fp64), ("fp32c", res_fp32c)):
+        rel = {f: np.abs(cat(rs, f) - cat(res_plain, f)) / np.abs(cat(res_plain, f))
+               for f in ("mit", "cfd")}
+        prec[name] = {"max_rel": {f: float(v.max()) for f, v in rel.items()},
+                      "mean_rel": {f: float(v.mean()) for f, v in rel.items()},
+                      "counts_identical": bool(
+                          (cat(rs, "occ_count") == cat(res_plain, "occ_count")).all() and
+                          (cat(rs, "site_count") == cat(res_plain, "site_count")).all())}
+
+    # ---- request latency: 1,000-guide requests, host buffers, one at a time (FP64)
+    g0 = batches[0]
+    lat = []
+    rng = np.random.default_rng(7)
+    for i in range(args.latency_requests):
+        lo = int(rng.integers(0, max(1, Q - 1000)))
+        t0 = time.perf_counter()
+        ix.score(g0[lo:lo + 1000], **kw)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.array(lat)
+
+    # ---- the step through host buffers (issl_score): PCIe both ways, never `value`
+    hres = ix.results(Q)
+    for _ in range(max(1, args.warmup)):
+        ix.score(g0, out=hres, **kw)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ix.score(g0, out=hres, **kw)
+    host_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    host_same = bool(np.array_equal(hres.cfd.view(np.uint64), res_fp64[0]["cfd"].view(np.uint64)))
+    log(f"host buffers: {host_ms:.3f} ms/call, {Q / host_ms * 1e3:,.0f} guides/s")
+
+    # ---- index construction on the device against the host builder above
+    t_b = time.perf_counter()
+    with issl.Index.build(sigs, occ, masks) as dix:
+        device_build_s = time.perf_counter() - t_b
+        dsizes, _ = dix.export(entries=False)
+    del occ, sigs
+    build = {"host_lists_s": round(host_build_s, 3), "host_threads": threads,
+             "device_build_s": round(device_build_s, 3),
+             "list_sizes_identical": bool(np.array_equal(dsizes, hx.list_sizes))}
+    log(f"index build: host lists {host_build_s:.2f} s, device {device_build_s:.2f} s, "
+        f"sizes identical {build['list_sizes_identical']}")
+
+    streamed = stream_leg(issl, hiprt, hx, g0, ix, args) if args.stream_slots > 0 else None
+    extraction = extract_leg(issl, args, masks) if args.extract_bases > 0 else None
+
+    # ---- serving: Poisson arrivals of 1,000-guide requests through the request loop
+    log(f"serving: {args.serve_requests} requests of 1,000 guides at {args.serve_rate:g} req/s")
+    srv = issl.Server(max_batch=65536)
+    sid = srv.add_index(hx)
+    srv_rng = np.random.default_rng(0x5E4E)
+    serve_poisson(issl, srv, [sid], [g0], args.serve_rate, 20, lambda r: 1000, srv_rng)
+    serving = serve_poisson(issl, srv, [sid], [g0], args.serve_rate, args.serve_requests,
+                            lambda r: 1000, srv_rng)
+    srv.close()
+    log(f"serving: p50 {serving['p50_ms']} ms, p99 {serving['p99_ms']} ms, "
+        f"mean batch {serving['mean_batch_guides']:.0f} guides")
+
+    # ---- 100,000 guides in one launch (FP64) on the same index
+    large = large_batch_leg(issl, hiprt, ix, hx, args, threads) if args.large_steps > 0 else None
+
+    # ---- CPU baseline: the oracle's OpenMP scorer on the host cores
+    cpu_base = None
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        n_cpu = min(args.cpu_sample, Q)
+        sample = g0[:n_cpu]
+        log(f"CPU scorer: {n_cpu} guides on {threads} threads, repeated for "
+            f">= {args.cpu_seconds:g} s")
+        t0 = time.perf_counter()
+        o = oracle_lib.score(hx, sample, issl.METHOD_MIT_AND_CFD, 0.0, threads=threads)
+        passes = 1
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            oracle_lib.score(hx, sample, issl.METHOD_MIT_AND_CFD, 0.0, threads=threads)
+            passes += 1
+        t1 = time.perf_counter()
+        cpu_rate = n_cpu * passes / (t1 - t0)
+        cpu_match = bool((o["cfd"].view(np.uint64) ==
+                          res_plain[0]["cfd"][:n_cpu].view(np.uint64)).all() and
+                         (o["mit"].view(np.uint64) ==
+                          res_plain[0]["mit"][:n_cpu].view(np.uint64)).all())
+        cpu_base = {"value": round(cpu_rate, 1), "unit": "guides/s", "cores": threads,
+                    "kind": "port",
+                    "sample": f"batch 0's first {n_cpu} guides scored {passes} times, index "
+                              f"preloaded, scoring loop only ({t1 - t0:.1f} s); {threads} OpenMP "
+                              f"threads (affinity {cpu['affinity_cpus']} CPUs, cgroup quota "
+                              f"{cpu['cgroup_cpu_quota']}, OMP_PROC_BIND={cpu['omp_proc_bind']}), "
+                              f"{cpu['model']}",
+                    "host": cpu, "plain_gpu_bit_exact": cpu_match}
+        log(f"CPU: {cpu_rate:,.0f} guides/s (GPU plain bit-exact: {cpu_match})")
+
+    c4 = serving_c4_leg(issl, hx, args, threads) if args.c4 else None
+
+    line.update({
+        "cpu_baseline": cpu_base,
+        "fp32c": {"value": round(Q / (ms_c * 1e-3), 1), "unit": "guides/s",
+                  "ms_per_step": round(ms_c, 4), "kernel_us": round(kc_us, 2),
+                  "mode": "BATCHED, FP32 per-site products, compensated FP32 lane sums"},
         "other_layout": other_leg,
         "plain": {"value": round(Q / (plain_ms * 1e-3), 1), "unit": "guides/s",
-                  "ms_per_step": round(plain_ms, 4), "kernel": "k_plain",
-                  "kernel_us": round(plain_k_ms * 1e3, 2),
-                  "achieved_GBps": round(bytes_ref_layout / (plain_k_ms * 1e-3) / 1e9, 1),
-                  "layout_GBps": round(bytes_inline / (plain_k_ms * 1e-3) / 1e9, 1),
-                  "batched_speedup": round(plain_ms / ms_per_step, 3)},
-        "precision": {"fp32c_vs_fp64_max_rel": {f: float(rel[f].max()) for f in rel},
-                      "fp32c_vs_fp64_mean_rel": {f: float(rel[f].mean()) for f in rel},
-                      "counts_identical": same_counts},
+                  "ms_per_step": round(plain_ms, 4), "kernel_us": round(plain_k_us, 2),
+                  "achieved_GBps": round(bytes_ref / (plain_k_us * 1e-6) / 1e9, 1),
+                  "batched_fp64_speedup": round(plain_ms / ms_per_step, 3),
+                  "batched_fp32c_speedup": round(plain_ms / ms_c, 3)},
+        "precision": prec,
         "host_buffers": {"value": round(Q / host_ms * 1e3, 1), "unit": "guides/s",
                          "ms_per_call": round(host_ms, 4), "calls": args.steps,
                          "same_as_device_buffers": host_same,
@@ -647,16 +845,18 @@ def main() -> int:
         "latency_ms": {"request_guides": 1000, "requests": int(lat.size),
                        "p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
-                       "mode": "one request at a time, host buffers in/out"},
-        "serving": dict(serving, request_guides=1000,
+                       "mode": "one request at a time, host buffers in/out, batched fp64"},
+        "serving": dict(serving, request_guides=1000, precision="fp32c",
                         mode="request loop, Poisson arrivals, host buffers in/out"),
+        "serving_c4": c4,
+        "large_batch": large,
         "index_build": build,
         "streamed": streamed,
         "extraction": extraction,
         "setup_s": round(setup_s, 1),
-    }
-    if cpu:
-        line["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+    })
+    if cpu_base:
+        line["gpu_vs_cpu"] = round(value / cpu_base["value"], 1)
     print(json.dumps(line), flush=True)
     ix.close()
     return 0
