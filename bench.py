@@ -61,21 +61,19 @@ def cpu_model() -> str:
 
 
 def cpu_info() -> dict:
-    """Host cores this process may use: the affinity mask, capped by a cgroup
-    CPU quota when one is set (threads beyond the quota only time-share)."""
+    """Host cores this process may use (the affinity mask), with the cgroup
+    CPU quota and OMP_NUM_THREADS recorded beside it."""
     try:
         aff = len(os.sched_getaffinity(0))
     except AttributeError:
         aff = os.cpu_count() or 1
     quota = None
     try:
-        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
-        if q != "max":
-            quota = int(q) / int(per)
-    except (OSError, ValueError):
+        quota = open("/sys/fs/cgroup/cpu.max").read().strip()
+    except OSError:
         pass
-    threads = aff if quota is None else max(1, min(aff, int(quota)))
-    return {"threads": threads, "affinity_cpus": aff, "cgroup_cpu_quota": quota,
+    return {"threads": aff, "affinity_cpus": aff, "cgroup_cpu_max": quota,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
             "system_cpus": os.cpu_count(), "model": cpu_model(),
             "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
 
@@ -473,10 +471,12 @@ def serving_c4_leg(issl, hx1, args, threads):
     ids, pools = [srv.add_index(hx1)], []
     g1 = workloads.make("genome100m_10x10", threads=threads, guide_batches=4)[3]
     pools.append(g1)
-    log("serving: generating genome300m_10x10")
+    log(f"serving: generating genome300m_10x10 with {threads} threads")
     sigs, occ, masks, g3, _ = workloads.make("genome300m_10x10", threads=threads)
+    log("serving: building the 300 M-site slice lists")
     hx3 = issl.build_lists(sigs, occ, masks, threads=threads)
     del sigs, occ
+    log("serving: admitting the 300 M-site index")
     ids.append(srv.add_index(hx3))
     pools.append(g3)
     del hx3
@@ -520,7 +520,7 @@ def main() -> int:
                     help="disjoint guide batches the timed steps rotate over")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed steps and their kernel timing (for profiling)")
-    ap.add_argument("--form", choices=["batched", "plain"], default="batched",
+    ap.add_argument("--form", choices=["batched", "plain", "ordered"], default="batched",
                     help="form of the timed steps with --headline-only (profiling)")
     ap.add_argument("--precision", choices=["fp64", "fp32c"], default="fp64",
                     help="precision of the timed steps with --headline-only (profiling)")
@@ -581,6 +581,7 @@ def main() -> int:
         return serving_main(args)
     cpu = cpu_info()
     threads = cpu["threads"]
+    log(f"host: {cpu}")
     t_setup = time.time()
     R = max(1, args.batches)
     log(f"generating workload {args.workload}: {R} timed guide batches and one held out, "
@@ -622,7 +623,8 @@ def main() -> int:
     kw = dict(method=issl.METHOD_MIT_AND_CFD, threshold=0.0, form=issl.FORM_BATCHED,
               precision=issl.PREC_FP64)
     if args.headline_only:
-        kw.update(form=issl.FORM_PLAIN if args.form == "plain" else issl.FORM_BATCHED,
+        kw.update(form={"plain": issl.FORM_PLAIN, "ordered": issl.FORM_ORDERED,
+                        "batched": issl.FORM_BATCHED}[args.form],
                   precision=issl.PREC_FP32C if args.precision == "fp32c" else issl.PREC_FP64)
     elif (args.form, args.precision) != ("batched", "fp64"):
         raise SystemExit("--form / --precision apply to --headline-only runs only")
@@ -639,7 +641,8 @@ def main() -> int:
 
     # ---- dominant kernel time, HIP events on each of K rotated steps' dispatches
     k_us, n_disp = kernel_us(hiprt, rot, ix, args.steps, **kw)
-    kernel = "k_plain" if kw["form"] == issl.FORM_PLAIN else kernel_name(issl, tn)
+    kernel = ("k_plain" if kw["form"] == issl.FORM_PLAIN else
+              "k_tiles (in order)" if kw["form"] == issl.FORM_ORDERED else kernel_name(issl, tn))
     steps_bytes = [16 * p_step[k % R] + 16 * S * Q for k in range(args.steps)]
     bytes_ref = int(np.mean(steps_bytes))
     bytes_layout = int(np.mean([word * p_step[k % R] + 16 * S * Q for k in range(args.steps)]))
@@ -727,6 +730,14 @@ def main() -> int:
     plain_k_us, _ = kernel_us(hiprt, rot, ix, args.plain_steps, **kwp)
     res_plain = rot.results()
     log(f"plain: {plain_ms:.4f} ms/step, kernel {plain_k_us:.1f} us")
+
+    # ---- ORDERED form (bit-exact FP64 in the tile kernel, one wave per guide)
+    kwo = dict(kw, form=issl.FORM_ORDERED)
+    ord_ms = timed(hiprt, rot, ix, args.steps, args.warmup, **kwo)
+    ord_k_us, _ = kernel_us(hiprt, rot, ix, args.steps, **kwo)
+    ord_same = all(np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
+                   for a, b in zip(rot.results(), res_plain) for k in a)
+    log(f"ordered: {ord_ms:.4f} ms/step, kernel {ord_k_us:.1f} us, bits of plain {ord_same}")
 
     # ---- precision against the in-order FP64 path (PLAIN), every batch
     cat = lambda rs, f: np.concatenate([r[f] for r in rs])  # noqa: E731
@@ -817,8 +828,8 @@ def main() -> int:
                     "kind": "port",
                     "sample": f"batch 0's first {n_cpu} guides scored {passes} times, index "
                               f"preloaded, scoring loop only ({t1 - t0:.1f} s); {threads} OpenMP "
-                              f"threads (affinity {cpu['affinity_cpus']} CPUs, cgroup quota "
-                              f"{cpu['cgroup_cpu_quota']}, OMP_PROC_BIND={cpu['omp_proc_bind']}), "
+                              f"threads (affinity {cpu['affinity_cpus']} CPUs, cgroup cpu.max "
+                              f"{cpu['cgroup_cpu_max']}, OMP_PROC_BIND={cpu['omp_proc_bind']}), "
                               f"{cpu['model']}",
                     "host": cpu, "plain_gpu_bit_exact": cpu_match}
         log(f"CPU: {cpu_rate:,.0f} guides/s (GPU plain bit-exact: {cpu_match})")
@@ -831,6 +842,14 @@ def main() -> int:
                   "ms_per_step": round(ms_c, 4), "kernel_us": round(kc_us, 2),
                   "mode": "BATCHED, FP32 per-site products, compensated FP32 lane sums"},
         "other_layout": other_leg,
+        "ordered": {"value": round(Q / (ord_ms * 1e-3), 1), "unit": "guides/s",
+                    "ms_per_step": round(ord_ms, 4), "kernel": "k_tiles (in order)",
+                    "kernel_us": round(ord_k_us, 2),
+                    "achieved_GBps": round(bytes_ref / (ord_k_us * 1e-6) / 1e9, 1),
+                    "same_bits_as_plain": bool(ord_same),
+                    "plain_speedup": round(plain_ms / ord_ms, 3),
+                    "mode": "ORDERED: FP64 hits added in the reference's order, one wave "
+                            "per guide in the persistent tile kernel"},
         "plain": {"value": round(Q / (plain_ms * 1e-3), 1), "unit": "guides/s",
                   "ms_per_step": round(plain_ms, 4), "kernel_us": round(plain_k_us, 2),
                   "achieved_GBps": round(bytes_ref / (plain_k_us * 1e-6) / 1e9, 1),

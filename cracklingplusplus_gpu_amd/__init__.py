@@ -20,7 +20,7 @@ from typing import Optional
 import numpy as np
 
 from . import _capi
-from ._capi import (FLAG_DISTINCT, FLAG_GRAPH, FORM_BATCHED, FORM_PLAIN, METHOD_AVG_MIT_CFD, METHOD_CFD,
+from ._capi import (FLAG_DISTINCT, FLAG_GRAPH, FORM_BATCHED, FORM_ORDERED, FORM_PLAIN, METHOD_AVG_MIT_CFD, METHOD_CFD,
                     METHOD_MIT, METHOD_MIT_AND_CFD, METHOD_MIT_OR_CFD, PREC_FP32C, PREC_FP64,
                     IsslError, check, lib)
 
@@ -28,7 +28,7 @@ __all__ = [
     "Index", "IndexArrays", "Results", "IsslError", "encode", "decode", "build_lists",
     "write_index", "read_index", "parse_masks", "count_sorted_sites", "score_tables",
     "synth", "device_count", "METHOD_MIT", "METHOD_CFD", "METHOD_MIT_AND_CFD",
-    "METHOD_MIT_OR_CFD", "METHOD_AVG_MIT_CFD", "FORM_PLAIN", "FORM_BATCHED", "PREC_FP64",
+    "METHOD_MIT_OR_CFD", "METHOD_AVG_MIT_CFD", "FORM_PLAIN", "FORM_BATCHED", "FORM_ORDERED", "PREC_FP64",
     "PREC_FP32C", "FLAG_DISTINCT", "FLAG_GRAPH", "METHOD_NAMES", "CLI_METHODS", "Server", "Request",
     "device_memory", "Extractor",
 ]

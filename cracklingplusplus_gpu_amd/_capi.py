@@ -25,7 +25,7 @@ ISSL_ERR_TIMEOUT = -7
 ISSL_ERR_STATE = -8
 
 METHOD_MIT, METHOD_CFD, METHOD_MIT_AND_CFD, METHOD_MIT_OR_CFD, METHOD_AVG_MIT_CFD = range(5)
-FORM_PLAIN, FORM_BATCHED = 0, 1
+FORM_PLAIN, FORM_BATCHED, FORM_ORDERED = 0, 1, 2
 PREC_FP64, PREC_FP32C = 0, 1
 FLAG_DISTINCT = 1
 FLAG_GRAPH = 2

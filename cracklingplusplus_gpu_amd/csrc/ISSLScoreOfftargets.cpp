@@ -14,7 +14,8 @@
 // in the working directory; five arguments are required (the reference
 // checks argc < 4 but reads argv[4] and argv[5]).  The distance limit is 4
 // whatever the third argument says, as in the reference (:53,369).
-// ISSL_FORM=batched selects the BATCHED form.
+// Scores in the ORDERED form (the reference's FP64 sums bit for bit);
+// ISSL_FORM=batched selects the BATCHED form, ISSL_FORM=plain the PLAIN kernel.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -77,8 +78,7 @@ int main(int argc, char **argv) {
     p.max_dist = 4;
     p.method = method;
     p.threshold = threshold;
-    const char *form = std::getenv("ISSL_FORM");
-    p.form = (form && std::string(form) == "batched") ? ISSL_FORM_BATCHED : ISSL_FORM_PLAIN;
+    p.form = issl_form_from_env();
     p.precision = ISSL_PREC_FP64;
     p.flags = ISSL_FLAG_DISTINCT;
     issl_results r{};

@@ -96,8 +96,7 @@ void ISSLScoringStage::score(std::vector<guideResults> &candidateGuides, bool st
     p.max_dist = 4;
     p.method = static_cast<int32_t>(config.method);
     p.threshold = config.scoreThreshold;
-    const char *form = std::getenv("ISSL_FORM");
-    p.form = (form && std::string(form) == "batched") ? ISSL_FORM_BATCHED : ISSL_FORM_PLAIN;
+    p.form = issl_form_from_env();
     p.precision = ISSL_PREC_FP64;
     issl_results r{};
     r.mit = mitv.data();
@@ -115,6 +114,14 @@ void ISSLScoringStage::score(std::vector<guideResults> &candidateGuides, bool st
     }
     std::cout << "\t" << with_commas(failed) << " of " << with_commas(n) << " failed here."
               << std::endl;
+}
+
+int issl_form_from_env() {
+    const char *form = std::getenv("ISSL_FORM");
+    const std::string f = form ? form : "";
+    if (f == "batched") return ISSL_FORM_BATCHED;
+    if (f == "plain") return ISSL_FORM_PLAIN;
+    return ISSL_FORM_ORDERED;
 }
 
 ISSLScoringModule::ISSLScoringModule(const cracklingConfig &cfg) : ISSLScoringStage(cfg) {}

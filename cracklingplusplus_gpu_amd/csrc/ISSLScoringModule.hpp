@@ -9,9 +9,11 @@
 //     (the reference re-reads the file inside every run,
 //     lib/ISSLScoringModule.cpp:72-150);
 //   * config.offTarget.threads does not apply (no OpenMP team);
-//   * ISSL_FORM=batched in the environment selects the BATCHED form
-//     (FP64), otherwise the PLAIN form gives the reference's FP64 sums bit
-//     for bit.
+//   * the ORDERED form scores by default: the reference's FP64 sums, counts
+//     and exit point bit for bit; ISSL_FORM=batched in the environment
+//     selects the BATCHED form (FP64, sums regrouped by tile) and
+//     ISSL_FORM=plain the one-guide-per-workgroup PLAIN kernel (the same bits
+//     as ORDERED).
 #pragma once
 
 #include <string>
@@ -20,6 +22,10 @@
 #include "crackling_types.hpp"
 
 struct issl_index;
+
+// Scoring form from the ISSL_FORM environment variable: "batched", "plain",
+// else ORDERED (an ISSL_FORM_* value of issl_gpu.h).
+int issl_form_from_env();
 
 // The part both pipeline modules share: configuration, the processGuide
 // gating (lib/ISSLScoringModule.cpp:488-513), the index handle and the

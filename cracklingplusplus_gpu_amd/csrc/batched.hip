@@ -334,6 +334,7 @@ struct PlanArgs {
     int S;
     uint64_t n_sites;
     uint32_t chunk;
+    uint32_t first_min;           // single tiles longer than this join the first list
     int s0, s1;                   // slice window: lists outside it are empty
     const GuideState *state_in;   // guides that took the exit visit nothing
     BatchWs ws;
@@ -407,9 +408,9 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         // Append the workgroup's tiles to the queue: one atomic per list per
         // workgroup, on the counter of its region.  The first list takes the
         // tiles likely to cost most (several-tile guides, and single tiles
-        // longer than half a chunk), so that they start early and the cheap
-        // ones fill the end.
-        const bool first = K > 1 || (K == 1 && P > a.chunk / 2);
+        // longer than first_min, half a chunk), so that they start early and
+        // the cheap ones fill the end.
+        const bool first = K > 1 || (K == 1 && P > a.first_min);
         const uint32_t ka = (lead && first) ? K : 0, kb = (lead && !first && K == 1) ? 1 : 0;
         uint32_t ia = ka, ib = kb;
 #pragma unroll
@@ -518,8 +519,27 @@ constexpr uint32_t kHitRing = 128;
 // is 1.0 (p = 31 lies past the sequence) and the MIT rank terms are read by
 // the same index (0 at kAbsent7).
 constexpr uint32_t kAbsent7 = 127;
-template <int PREC, int DEPTH, bool QUEUE, bool RES>
+
+// Value of lane `src` (wave-uniform) of a double.
+__device__ __forceinline__ double readlane_f64(double v, uint32_t src) {
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b)), static_cast<int>(src)));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b >> 32)), static_cast<int>(src)));
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+// ORD: every guide is a single tile (the planner's tile length covers the
+// longest guide), and each batch of hits is added to two wave-uniform FP64
+// sums one hit at a time, in hit-ring order -- the guide's slice-major
+// posting order -- with the early-exit test after each hit, as
+// lib/ISSLScoringModule.cpp:231-401 adds and stops.  Sums, counts and the
+// exit point are the reference's bit for bit; after the exit the wave stops
+// streaming the guide.  (FP64 and the tile queue only.)
+template <int PREC, int DEPTH, bool QUEUE, bool RES, bool ORD = false>
 __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
+    static_assert(!ORD || (PREC == ISSL_PREC_FP64 && QUEUE), "in-order tiles are FP64 and queued");
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
     // Site word as streamed and ringed; the hit ring holds the word (or, in
     // RES form, the clipped occurrence count).
@@ -761,6 +781,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         F2 fm{0.f, 0.f}, fc{0.f, 0.f};
         uint64_t occ_sum = 0;
         uint32_t sites = 0;   // hits of this tile in this lane (< 2^26)
+        // ORD: the guide's running sums (wave-uniform) and whether the exit
+        // has been taken (then nothing more of the guide is scored).
+        double run_m = 0.0, run_c = 0.0;
+        bool stop = false;
 
         // Candidates (dist <= 4) wait in the ring in position order.  They are
         // checked 64 at a time against the first-seen rule, one per lane; the
@@ -771,6 +795,14 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         uint32_t hhead = 0, htail = 0;  // hit ring cursors
         auto score_hits = [&](uint32_t cnt) {
             __builtin_amdgcn_wave_barrier();
+            if constexpr (ORD) {
+                if (stop) {   // past the exit: nothing more counts
+                    hhead += cnt;
+                    return;
+                }
+            }
+            uint32_t h_occ = 0, h_id = 0;
+            Real h_tm = 0, h_tc = 0;
             if (static_cast<uint32_t>(lane) < cnt) {
                 const uint32_t e = (hhead + lane) & (kHitRing - 1);
                 const uint32_t ent = s_hpos[wave][e];
@@ -795,13 +827,16 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                         a0 + (ent & ((1u << kRelBits) - 1)) + s_delta[wave][buf][sl];
                     const uint64_t pe = a.post[addr];
                     occ = static_cast<uint32_t>(pe >> 32);
-                    if (a.mark_distinct)
-                        atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
-                                     (static_cast<uint32_t>(pe) >> 6),
-                                 1ULL << (static_cast<uint32_t>(pe) & 63));
+                    h_id = static_cast<uint32_t>(pe);
+                    if (!ORD && a.mark_distinct)
+                        atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (h_id >> 6),
+                                 1ULL << (h_id & 63));
                 }
-                occ_sum += occ;
-                sites += 1;
+                if constexpr (!ORD) {
+                    occ_sum += occ;
+                    sites += 1;
+                }
+                h_occ = occ;
                 // Keep occ a 32-bit value: otherwise the compiler widens it
                 // back to the 64-bit word it was cut from and converts that.
                 asm("" : "+v"(occ));
@@ -834,7 +869,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     tc = (dist == 0 ? Real(1) : c) * ro;
                 }
                 if (a.calc_mit && dist > 0) tm = mv * ro;
-                if constexpr (PREC == ISSL_PREC_FP32C) {
+                if constexpr (ORD) {
+                    h_tm = tm;
+                    h_tc = tc;
+                } else if constexpr (PREC == ISSL_PREC_FP32C) {
                     fm = f2_add(fm, tm);
                     fc = f2_add(fc, tc);
                 } else {
@@ -842,11 +880,45 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     ac += tc;
                 }
             }
+            if constexpr (ORD) {
+                // The hits in order, one at a time: lane i holds hit i.
+                uint32_t acc = cnt;
+                if (a.sc.use_cut) {
+                    for (uint32_t i = 0; i < cnt; ++i) {
+                        run_m += readlane_f64(static_cast<double>(h_tm), i);
+                        run_c += readlane_f64(static_cast<double>(h_tc), i);
+                        if (bexit(a.sc.method, run_m, run_c, a.sc.cut)) {
+                            acc = i + 1;
+                            stop = true;
+                            break;
+                        }
+                    }
+                } else {
+                    for (uint32_t i = 0; i < cnt; ++i) {
+                        run_m += readlane_f64(static_cast<double>(h_tm), i);
+                        run_c += readlane_f64(static_cast<double>(h_tc), i);
+                    }
+                }
+                // Counts and distinct sites of the hits up to the exit.
+                if (static_cast<uint32_t>(lane) < acc) {
+                    occ_sum += h_occ;
+                    sites += 1;
+                    if (a.mark_distinct)
+                        atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (h_id >> 6),
+                                 1ULL << (h_id & 63));
+                }
+            }
             hhead += cnt;
             __builtin_amdgcn_wave_barrier();
         };
         auto drain = [&](uint32_t cnt) {
             __builtin_amdgcn_wave_barrier();
+            if constexpr (ORD) {
+                if (stop) {   // past the exit: the candidates are not scored
+                    head += cnt;
+                    return;
+                }
+            }
             bool hit = false;
             uint32_t ent = 0, pk = 0;
             Word wv = 0;
@@ -997,14 +1069,20 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             }
         };
         uint32_t base = a0;
-        for (; base + 64 * DEPTH <= a1; base += 64 * DEPTH) round(base, std::true_type{});
-        if (base < a1) round(base, std::false_type{});
+        for (; base + 64 * DEPTH <= a1; base += 64 * DEPTH) {
+            if (ORD && stop) break;
+            round(base, std::true_type{});
+        }
+        if (base < a1 && !(ORD && stop)) round(base, std::false_type{});
         while (tail != head) drain(std::min<uint32_t>(64, tail - head));
         if (htail != hhead) score_hits(htail - hhead);
         // Lane partials as doubles (a compensated pair converts exactly),
         // combined in a fixed butterfly: every lane ends with the same sums.
         double dm, dc;
-        if constexpr (PREC == ISSL_PREC_FP32C) {
+        if constexpr (ORD) {
+            dm = 0.0;   // the sums are run_m, run_c; the butterfly reduces the counts
+            dc = 0.0;
+        } else if constexpr (PREC == ISSL_PREC_FP32C) {
             dm = static_cast<double>(fm.s) + static_cast<double>(fm.c);
             dc = static_cast<double>(fc.s) + static_cast<double>(fc.c);
         } else {
@@ -1017,6 +1095,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         tile_reduce_step<4>(dm, dc, occ_sum, sites);
         tile_reduce_step<2>(dm, dc, occ_sum, sites);
         tile_reduce_step<1>(dm, dc, occ_sum, sites);
+        if constexpr (ORD) {
+            dm = run_m;
+            dc = run_c;
+        }
         if (lane == 0) {
             TilePart tp;
             tp.m = dm;
@@ -1608,6 +1690,26 @@ int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st, 
     return ISSL_OK;
 }
 
+// The in-order tile kernel (FP64, queued), either site-word layout.
+int launch_ordered_depth(const TileArgs &ta, int depth, int grid, hipStream_t st, hipEvent_t e0,
+                         hipEvent_t e1) {
+    const dim3 g(grid), b(kTileBlock);
+    constexpr int P = ISSL_PREC_FP64;
+    if (!ta.ws.tlist) return fail(ISSL_ERR_STATE, "in-order tiles need the tile queue");
+    if (ta.res && ta.ws.gres) {
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<P, 4, true, true, true>), g, b, 0, st, e0, e1, 0, ta);
+        else if (depth == 8) hipExtLaunchKernelGGL((k_tiles<P, 8, true, true, true>), g, b, 0, st, e0, e1, 0, ta);
+        else if (depth == 16) hipExtLaunchKernelGGL((k_tiles<P, 16, true, true, true>), g, b, 0, st, e0, e1, 0, ta);
+        else return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
+    } else {
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<P, 4, true, false, true>), g, b, 0, st, e0, e1, 0, ta);
+        else if (depth == 8) hipExtLaunchKernelGGL((k_tiles<P, 8, true, false, true>), g, b, 0, st, e0, e1, 0, ta);
+        else if (depth == 16) hipExtLaunchKernelGGL((k_tiles<P, 16, true, false, true>), g, b, 0, st, e0, e1, 0, ta);
+        else return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
+    }
+    return ISSL_OK;
+}
+
 template <int PREC>
 int launch_direct_depth(const StreamArgs &da, int depth, uint32_t grid, hipStream_t st,
                         hipEvent_t e0, hipEvent_t e1) {
@@ -1644,14 +1746,19 @@ DevResults offset_results(const DevResults &o, uint64_t i) {
     return r;
 }
 
-// One launch sequence over n <= kMaxBatch guides.
+// One launch sequence over n <= kMaxBatch guides.  ordered: the in-order
+// form (one tile of length ord_chunk per guide, FP64 sums in visit order,
+// the exit taken by the tile kernel); else BATCHED.
 int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                  int precision, const issl_tuning &tn, const DevResults &out, uint64_t *distinct,
-                 Workspace &ws, hipStream_t st, const Window &win) {
+                 Workspace &ws, hipStream_t st, const Window &win, bool ordered = false,
+                 uint32_t ord_chunk = 0) {
     const int S = static_cast<int>(ix->S);
     const bool carry = win.state_in || win.state_out;
+    if (ordered && (carry || precision != ISSL_PREC_FP64))
+        return fail(ISSL_ERR_STATE, "the in-order form is FP64 over a resident index");
     // Guide streams (k_stream) read the resident index only.
-    const bool direct = tn.heavy_kpost != 0 && !carry && win.s0 == 0 && win.s1 == S;
+    const bool direct = !ordered && tn.heavy_kpost != 0 && !carry && win.s0 == 0 && win.s1 == S;
     // The tile kernel streams the list-residual words when the window has
     // them and the planner's queue feeds it.
     const bool res = win.res != nullptr && !direct;
@@ -1666,7 +1773,10 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     const bool fp32 = precision == ISSL_PREC_FP32C;
     const void *mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
     const void *cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
-    const int finish_single = (!carry && !sc.use_cut) ? 1 : 0;
+    // A guide's single tile finishes it when no exit is armed (its sum is the
+    // tile's partial) and always in the in-order form (the tile kernel takes
+    // the exit itself).
+    const int finish_single = (!carry && (!sc.use_cut || ordered)) ? 1 : 0;
     const bool tiles = !direct || tn.heavy_kpost != kHeavyNever;
     const uint32_t gw = static_cast<uint32_t>((n + 3) / 4);
 
@@ -1706,8 +1816,9 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
         }
         // (launched below with the tile kernel, the two timed together)
     } else {
-        PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, static_cast<uint32_t>(tn.chunk),
-                    win.s0, win.s1, win.state_in, b};
+        const uint32_t plan_chunk = ordered ? ord_chunk : static_cast<uint32_t>(tn.chunk);
+        PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, plan_chunk,
+                    static_cast<uint32_t>(tn.chunk) / 2, win.s0, win.s1, win.state_in, b};
         const uint64_t G = kPlanWaves * plan_guides_per_wave(S);
         hipLaunchKernelGGL(k_plan, dim3(static_cast<uint32_t>((n + G - 1) / G)),
                            dim3(kPlanBlock), 0, st, pa);
@@ -1724,7 +1835,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     ta.cfd = cfd;
     ta.n = static_cast<uint32_t>(n);
     ta.S = S;
-    ta.mark_distinct = (distinct && !sc.use_cut) ? 1 : 0;
+    // (with an exit armed, BATCHED marks in the finalizer, up to the exit)
+    ta.mark_distinct = (distinct && (!sc.use_cut || ordered)) ? 1 : 0;
     ta.calc_mit = sc.calc_mit;
     ta.calc_cfd = sc.calc_cfd;
     ta.distinct = distinct;
@@ -1736,6 +1848,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
     grid = std::min<uint64_t>(grid, (n * kMaxTiles + kTileWaves - 1) / kTileWaves);
     auto tiles_launch = [&](hipEvent_t e0, hipEvent_t e1) {
+        if (ordered) return launch_ordered_depth(ta, tn.depth, static_cast<int>(grid), st, e0, e1);
         return fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st, e0, e1)
                     : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st, e0, e1);
     };
@@ -1767,7 +1880,9 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     fa.state_in = win.state_in;
     fa.state_out = win.state_out;
     fa.ws = b;
-    if (!sc.use_cut)
+    // (in order: the tile kernel finished every guide with postings; this
+    // writes those without any and sets the queue counters back to zero)
+    if (!sc.use_cut || ordered)
         hipLaunchKernelGGL(k_finalize_sum, dim3(static_cast<uint32_t>((n + 63) / 64)), dim3(64), 0, st, fa);
     else if (fp32)
         hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP32C>), dim3(gw), dim3(256), 0, st, fa);
@@ -1789,6 +1904,40 @@ issl_tuning default_tuning() {
     t.direct_depth = 8;
     t.heavy_kpost = 0;  // every guide on the tile kernel
     return t;
+}
+
+// Longest guide the in-order form takes in one tile: tile positions are
+// kRelBits wide.
+constexpr uint64_t kOrdMaxPostings = (1ULL << kRelBits) - 64;
+
+bool ordered_fits(const issl_index *ix) {
+    return ix->d_inline && !ix->src && ix->p_max <= kOrdMaxPostings;
+}
+
+int launch_ordered(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
+                   const issl_tuning &tn_in, const DevResults &out, uint64_t *distinct,
+                   Workspace &ws, hipStream_t st) {
+    if (n == 0) return ISSL_OK;
+    if (!ordered_fits(ix)) return fail(ISSL_ERR_STATE, "index too long for the in-order tiles");
+    issl_tuning tn = tn_in;
+    if (tn.chunk <= 0) tn = default_tuning();
+    // One tile per guide: the tile covers the longest possible guide.
+    const uint32_t ord_chunk =
+        static_cast<uint32_t>(std::max<uint64_t>(64, (ix->p_max + 63) / 64 * 64));
+    Window w;
+    w.s0 = 0;
+    w.s1 = static_cast<int>(ix->S);
+    w.post = ix->d_post;
+    w.inl = ix->d_inline;
+    w.res = ix->res_use ? ix->d_res : nullptr;
+    w.res_shift = ix->res_shift;
+    for (uint64_t i = 0; i < n; i += kMaxBatch) {
+        const uint64_t m = std::min<uint64_t>(kMaxBatch, n - i);
+        const int rc = launch_batch(ix, d_guides + i, m, sc, ISSL_PREC_FP64, tn, offset_results(out, i),
+                                    distinct, ws, st, w, true, ord_chunk);
+        if (rc != ISSL_OK) return rc;
+    }
+    return ISSL_OK;
 }
 
 int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,

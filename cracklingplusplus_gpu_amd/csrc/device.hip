@@ -595,12 +595,12 @@ int check_params(const issl_index *ix, const issl_params *p, ScoreScalars *sc) {
         return fail(ISSL_ERR_ARG,
                     "max_dist must be 4: the reference scores dist <= 4 regardless of its argument");
     if (p->method < 0 || p->method > 4) return fail(ISSL_ERR_ARG, "Scoring method is not valid.");
-    if (p->form != ISSL_FORM_PLAIN && p->form != ISSL_FORM_BATCHED)
+    if (p->form != ISSL_FORM_PLAIN && p->form != ISSL_FORM_BATCHED && p->form != ISSL_FORM_ORDERED)
         return fail(ISSL_ERR_ARG, "unknown form");
     if (p->precision != ISSL_PREC_FP64 && p->precision != ISSL_PREC_FP32C)
         return fail(ISSL_ERR_ARG, "unknown precision");
-    if (p->form == ISSL_FORM_PLAIN && p->precision != ISSL_PREC_FP64)
-        return fail(ISSL_ERR_ARG, "the PLAIN form computes in FP64 only");
+    if (p->form != ISSL_FORM_BATCHED && p->precision != ISSL_PREC_FP64)
+        return fail(ISSL_ERR_ARG, "the PLAIN and ORDERED forms compute in FP64 only");
     sc->method = p->method;
     sc->calc_mit = p->method != ISSL_METHOD_CFD;
     sc->calc_cfd = p->method != ISSL_METHOD_MIT;
@@ -1112,9 +1112,7 @@ int score_on_device(issl_index *ix, const uint64_t *d_guides, uint64_t n, const 
                     const ScoreScalars &sc, const DevResults &o, hipStream_t st) {
     uint64_t *distinct = (p->flags & ISSL_FLAG_DISTINCT) ? ix->d_distinct : nullptr;
     if (ix->src) return score_streamed(ix, d_guides, n, p, sc, o, distinct, st);
-    if (p->form == ISSL_FORM_PLAIN) return launch_plain(ix, d_guides, n, sc, o, distinct, nullptr, nullptr, st);
-    issl_tuning tn = ix->tuning;
-    return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, tn, o, distinct, ix->ws, st);
+    return score_with(ix, d_guides, n, p, sc, ix->tuning, o, ix->ws, st);
 }
 
 }  // namespace
@@ -1130,7 +1128,10 @@ int score_with(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_
                hipStream_t st) {
     uint64_t *distinct = (p->flags & ISSL_FLAG_DISTINCT) ? ix->d_distinct : nullptr;
     if (ix->src) return fail(ISSL_ERR_ARG, "a streamed index is scored through issl_score or issl_score_device");
-    if (p->form == ISSL_FORM_PLAIN) return launch_plain(ix, d_guides, n, sc, o, distinct, nullptr, nullptr, st);
+    if (p->form == ISSL_FORM_ORDERED && ordered_fits(ix))
+        return launch_ordered(ix, d_guides, n, sc, tn, o, distinct, ws, st);
+    if (p->form != ISSL_FORM_BATCHED)
+        return launch_plain(ix, d_guides, n, sc, o, distinct, nullptr, nullptr, st);
     return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, tn, o, distinct, ws, st);
 }
 

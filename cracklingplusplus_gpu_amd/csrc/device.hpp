@@ -232,6 +232,14 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
                    int precision, int chunk, const issl_tuning &tn, const DevResults &out,
                    uint64_t *distinct, Workspace &ws, hipStream_t stream,
                    const Window *win = nullptr);
+// The in-order form (ISSL_FORM_ORDERED): the planner, the tile kernel with
+// one tile per guide and FP64 sums added hit by hit in the reference's visit
+// order, and the finalizer for guides without postings.  ordered_fits: the
+// index is resident and no guide can visit 2^26 - 64 postings or more.
+bool ordered_fits(const issl_index *ix);
+int launch_ordered(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
+                   const issl_tuning &tn, const DevResults &out, uint64_t *distinct,
+                   Workspace &ws, hipStream_t stream);
 int build_inline_layout(issl_index *ix);
 
 // The index's scoring workspace is shared by all its calls: acquire makes a

@@ -51,9 +51,16 @@ extern "C" {
  *             reference's early-exit point).
  *   BATCHED : persistent waves over (guide, chunk) tiles of the guide's
  *             slice-major posting sequence, sites streamed from a
- *             signature-inline device layout; precision selectable. */
+ *             signature-inline device layout; precision selectable.
+ *   ORDERED : the BATCHED launch structure with one tile, and so one wave,
+ *             per guide: FP64 terms added one hit at a time in the
+ *             reference's order with the exit test after each, so results
+ *             are PLAIN's bit for bit (FP64 only).  A streamed index, or one
+ *             whose guides could visit 2^26 - 64 postings or more, is scored
+ *             by the PLAIN kernel instead (the same bits). */
 #define ISSL_FORM_PLAIN    0
 #define ISSL_FORM_BATCHED  1
+#define ISSL_FORM_ORDERED  2
 
 /* Precision of the BATCHED form (PLAIN is always FP64).
  *   FP64  : FP64 per-site terms and sums.

@@ -200,6 +200,26 @@ static double cfd_site(uint64_t q, uint64_t ot, uint64_t dist)
     return s;
 }
 
+/* Relative distance of the exit tests' operands from the cut
+ * (ISSLScoringModule.cpp:365-395): how far the running sums are from
+ * flipping a test.  0 when an operand equals the cut; +inf for an infinite
+ * cut (threshold 0) and for a cut of 0 (threshold 100): a sum of nonnegative
+ * terms is 0 or positive whatever order adds them. */
+static double exit_distance(int method, double m, double c, double cut)
+{
+    if (isinf(cut) || cut == 0.0)
+        return HUGE_VAL;
+    double d;
+    switch (method) {
+    case OR_MIT_AND_CFD:
+    case OR_MIT_OR_CFD: d = fmin(fabs(m - cut), fabs(c - cut)); break;
+    case OR_AVG:        d = fabs((m + c) / 2.0 - cut); break;
+    case OR_MIT:        d = fabs(m - cut); break;
+    default:            d = fabs(c - cut); break;
+    }
+    return d / fabs(cut);
+}
+
 /* Exit tests of ISSLScoringModule.cpp:365-395. */
 static int exit_now(int method, double m, double c, double cut)
 {
@@ -219,7 +239,8 @@ int or_score(const uint64_t *sigs, uint64_t n_sites, uint64_t seq_len,
              int method, double threshold, int threads,
              double *mit_out, double *cfd_out, uint64_t *occ_out,
              uint64_t *site_out, char *pass_out,
-             double *mit_sum_out, double *cfd_sum_out, uint64_t *distinct)
+             double *mit_sum_out, double *cfd_sum_out, uint64_t *distinct,
+             double *margin_out)
 {
     if (method < 0 || method > 4)
         return -1;
@@ -270,6 +291,7 @@ int or_score(const uint64_t *sigs, uint64_t n_sites, uint64_t seq_len,
             double tot_mit = 0.0, tot_cfd = 0.0;
             uint64_t occ_sum = 0, site_sum = 0;
             const double cut = (10000.0 - threshold * 100) / threshold;
+            double margin = HUGE_VAL;
             int go_on = 1;
             for (uint64_t s = 0; s < n_slices && go_on; ++s) {
                 uint64_t key = 0;
@@ -301,6 +323,11 @@ int or_score(const uint64_t *sigs, uint64_t n_sites, uint64_t seq_len,
                         tot_mit += mit_lookup(&mm, fold) * (double)occ;
                     if (calc_cfd)
                         tot_cfd += cfd_site(q, sigs[id], dist) * (double)occ;
+                    if (margin_out) {
+                        const double d = exit_distance(method, tot_mit, tot_cfd, cut);
+                        if (d < margin)
+                            margin = d;
+                    }
                     if (exit_now(method, tot_mit, tot_cfd, cut)) {
                         go_on = 0;
                         break;
@@ -324,6 +351,7 @@ int or_score(const uint64_t *sigs, uint64_t n_sites, uint64_t seq_len,
             if (pass_out) pass_out[gi] = reject ? '0' : '1';
             if (mit_sum_out) mit_sum_out[gi] = tot_mit;
             if (cfd_sum_out) cfd_sum_out[gi] = tot_cfd;
+            if (margin_out) margin_out[gi] = margin;
             memset(seen, 0, words * sizeof(uint64_t)); /* ISSLScoringModule.cpp:461 */
         }
         free(seen);

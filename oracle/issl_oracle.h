@@ -72,6 +72,13 @@ void or_cfd_tables(double *pos320, double *pam16);
  *   mit_sum/cfd_sum : the raw sums before the transform
  *   distinct  : (n_sites/64+1)-word bitmap, bit id set for every scored
  *               site (ISSLScoreOfftargets.cpp:375-377)
+ *   margin    : over every hit up to the exit, the smallest relative
+ *               distance |operand - cut| / |cut| of the exit test's operands
+ *               (the sums, or their mean for avgMitCfd) from the cut; a sum
+ *               order that differs only by rounding can move the exit only
+ *               for guides whose margin is within that rounding.  +inf with
+ *               the threshold disabled and for threshold 100 (cut 0: sums
+ *               of nonnegative terms are 0 or positive in any order).
  * threads <= 0 uses the OpenMP default.  Returns 0, or -1 on a bad method
  * or index. */
 int or_score(const uint64_t *sigs, uint64_t n_sites, uint64_t seq_len,
@@ -81,7 +88,8 @@ int or_score(const uint64_t *sigs, uint64_t n_sites, uint64_t seq_len,
              int method, double threshold, int threads,
              double *mit, double *cfd, uint64_t *occ_count,
              uint64_t *site_count, char *pass,
-             double *mit_sum, double *cfd_sum, uint64_t *distinct);
+             double *mit_sum, double *cfd_sum, uint64_t *distinct,
+             double *margin);
 
 /* Brute-force candidate set of one guide over every site:
  *   { id : popcount(fold(q ^ sig[id])) <= 4  and  exists s with

@@ -12,6 +12,39 @@ for p in (ROOT, TESTS):
 
 GOLDEN = os.path.join(TESTS, "golden")
 GOLDEN_CASES = ("legacy39_5x4", "tiny4k_5x4", "mid50k_10x10", "small200k_8x8", "uniform1m_5x4")
+# Thresholds of the golden vectors: 0 disables the exit; 75; 100 (cut 0);
+# 99.99 (cut just above 0); 1 (cut 9,900); -5 (cut -2,100: exit at the first hit).
+GOLDEN_THRESHOLDS = (0.0, 75.0, 100.0, 99.99, 1.0, -5.0)
+
+
+def tkey(t: float) -> str:
+    """Key of a threshold in the golden files (tests/golden/make_golden.py)."""
+    return str(int(t)) if t == int(t) else repr(t)
+
+
+def cut_of(t: float) -> float:
+    """maximum_sum of lib/ISSLScoringModule.cpp:225."""
+    return (10000.0 - t * 100) / t
+
+
+def call_margin(o: dict, method: int, t: float):
+    """Relative distance of the pass/fail test's operands from the threshold
+    (lib/ISSLScoringModule.cpp:404-455), from oracle scores: a call can differ
+    between sum orders only where this is within their rounding."""
+    vm, vc = o["mit"], o["cfd"]
+    if t == 0 or cut_of(t) == 0:
+        # no call rejects at t = 0; at t = 100 a score is below 100 exactly
+        # when its sum is positive, which no order of nonnegative terms changes
+        return np.full(vm.shape, np.inf)
+    if method in (2, 3):
+        d = np.minimum(np.abs(vm - t), np.abs(vc - t))
+    elif method == 4:
+        d = np.abs((vm + vc) / 2 - t)
+    elif method == 0:
+        d = np.abs(vm - t)
+    else:
+        d = np.abs(vc - t)
+    return d / abs(t)
 
 
 def pytest_configure(config):

@@ -5,7 +5,9 @@ Runs in the development container only: it drives oracle/_ref/issl_ref_harness
 from /root/reference by `make -C oracle ref`) on indexes made by this
 package, and stores the reference's per-guide outputs
 (mitOfftargetscore, cfdOfftargetscore, passedOffTargetScore) for every
-method and for threshold 0 (early exit disabled) and 75.
+method and for thresholds 0 (early exit disabled), 75, 100 (cut 0: exit at
+the first positive sum), 99.99 (cut just above 0), 1 (cut 9,900) and -5
+(cut -2,100: exit at the first hit).
 
 Also copies two data files of the reference's own tests
 (tests/data/test_genome_offTargets.fa and the legacy-layout
@@ -33,7 +35,12 @@ from cracklingplusplus_gpu_amd import workloads  # noqa: E402
 
 REF = "/root/reference"
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "issl_ref_harness")
-THRESHOLDS = (0.0, 75.0)
+THRESHOLDS = (0.0, 75.0, 100.0, 99.99, 1.0, -5.0)
+
+
+def tkey(t: float) -> str:
+    """Key of a threshold in the golden files: 0, 75, 100, 99.99, 1, -5."""
+    return str(int(t)) if t == int(t) else repr(t)
 METHODS = (0, 1, 2, 3, 4)
 
 
@@ -91,7 +98,7 @@ def golden_case(name: str, ix: issl.IndexArrays, guides: np.ndarray, index_path:
         for m in METHODS:
             for t in THRESHOLDS:
                 mit, cfd, passed = run_ref(index_path, gtxt, m, t, len(guides))
-                key = f"m{m}_t{int(t)}"
+                key = f"m{m}_t{tkey(t)}"
                 data[key + "_mit"] = mit
                 data[key + "_cfd"] = cfd
                 data[key + "_pass"] = passed

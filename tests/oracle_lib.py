@@ -37,7 +37,7 @@ def lib():
         L.or_score.argtypes = [c_void_p, c_uint64, c_uint64, c_void_p, c_uint64, c_void_p,
                                c_void_p, c_void_p, c_uint64, c_int, c_double, c_int,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                               c_void_p, c_void_p]
+                               c_void_p, c_void_p, c_void_p]
         L.or_extract.restype = c_uint64
         L.or_extract.argtypes = [c_char_p, c_uint64, c_void_p, c_uint64]
         L.or_candidates_brute.restype = c_uint64
@@ -65,8 +65,10 @@ def cfd_tables():
     return p, a
 
 
-def score(ix, guides, method: int, threshold: float, threads: int = 0, distinct: bool = False):
-    """Oracle scores; returns a dict of per-guide arrays (and the distinct bitmap)."""
+def score(ix, guides, method: int, threshold: float, threads: int = 0, distinct: bool = False,
+          margin: bool = False):
+    """Oracle scores; returns a dict of per-guide arrays (and the distinct bitmap,
+    and the exit margin of or_score when asked)."""
     g = np.ascontiguousarray(guides, dtype=np.uint64)
     n = g.shape[0]
     out = {k: np.zeros(n) for k in ("mit", "cfd", "mit_sum", "cfd_sum")}
@@ -74,15 +76,18 @@ def score(ix, guides, method: int, threshold: float, threads: int = 0, distinct:
     out["site_count"] = np.zeros(n, dtype=np.uint64)
     out["pass"] = np.zeros(n, dtype=np.uint8)
     bm = np.zeros(ix.n_sites // 64 + 1, dtype=np.uint64) if distinct else None
+    mg = np.zeros(n) if margin else None
     rc = lib().or_score(_p(ix.sigs), ix.n_sites, ix.seq_len, _p(ix.masks), ix.n_slices,
                         _p(ix.list_sizes), _p(ix.entries), _p(g), n, method, threshold, threads,
                         _p(out["mit"]), _p(out["cfd"]), _p(out["occ_count"]),
                         _p(out["site_count"]), _p(out["pass"]), _p(out["mit_sum"]),
-                        _p(out["cfd_sum"]), _p(bm))
+                        _p(out["cfd_sum"]), _p(bm), _p(mg))
     if rc != 0:
         raise ValueError("oracle rejected the arguments")
     if distinct:
         out["distinct"] = bm
+    if margin:
+        out["margin"] = mg
     return out
 
 

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as orc
-from conftest import GOLDEN, GOLDEN_CASES, golden_index, load_golden
+from conftest import GOLDEN, GOLDEN_CASES, GOLDEN_THRESHOLDS, cut_of, golden_index, load_golden, tkey
 
 
 def test_encode_hand_vectors():
@@ -84,9 +84,9 @@ def test_oracle_matches_reference_goldens(name):
     gd = load_golden(name)
     ix = golden_index(name)
     g = gd["guides"]
-    for m, t in itertools.product(range(5), (0, 75)):
-        r = orc.score(ix, g, m, float(t))
-        key = f"m{m}_t{t}"
+    for m, t in itertools.product(range(5), GOLDEN_THRESHOLDS):
+        r = orc.score(ix, g, m, t)
+        key = f"m{m}_t{tkey(t)}"
         assert (r["mit"].view(np.uint64) == gd[key + "_mit"].view(np.uint64)).all(), key
         assert (r["cfd"].view(np.uint64) == gd[key + "_cfd"].view(np.uint64)).all(), key
         assert (r["pass"] == gd[key + "_pass"]).all(), key
@@ -108,6 +108,43 @@ def test_oracle_candidate_sets_match_brute_force(name):
         union.update(ids.tolist())
     bm = r["distinct"]
     assert sum(bin(int(w)).count("1") for w in bm) == len(union)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_negative_threshold_exits_at_first_hit(name):
+    """t = -5: maximum_sum = -2,100 < 0 <= any running sum, so the loop stops
+    right after the first site within distance 4 (lib/ISSLScoringModule.cpp:
+    225,365-401): one site counted, its terms alone in the sums."""
+    gd = load_golden(name)
+    ix = golden_index(name)
+    g = gd["guides"]
+    full = orc.score(ix, g, 2, 0.0)
+    for m in range(5):
+        r = orc.score(ix, g, m, -5.0)
+        assert (r["site_count"] == np.minimum(full["site_count"], 1)).all(), m
+        assert (r["pass"] == ord("1")).all()   # scores are >= 0 > -5
+
+
+@pytest.mark.parametrize("name", ["tiny4k_5x4", "mid50k_10x10"])
+def test_exit_margin(name):
+    """The margin of or_score bounds the distance of the final sums from the
+    cut, is infinite with the exit disabled, and is 0 only at the cut."""
+    gd = load_golden(name)
+    ix = golden_index(name)
+    g = gd["guides"]
+    r0 = orc.score(ix, g, 2, 0.0, margin=True)
+    assert np.isinf(r0["margin"]).all()
+    for m, t in itertools.product(range(5), (75.0, 99.99, 1.0)):
+        r = orc.score(ix, g, m, t, margin=True)
+        cut = cut_of(t)
+        fm = np.abs(r["mit_sum"] - cut) / cut
+        fc = np.abs(r["cfd_sum"] - cut) / cut
+        fa = np.abs((r["mit_sum"] + r["cfd_sum"]) / 2 - cut) / cut
+        last = {0: fm, 1: fc, 2: np.minimum(fm, fc), 3: np.minimum(fm, fc), 4: fa}[m]
+        hit = r["site_count"] > 0
+        assert (r["margin"][hit] <= last[hit]).all(), (m, t)
+        assert (r["margin"][hit] > 0).all(), (m, t)
+        assert np.isinf(r["margin"][~hit]).all()
 
 
 def test_threshold_zero_disables_exit():
