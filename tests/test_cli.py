@@ -50,8 +50,16 @@ def test_module_disabled_skips(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["", "plain", "batched"])
+@pytest.mark.parametrize("t", ["75", "0", "99.99", "-5"])
 @pytest.mark.parametrize("name", ["and", "or", "avg", "mit", "cfd"])
-def test_cli_output_matches_reference(tmp_path, name, gpu_available):
+def test_cli_output_matches_reference(tmp_path, name, t, form, gpu_available):
+    """stdout and the results file, byte for byte (ISSLScoreOfftargets.cpp:
+    590-625: SEQ, MIT and CFD as %f or -1, the occurrence-weighted count).
+    The default (ORDERED) and PLAIN forms give the reference's sums bit for
+    bit; BATCHED's FP64 sums differ from them by rounding only, below the six
+    decimals printed for these guides."""
+    from conftest import tkey
     gd = load_golden("tiny4k_5x4")
     hx = golden_index("tiny4k_5x4")
     g = gd["guides"]
@@ -59,23 +67,29 @@ def test_cli_output_matches_reference(tmp_path, name, gpu_available):
     q = tmp_path / "queries.txt"
     q.write_text("".join(s + "\n" for s in seqs))
     out = tmp_path / "res.txt"
-    r = run([CLI, TINY, str(q), "4", "75", name, str(out)])
+    env = dict(os.environ)
+    env.pop("ISSL_FORM", None)
+    if form:
+        env["ISSL_FORM"] = form
+    r = run([CLI, TINY, str(q), "4", t, name, str(out)], env=env)
     assert r.returncode == 0, r.stderr
     m = CLI_METHOD[name]
-    key = f"m{m}_t75"
+    key = f"m{m}_t{tkey(float(t))}"
     mit, cfd = gd[key + "_mit"], gd[key + "_cfd"]
-    o = orc.score(hx, g, m, 75.0, distinct=True)
-    lines = [l for l in r.stdout.splitlines() if l.count("\t") == 2]
-    assert len(lines) == len(seqs)
-    for i, l in enumerate(lines):
+    o = orc.score(hx, g, m, float(t), distinct=True)
+    want = []
+    for i in range(len(seqs)):
         ms = "%f" % mit[i] if m != 1 else "-1"
         cs = "%f" % cfd[i] if m != 0 else "-1"
-        assert l == f"{seqs[i]}\t{ms}\t{cs}"
+        want.append((seqs[i], ms, cs, int(o["occ_count"][i])))
+    lines = [l for l in r.stdout.splitlines() if l.count("\t") == 2]
+    assert len(lines) == len(seqs)
     file_lines = out.read_text().splitlines()
     assert file_lines[0] == "# Query\tMIT_Score\tCFD_Score\tOfftarget_Count"
-    for i, l in enumerate(file_lines[1:]):
-        parts = l.split("\t")
-        assert parts[0] == seqs[i] and int(parts[3]) == int(o["occ_count"][i])
+    assert len(file_lines) == len(seqs) + 1
+    for i, (sq, ms, cs, cnt) in enumerate(want):
+        assert lines[i] == f"{sq}\t{ms}\t{cs}", i
+        assert file_lines[1 + i] == f"{sq}\t{ms}\t{cs}\t{cnt}", i
     want_distinct = sum(bin(int(w)).count("1") for w in o["distinct"])
     assert f"Distinct off-targets: {want_distinct}" in r.stderr
     assert f"Total off-targets scored: {int(o['occ_count'].sum())}" in r.stderr
@@ -216,3 +230,93 @@ EXTRACT = os.path.join(BIN, "ExtractOfftargets")
 def test_extract_cli_usage():
     r = run([EXTRACT, "out.txt"])
     assert r.returncode == 1 and "Usage:" in r.stderr
+
+
+# ---------------------------------------------------------------- index builder
+
+CREATE = os.path.join(BIN, "ISSLCreateIndex")
+
+
+def sites_text(hx, ending="\n"):
+    """Sorted site lines as ExtractOfftargets writes them: each distinct site
+    repeated by its occurrence count, ids in text order."""
+    occ = hx.occurrences()
+    seqs = issl.decode(hx.sigs)
+    assert seqs == sorted(seqs)
+    return "".join((sq[:hx.seq_len] + ending) * int(o) for sq, o in zip(seqs, occ))
+
+
+def masks_text(masks, seq_len=20, ending="\n"):
+    return "".join("".join("1" if (int(m) >> j) & 1 else "0" for j in range(seq_len)) + ending
+                   for m in masks)
+
+
+@pytest.mark.parametrize("ending", ["\n", "\r\n"])
+def test_create_index_reproduces_committed_index(tmp_path, ending):
+    """ISSLCreateIndex.cpp:284-493 on the committed index's sites: the same
+    bytes, whatever the line ending (getLineEnding, :32-52)."""
+    hx = golden_index("tiny4k_5x4")
+    sites = tmp_path / "sites.txt"
+    sites.write_bytes(sites_text(hx, ending).encode())
+    sc = tmp_path / "slices.txt"
+    sc.write_bytes(masks_text(hx.masks, 20, ending).encode())
+    out = tmp_path / "out.issl"
+    r = run([CREATE, str(sites), str(sc), "20", str(out)])
+    assert r.returncode == 0, r.stderr
+    assert f"Number of sequences: {int(hx.occurrences().sum())}" in r.stdout
+    assert out.read_bytes() == open(TINY, "rb").read()
+
+
+def test_create_index_reproduces_reference_fixture_lists(tmp_path):
+    """The reference's own test index (tests/data/test_genome_offTargets_indexed.issl,
+    legacy layout): the same signatures, masks, list sizes and entries."""
+    hx = golden_index("legacy39_5x4")
+    sites = tmp_path / "sites.txt"
+    sites.write_text(sites_text(hx))
+    sc = tmp_path / "slices.txt"
+    sc.write_text(masks_text(hx.masks))
+    out = tmp_path / "out.issl"
+    r = run([CREATE, str(sites), str(sc), "20", str(out)])
+    assert r.returncode == 0, r.stderr
+    got = issl.read_index(str(out))
+    for f in ("sigs", "masks", "list_sizes", "entries"):
+        assert np.array_equal(getattr(got, f), getattr(hx, f)), f
+
+
+def test_create_index_errors(tmp_path):
+    hx = golden_index("tiny4k_5x4")
+    sites = tmp_path / "sites.txt"
+    sites.write_text(sites_text(hx))
+    sc = tmp_path / "slices.txt"
+    sc.write_text(masks_text(hx.masks))
+    out = str(tmp_path / "o.issl")
+    r = run([CREATE, str(sites), str(sc), "20"])
+    assert r.returncode == 1 and "Usage:" in r.stderr
+    r = run([CREATE, str(sites), str(sc), "22", out])
+    assert r.returncode == 1 and "greater than 21" in r.stderr
+    r = run([CREATE, str(tmp_path / "missing.txt"), str(sc), "20", out])
+    assert r.returncode == 1 and "Could not find" in r.stderr
+    ragged = tmp_path / "ragged.txt"
+    ragged.write_text(sites_text(hx) + "ACGT\n")
+    r = run([CREATE, str(ragged), str(sc), "20", out])
+    assert r.returncode == 1 and "not a multiple" in r.stderr
+    assert not os.path.exists(out)
+
+
+def test_create_index_21_positions_is_written_and_refused_by_the_scorer(tmp_path):
+    """seqLength 21 is the builder's limit (ISSLCreateIndex.cpp:296); the
+    scorer refuses such an index (its distance is taken on 20 positions)."""
+    rng = np.random.default_rng(5)
+    seqs = sorted({"".join("ACGT"[x] for x in rng.integers(0, 4, 21)) for _ in range(500)})
+    sites = tmp_path / "sites.txt"
+    sites.write_text("".join(s + "\n" for s in seqs))
+    sc = tmp_path / "slices.txt"
+    sc.write_text(masks_text([0xF << (4 * i) for i in range(5)], 21))
+    out = str(tmp_path / "o.issl")
+    r = run([CREATE, str(sites), str(sc), "21", out])
+    assert r.returncode == 0, r.stderr
+    got = issl.read_index(out)
+    assert got.seq_len == 21 and got.n_sites == len(seqs)
+    with pytest.raises(issl.IsslError) as e:
+        issl.Index.open(out)
+    assert e.value.code == issl._capi.ISSL_ERR_FORMAT
