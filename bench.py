@@ -31,6 +31,13 @@ import sys
 import tempfile
 import time
 
+# The host cores this process may use, read before any OpenMP runtime loads:
+# with OMP_PROC_BIND set, the runtime binds the main thread to one core when it
+# starts, and the affinity mask then shows that core alone.
+try:
+    AFFINITY_CPUS = len(os.sched_getaffinity(0))
+except AttributeError:
+    AFFINITY_CPUS = os.cpu_count() or 1
 # The CPU scorer's OpenMP threads stay on neighbouring cores; the runtime reads
 # this when it loads, so it is set before any library that links it.
 os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -61,12 +68,9 @@ def cpu_model() -> str:
 
 
 def cpu_info() -> dict:
-    """Host cores this process may use (the affinity mask), with the cgroup
-    CPU quota and OMP_NUM_THREADS recorded beside it."""
-    try:
-        aff = len(os.sched_getaffinity(0))
-    except AttributeError:
-        aff = os.cpu_count() or 1
+    """Host cores this process may use (its affinity mask at start-up), with
+    the cgroup CPU quota and OMP_NUM_THREADS recorded beside it."""
+    aff = AFFINITY_CPUS
     quota = None
     try:
         quota = open("/sys/fs/cgroup/cpu.max").read().strip()

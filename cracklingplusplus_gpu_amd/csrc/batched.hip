@@ -537,6 +537,13 @@ __device__ __forceinline__ double readlane_f64(double v, uint32_t src) {
 // lib/ISSLScoringModule.cpp:231-401 adds and stops.  Sums, counts and the
 // exit point are the reference's bit for bit; after the exit the wave stops
 // streaming the guide.  (FP64 and the tile queue only.)
+// Resident waves per SIMD the register allocation aims for: 5 at depths 4
+// and 8, 4 at depth 16.
+template <int DEPTH>
+constexpr int tile_waves_per_simd() {
+    return DEPTH <= 8 ? 5 : 4;
+}
+
 template <int PREC, int DEPTH, bool QUEUE, bool RES, bool ORD = false>
 __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     static_assert(!ORD || (PREC == ISSL_PREC_FP64 && QUEUE), "in-order tiles are FP64 and queued");
@@ -548,20 +555,23 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __shared__ uint64_t s_fold[kMaxSlices];
     __shared__ uint64_t s_posmask[kNoPos + 1];
     __shared__ uint32_t s_rank[3][RES ? 128 : kRankTerms];
-    // RES: slice s's residual positions, ascending (the code field's order).
-    // (at most kResMaxCodes: the layout is built when every slice has
-    // kResMinWidth or more positions)
-    __shared__ uint8_t s_rabs[RES ? kMaxSlices : 1][kResMaxCodes];
-    // RES: the slices holding each of those positions (posmask by residual
-    // index), read beside the position so stage A needs one LDS round trip.
-    __shared__ uint64_t s_rpm[RES ? kMaxSlices : 1][kResMaxCodes];
     // The current tile's CFD factors by (position, guide ^ site there).
     __shared__ Real s_cq[kTileWaves][(kNoPos + 1) * 4];
-    // Tile plans, double-buffered: the next tile's plan lands in the other half.
-    __shared__ uint32_t s_pre[kTileWaves][2][kMaxSlices + 1];
-    __shared__ uint64_t s_delta[kTileWaves][2][kMaxSlices];
-    // RES: the guide's residual codes per slice | slice << 24.
-    __shared__ uint32_t s_gres[RES ? kTileWaves : 1][2][kMaxSlices];
+    // Arrays sized by the slice count S, in dynamic LDS (tiles_lds_bytes):
+    //   delta [kTileWaves][2][S]   tile plans' posting offsets, double-buffered
+    //                              (the next tile's plan lands in the other half)
+    //   rpm   [S][kResMaxCodes]    RES: the slices holding each residual
+    //                              position (posmask by residual index), read
+    //                              beside the position so stage A needs one
+    //                              LDS round trip
+    //   pre   [kTileWaves][2][S+1] tile plans' prefix of the list lengths
+    //   gres  [kTileWaves][2][S]   RES: the guide's residual codes per slice |
+    //                              slice << 24
+    //   rabs  [S][kResMaxCodes]    RES: slice s's residual positions,
+    //                              ascending (the code field's order; at most
+    //                              kResMaxCodes, as the layout needs every
+    //                              slice to have kResMinWidth or more positions)
+    extern __shared__ __align__(16) char s_dyn[];
     // Candidate ring: tile-relative position | slice << kRelBits, and the site word.
     __shared__ uint32_t s_rpos[kTileWaves][kRing];
     __shared__ Word s_rword[kTileWaves][kRing];
@@ -569,7 +579,19 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __shared__ uint32_t s_hpos[kTileWaves][kHitRing];
     __shared__ Word s_hword[kTileWaves][kHitRing];
     __shared__ uint32_t s_hmis[kTileWaves][kHitRing];
+    // ORD: a batch's hit terms (MIT, CFD) in hit order, for the in-order adds.
+    __shared__ double2 s_ord[ORD ? kTileWaves : 1][ORD ? 64 : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int SD = a.S;
+    uint64_t *const d_delta = reinterpret_cast<uint64_t *>(s_dyn);
+    uint64_t *const d_rpm = d_delta + kTileWaves * 2 * SD;
+    uint32_t *const d_pre = reinterpret_cast<uint32_t *>(d_rpm + (RES ? SD * kResMaxCodes : 0));
+    uint32_t *const d_gres = d_pre + kTileWaves * 2 * (SD + 1);
+    uint8_t *const d_rabs = reinterpret_cast<uint8_t *>(d_gres + (RES ? kTileWaves * 2 * SD : 0));
+    // This wave's plan buffers b = 0, 1.
+    auto pre_of = [&](int b) { return d_pre + (wave * 2 + b) * (SD + 1); };
+    auto delta_of = [&](int b) { return d_delta + (wave * 2 + b) * SD; };
+    auto gres_of = [&](int b) { return d_gres + (wave * 2 + b) * SD; };
     const Real *cfdg = static_cast<const Real *>(a.cfd);
     const Real *mitg = static_cast<const Real *>(a.mit);
     for (int i = tid; i < kCfdEntries; i += kTileBlock) s_cfd[i] = cfdg[i];
@@ -592,8 +614,9 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             uint32_t j = 0;
             for (uint32_t p = 0; p < static_cast<uint32_t>(kResPositions); ++p)
                 if (!((f >> (2 * p)) & 1ULL) && j < static_cast<uint32_t>(kResMaxCodes))
-                    s_rabs[tid][j++] = static_cast<uint8_t>(p);
-            for (; j < static_cast<uint32_t>(kResMaxCodes); ++j) s_rabs[tid][j] = static_cast<uint8_t>(kNoPos);
+                    d_rabs[tid * kResMaxCodes + j++] = static_cast<uint8_t>(p);
+            for (; j < static_cast<uint32_t>(kResMaxCodes); ++j)
+                d_rabs[tid * kResMaxCodes + j] = static_cast<uint8_t>(kNoPos);
         }
     } else {
         build_rank_terms(s_rank, tid, kTileBlock);
@@ -603,8 +626,8 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     __syncthreads();
     if constexpr (RES) {
         for (int i = tid; i < a.S * kResMaxCodes; i += kTileBlock) {
-            const uint32_t p = s_rabs[i / kResMaxCodes][i % kResMaxCodes];
-            s_rpm[i / kResMaxCodes][i % kResMaxCodes] = p < kNoPos ? s_posmask[p] : 0ULL;
+            const uint32_t p = d_rabs[i];
+            d_rpm[i] = p < kNoPos ? s_posmask[p] : 0ULL;
         }
         __syncthreads();
     }
@@ -740,11 +763,11 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     };
     auto plan_commit = [&](const Plan &m, int b) {
         if (lane < S) {
-            s_pre[wave][b][lane] = m.pre;
-            s_delta[wave][b][lane] = m.dl;
-            if constexpr (RES) s_gres[wave][b][lane] = m.gr;
+            pre_of(b)[lane] = m.pre;
+            delta_of(b)[lane] = m.dl;
+            if constexpr (RES) gres_of(b)[lane] = m.gr;
         }
-        if (lane == 0) s_pre[wave][b][S] = m.P;  // S may be 64: no lane S
+        if (lane == 0) pre_of(b)[S] = m.P;  // S may be 64: no lane S
         __builtin_amdgcn_wave_barrier();
     };
 
@@ -824,7 +847,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 if (esc || a.mark_distinct) {
                     const int sl = static_cast<int>(ent >> kRelBits);
                     const uint64_t addr =
-                        a0 + (ent & ((1u << kRelBits) - 1)) + s_delta[wave][buf][sl];
+                        a0 + (ent & ((1u << kRelBits) - 1)) + delta_of(buf)[sl];
                     const uint64_t pe = a.post[addr];
                     occ = static_cast<uint32_t>(pe >> 32);
                     h_id = static_cast<uint32_t>(pe);
@@ -881,24 +904,43 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 }
             }
             if constexpr (ORD) {
-                // The hits in order, one at a time: lane i holds hit i.
-                uint32_t acc = cnt;
-                if (a.sc.use_cut) {
-                    for (uint32_t i = 0; i < cnt; ++i) {
-                        run_m += readlane_f64(static_cast<double>(h_tm), i);
-                        run_c += readlane_f64(static_cast<double>(h_tc), i);
-                        if (bexit(a.sc.method, run_m, run_c, a.sc.cut)) {
-                            acc = i + 1;
-                            stop = true;
-                            break;
+                // The hits in order, one at a time: hit i's terms sit at
+                // s_ord[i], and lane 0 adds them to the running sums (the
+                // reads run ahead of the dependent adds); the sums and the
+                // exit point are then broadcast to the wave.
+                if (static_cast<uint32_t>(lane) < cnt)
+                    s_ord[wave][lane] = make_double2(static_cast<double>(h_tm), static_cast<double>(h_tc));
+                __builtin_amdgcn_wave_barrier();
+                uint32_t acc = cnt, ex = 0;
+                if (lane == 0) {
+                    double m = run_m, c = run_c;
+                    if (a.sc.use_cut) {
+                        for (uint32_t i = 0; i < cnt; ++i) {
+                            const double2 t = s_ord[wave][i];
+                            m += t.x;
+                            c += t.y;
+                            if (bexit(a.sc.method, m, c, a.sc.cut)) {
+                                acc = i + 1;
+                                ex = 1;
+                                break;
+                            }
+                        }
+                    } else {
+#pragma unroll 8
+                        for (uint32_t i = 0; i < cnt; ++i) {
+                            const double2 t = s_ord[wave][i];
+                            m += t.x;
+                            c += t.y;
                         }
                     }
-                } else {
-                    for (uint32_t i = 0; i < cnt; ++i) {
-                        run_m += readlane_f64(static_cast<double>(h_tm), i);
-                        run_c += readlane_f64(static_cast<double>(h_tc), i);
-                    }
+                    run_m = m;
+                    run_c = c;
                 }
+                __builtin_amdgcn_wave_barrier();
+                run_m = readlane_f64(run_m, 0);
+                run_c = readlane_f64(run_c, 0);
+                acc = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(acc)));
+                stop = __builtin_amdgcn_readfirstlane(static_cast<int>(ex)) != 0;
                 // Counts and distinct sites of the hits up to the exit.
                 if (static_cast<uint32_t>(lane) < acc) {
                     occ_sum += h_occ;
@@ -939,8 +981,8 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const uint32_t b2 = f ? static_cast<uint32_t>(__builtin_ctz(f)) : 0u;
-                        const uint32_t p = s_rabs[sl][b2 >> 1];
-                        const uint64_t pm = s_rpm[sl][b2 >> 1];
+                        const uint32_t p = d_rabs[sl * kResMaxCodes + (b2 >> 1)];
+                        const uint64_t pm = d_rpm[sl * kResMaxCodes + (b2 >> 1)];
                         ix[i] = f ? ((p << 2) | ((x >> b2) & 3u)) : kAbsent7;
                         nm |= f ? pm : 0ULL;
                         f &= f - 1;
@@ -985,11 +1027,14 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         int cur = 0;
         uint32_t nb;
         uint64_t dl;
+        const uint32_t *const t_pre = pre_of(buf);
+        const uint64_t *const t_delta = delta_of(buf);
+        const uint32_t *const t_gres = gres_of(buf);
         {
             const uint32_t p0 = a0 + lane;
-            while (cur < S - 1 && p0 >= s_pre[wave][buf][cur + 1]) ++cur;
-            nb = s_pre[wave][buf][cur + 1];
-            dl = s_delta[wave][buf][cur];
+            while (cur < S - 1 && p0 >= t_pre[cur + 1]) ++cur;
+            nb = t_pre[cur + 1];
+            dl = t_delta[cur];
         }
         // The filter works on 32-bit halves: positions 0..15 in the low word,
         // 16..19 in bits 0..7 of the high word; the mask drops the occurrence
@@ -999,7 +1044,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         // RES: the same for the residual words, and the guide's residual codes
         // for the current slice | slice << 24.
         const uint32_t *res_p = RES ? a.res + dl : nullptr;
-        uint32_t gsl = RES ? s_gres[wave][buf][cur] : 0u;
+        uint32_t gsl = RES ? t_gres[cur] : 0u;
         // One round: DEPTH loads of 64 consecutive postings, then the filter.
         // FULL rounds lie inside the tile; in the last round lanes past the
         // tile read its last posting again (every lane loads without a
@@ -1015,13 +1060,13 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 if (pc >= nb) {
                     do {
                         ++cur;
-                        nb = s_pre[wave][buf][cur + 1];
+                        nb = t_pre[cur + 1];
                     } while (pc >= nb);
                     if constexpr (RES) {
-                        res_p = a.res + s_delta[wave][buf][cur];
-                        gsl = s_gres[wave][buf][cur];
+                        res_p = a.res + t_delta[cur];
+                        gsl = t_gres[cur];
                     } else {
-                        site_p = a.site + s_delta[wave][buf][cur];
+                        site_p = a.site + t_delta[cur];
                     }
                 }
                 if constexpr (RES) {
@@ -1631,39 +1676,70 @@ namespace {
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-// fresh: the buffer was (re)allocated, so the queue counters are not zero.
+// Sizes of the workspace's arrays for n guides over S slices.
+struct WsSizes {
+    size_t pre, gres, delta, u32, part, list, ctr, total;
+    WsSizes(uint64_t n, int S, bool res) {
+        pre = align256(n * (S + 1) * 4);
+        gres = res ? align256(n * std::max(S, 1) * 4) : 0;
+        delta = align256(n * std::max(S, 1) * 8);
+        u32 = align256(n * 4);
+        part = align256(n * kMaxTiles * sizeof(TilePart));
+        list = align256(kRegions * region_guides(n, S) * (kMaxTiles + 1) * 4);
+        ctr = align256(kQueueCtrs * 4);
+        total = ctr + pre + delta + 2 * u32 + part + list + gres;
+    }
+};
+
+// fresh: the queue counters are not known to be zero (the buffer was
+// (re)allocated, or is the caller's).
 int carve(Workspace &ws, uint64_t n, int S, bool res, BatchWs *b, bool *fresh) {
-    const size_t sz_pre = align256(n * (S + 1) * 4);
-    const size_t sz_gres = res ? align256(n * std::max(S, 1) * 4) : 0;
-    const size_t sz_delta = align256(n * std::max(S, 1) * 8);
-    const size_t sz_u32 = align256(n * 4);
-    const size_t sz_part = align256(n * kMaxTiles * sizeof(TilePart));
-    const size_t sz_list = align256(kRegions * region_guides(n, S) * (kMaxTiles + 1) * 4);
-    const size_t sz_ctr = align256(kQueueCtrs * 4);
-    const void *old = ws.tiles.p;
-    int rc = ws.tiles.reserve(sz_ctr + sz_pre + sz_delta + 2 * sz_u32 + sz_part + sz_list + sz_gres);
-    if (rc != ISSL_OK) return rc;
-    *fresh = ws.tiles.p != old;
-    char *p = static_cast<char *>(ws.tiles.p);
+    const WsSizes z(n, S, res);
+    char *p;
+    if (ws.ext) {
+        if (z.total > ws.ext_bytes)
+            return fail(ISSL_ERR_CAPACITY, "the scoring workspace for " + std::to_string(n) +
+                                               " guides needs " + std::to_string(z.total) +
+                                               " bytes, more than the " +
+                                               std::to_string(ws.ext_bytes) + " available");
+        p = ws.ext;
+        *fresh = true;
+    } else {
+        const void *old = ws.tiles.p;
+        int rc = ws.tiles.reserve(z.total);
+        if (rc != ISSL_OK) return rc;
+        *fresh = ws.tiles.p != old;
+        p = static_cast<char *>(ws.tiles.p);
+    }
     // The queue counters come first, at the same place for every n: they are
     // zeroed once and then kept at zero between launch sequences, while the
     // arrays after them move with n.
     b->tctr = reinterpret_cast<uint32_t *>(p);
-    p += sz_ctr;
+    p += z.ctr;
     b->pre = reinterpret_cast<uint32_t *>(p);
-    p += sz_pre;
+    p += z.pre;
     b->delta = reinterpret_cast<uint64_t *>(p);
-    p += sz_delta;
+    p += z.delta;
     b->gchunk = reinterpret_cast<uint32_t *>(p);
-    p += sz_u32;
+    p += z.u32;
     b->gtiles = reinterpret_cast<uint32_t *>(p);
-    p += sz_u32;
+    p += z.u32;
     b->part = reinterpret_cast<TilePart *>(p);
-    p += sz_part;
+    p += z.part;
     b->tlist = reinterpret_cast<uint32_t *>(p);
-    p += sz_list;
+    p += z.list;
     b->gres = res ? reinterpret_cast<uint32_t *>(p) : nullptr;
     return ISSL_OK;
+}
+
+// Dynamic LDS of the tile kernel: its S-sized arrays (see k_tiles).
+size_t tiles_lds_bytes(int S, bool res) {
+    const size_t s = static_cast<size_t>(S);
+    size_t b = kTileWaves * 2 * s * 8;                  // delta
+    if (res) b += s * kResMaxCodes * 8;                 // rpm
+    b += kTileWaves * 2 * (s + 1) * 4;                  // pre
+    if (res) b += kTileWaves * 2 * s * 4 + s * kResMaxCodes;   // gres, rabs
+    return (b + 15) / 16 * 16;
 }
 
 // e0/e1: timing events attached to the dispatch (null when timing is off).
@@ -1674,18 +1750,19 @@ int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st, 
     const dim3 g(grid), b(kTileBlock);
     if (depth != 4 && depth != 8 && depth != 16)
         return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
+    const size_t lds_res = tiles_lds_bytes(ta.S, true), lds_inl = tiles_lds_bytes(ta.S, false);
     if (ta.res && ta.ws.tlist && ta.ws.gres) {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true, true>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true, true>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true, true>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
+        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
+        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
     } else if (ta.ws.tlist) {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true, false>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true, false>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true, false>), g, b, lds_inl, st, e0, e1, 0, ta);
+        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true, false>), g, b, lds_inl, st, e0, e1, 0, ta);
+        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true, false>), g, b, lds_inl, st, e0, e1, 0, ta);
     } else {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, false, false>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, false, false>), g, b, 0, st, e0, e1, 0, ta);
-        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, false, false>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, false, false>), g, b, lds_inl, st, e0, e1, 0, ta);
+        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, false, false>), g, b, lds_inl, st, e0, e1, 0, ta);
+        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, false, false>), g, b, lds_inl, st, e0, e1, 0, ta);
     }
     return ISSL_OK;
 }
@@ -1696,15 +1773,16 @@ int launch_ordered_depth(const TileArgs &ta, int depth, int grid, hipStream_t st
     const dim3 g(grid), b(kTileBlock);
     constexpr int P = ISSL_PREC_FP64;
     if (!ta.ws.tlist) return fail(ISSL_ERR_STATE, "in-order tiles need the tile queue");
+    const size_t lds_res = tiles_lds_bytes(ta.S, true), lds_inl = tiles_lds_bytes(ta.S, false);
     if (ta.res && ta.ws.gres) {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<P, 4, true, true, true>), g, b, 0, st, e0, e1, 0, ta);
-        else if (depth == 8) hipExtLaunchKernelGGL((k_tiles<P, 8, true, true, true>), g, b, 0, st, e0, e1, 0, ta);
-        else if (depth == 16) hipExtLaunchKernelGGL((k_tiles<P, 16, true, true, true>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<P, 4, true, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
+        else if (depth == 8) hipExtLaunchKernelGGL((k_tiles<P, 8, true, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
+        else if (depth == 16) hipExtLaunchKernelGGL((k_tiles<P, 16, true, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
         else return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
     } else {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<P, 4, true, false, true>), g, b, 0, st, e0, e1, 0, ta);
-        else if (depth == 8) hipExtLaunchKernelGGL((k_tiles<P, 8, true, false, true>), g, b, 0, st, e0, e1, 0, ta);
-        else if (depth == 16) hipExtLaunchKernelGGL((k_tiles<P, 16, true, false, true>), g, b, 0, st, e0, e1, 0, ta);
+        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<P, 4, true, false, true>), g, b, lds_inl, st, e0, e1, 0, ta);
+        else if (depth == 8) hipExtLaunchKernelGGL((k_tiles<P, 8, true, false, true>), g, b, lds_inl, st, e0, e1, 0, ta);
+        else if (depth == 16) hipExtLaunchKernelGGL((k_tiles<P, 16, true, false, true>), g, b, lds_inl, st, e0, e1, 0, ta);
         else return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
     }
     return ISSL_OK;
@@ -1894,6 +1972,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
 }
 
 }  // namespace
+
+size_t batch_ws_bytes(uint64_t n, int S, bool res) { return WsSizes(n, S, res).total; }
 
 issl_tuning default_tuning() {
     issl_tuning t{};

@@ -132,6 +132,12 @@ struct Workspace {
     // The tile queue's counters in `tiles` may be nonzero: a launch sequence
     // stopped before its finalizer was queued.
     bool queue_dirty = false;
+    // A caller's device buffer to carve the BATCHED / ORDERED workspace from
+    // instead of `tiles` (nothing is allocated; a launch needing more than
+    // ext_bytes fails with ISSL_ERR_CAPACITY; its queue counters are cleared
+    // by a fill kernel at the start of every launch sequence).
+    char *ext = nullptr;
+    size_t ext_bytes = 0;
 };
 
 // Running state of one guide between slice windows: the two sums in FP64,
@@ -228,6 +234,9 @@ namespace issl {
 int launch_plain(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                  const DevResults &out, uint64_t *distinct, const uint64_t *cand_off,
                  uint32_t *cand_ids, hipStream_t stream, const Window *win = nullptr);
+// Bytes of the BATCHED / ORDERED workspace one launch sequence carves for n
+// guides over S slices (res: with the list-residual codes).
+size_t batch_ws_bytes(uint64_t n, int S, bool res);
 int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                    int precision, int chunk, const issl_tuning &tn, const DevResults &out,
                    uint64_t *distinct, Workspace &ws, hipStream_t stream,

@@ -74,3 +74,21 @@ def gpu_available():
     if issl.device_count() <= 0:
         pytest.fail("no HIP device: GPU tests need an MI355X")
     return True
+
+
+@pytest.fixture(scope="session")
+def c1_index(gpu_available):
+    """The 100 M-site, 10-slice index (genome100m_10x10) resident on device 0,
+    its host arrays, its 10,000 guides and their kinds; shared by the
+    full-size modules."""
+    import cracklingplusplus_gpu_amd as issl
+    from cracklingplusplus_gpu_amd import workloads
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    print(f"[gpu-full] building genome100m_10x10 ({threads} threads)", flush=True)
+    sigs, occ, masks, guides, kinds = workloads.make("genome100m_10x10", threads=threads)
+    hx = issl.build_lists(sigs, occ, masks, threads=threads)
+    del sigs, occ
+    print("[gpu-full] uploading genome100m_10x10", flush=True)
+    ix = issl.Index.from_arrays(hx)
+    yield hx, ix, guides, kinds
+    ix.close()

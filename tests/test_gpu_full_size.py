@@ -21,7 +21,6 @@ import numpy as np
 import pytest
 
 import cracklingplusplus_gpu_amd as issl
-from cracklingplusplus_gpu_amd import workloads
 import oracle_lib as orc
 
 pytestmark = pytest.mark.gpu
@@ -34,15 +33,8 @@ def bits(a):
 
 
 @pytest.fixture(scope="module")
-def c1(gpu_available):
-    print("[gpu-full] building genome100m_10x10", flush=True)
-    sigs, occ, masks, guides, kinds = workloads.make("genome100m_10x10", threads=16)
-    hx = issl.build_lists(sigs, occ, masks, threads=16)
-    del sigs, occ
-    print("[gpu-full] uploading", flush=True)
-    ix = issl.Index.from_arrays(hx)
-    yield hx, ix, guides, kinds
-    ix.close()
+def c1(c1_index):
+    return c1_index
 
 
 def sample(guides, kinds, per_kind, seed=11):
