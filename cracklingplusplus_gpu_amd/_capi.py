@@ -111,6 +111,8 @@ SIGNATURES = [
                            POINTER(issl_results)]),
     ("issl_score_device", c_int, [c_void_p, c_void_p, c_uint64, POINTER(issl_params),
                                   POINTER(issl_results), c_void_p]),
+    ("issl_score_device_noalloc", c_int, [c_void_p, c_void_p, c_uint64, POINTER(issl_params),
+                                          POINTER(issl_results), c_void_p]),
     ("issl_candidates", c_int, [c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_uint64]),
     ("issl_autotune", c_int, [c_void_p, c_void_p, c_uint64, c_char_p, POINTER(issl_tuning)]),
     ("issl_autotune_precision", c_int, [c_void_p, c_void_p, c_uint64, c_int32, c_char_p,

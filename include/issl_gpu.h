@@ -238,6 +238,21 @@ int issl_score_device(issl_index *ix, const uint64_t *guides_device, uint64_t n,
                       const issl_params *params, issl_results *out_device,
                       void *stream);
 
+/* issl_score_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * BATCHED and ORDERED forms carve their scoring workspace (tile plans,
+ * per-tile partials, the tile queue) from a 64 MiB device array that each
+ * GPU the library runs on holds until the process ends; PLAIN needs none.
+ * A call whose workspace would exceed it (about 27,000 guides at 10 slices;
+ * the message gives the limit for the index) fails with ISSL_ERR_CAPACITY.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race.  A resident index only; ISSL_FLAG_GRAPH is refused.  Results equal
+ * issl_score_device's bit for bit. */
+int issl_score_device_noalloc(issl_index *ix, const uint64_t *guides_device, uint64_t n,
+                              const issl_params *params, issl_results *out_device,
+                              void *stream);
+
 /* Candidate sets: for every guide the ids of the sites it scores with the
  * threshold disabled (dist <= 4, each site once, in the reference's
  * slice-major posting order), CSR form.  offsets[n+1]; ids[capacity].
