@@ -416,6 +416,50 @@ def kernel_name(issl, tn):
             "k_stream" if tn.heavy_kpost == issl._capi.HEAVY_NEVER else "k_stream+k_tiles")
 
 
+def tail_leg(issl, hiprt, ix, hx, batches, kinds_all, args):
+    """A heavy tail in one launch: the posting volume of batch 0's guides, and
+    the tile kernel's time for that mixed batch against its two parts
+    (repeat-neighbourhood guides, unique guides) scored alone, and against a
+    tail-free control: guides of the other batches without the repeat
+    neighbourhoods, taken in order until their postings equal the mixed
+    batch's.  BATCHED FP64."""
+    Q = batches[0].shape[0]
+    batch, kinds = batches[0], kinds_all[:Q]
+    P = postings_per_guide(hx, batch)
+    med = float(np.median(P))
+    heavy = kinds == 3
+    out = {"guides": int(batch.shape[0]), "heavy_guides": int(heavy.sum()),
+           "postings_median": med,
+           "postings_p50_p99_max": [int(np.percentile(P, 50)), int(np.percentile(P, 99)),
+                                    int(P.max())],
+           "heavy_postings_p50_max_over_median": [round(float(np.median(P[heavy])) / med, 1),
+                                                  round(float(P[heavy].max()) / med, 1)],
+           "heavy_share_of_postings": round(float(P[heavy].sum()) / float(P.sum()), 3)}
+    kw = dict(method=issl.METHOD_MIT_AND_CFD, threshold=0.0, form=issl.FORM_BATCHED,
+              precision=issl.PREC_FP64)
+    pool = np.concatenate([b[kinds_all[i * Q:(i + 1) * Q] != 3] for i, b in enumerate(batches) if i])
+    Pc = np.cumsum(postings_per_guide(hx, pool))
+    n_ctl = min(int(np.searchsorted(Pc, P.sum())) + 1, pool.shape[0])
+    control = pool[:n_ctl]
+    out.update({"control_guides": n_ctl, "control_postings": int(Pc[n_ctl - 1]),
+                "mixed_postings": int(P.sum())})
+    times = {}
+    for name, g in (("mixed", batch), ("heavy_only", batch[heavy]), ("unique_only", batch[~heavy]),
+                    ("control", control)):
+        rot = Rotation(hiprt, [g])
+        timed(hiprt, rot, ix, 2, 2, **kw)
+        times[name], _ = kernel_us(hiprt, rot, ix, args.steps, **kw)
+    out.update({f"{k}_kernel_us": round(v, 2) for k, v in times.items()})
+    out["mixed_over_parts"] = round(times["mixed"] / (times["heavy_only"] + times["unique_only"]), 3)
+    out["mixed_over_control"] = round(times["mixed"] / times["control"], 3)
+    log(f"tail: heavy guides at {out['heavy_postings_p50_max_over_median']}x the median postings, "
+        f"{out['heavy_share_of_postings']:.0%} of the postings; mixed {times['mixed']:.1f} us, "
+        f"heavy alone {times['heavy_only']:.1f} us, unique alone {times['unique_only']:.1f} us; "
+        f"control ({n_ctl:,} tail-free guides, {out['control_postings']:,} postings) "
+        f"{times['control']:.1f} us: mixed/control {out['mixed_over_control']}")
+    return out
+
+
 def large_batch_leg(issl, hiprt, ix, hx, args, threads):
     """100,000 guides in one launch on the same index (30 % repeat
     neighbourhoods, 70 % unique sites), BATCHED FP64, tuned on a held-out
@@ -732,7 +776,10 @@ def main() -> int:
     kwp = dict(kw, form=issl.FORM_PLAIN)
     plain_ms = timed(hiprt, rot, ix, args.plain_steps, 1, **kwp)
     plain_k_us, _ = kernel_us(hiprt, rot, ix, args.plain_steps, **kwp)
-    res_plain = rot.results()
+    # PLAIN scores batches 0 .. plain_steps-1 only (its warm-up step is batch 0):
+    # every comparison with it covers those batches.
+    covered = min(R, max(1, args.plain_steps))
+    res_plain = rot.results()[:covered]
     log(f"plain: {plain_ms:.4f} ms/step, kernel {plain_k_us:.1f} us")
 
     # ---- ORDERED form (bit-exact FP64 in the tile kernel, one wave per guide)
@@ -740,13 +787,13 @@ def main() -> int:
     ord_ms = timed(hiprt, rot, ix, args.steps, args.warmup, **kwo)
     ord_k_us, _ = kernel_us(hiprt, rot, ix, args.steps, **kwo)
     ord_same = all(np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
-                   for a, b in zip(rot.results(), res_plain) for k in a)
+                   for a, b in zip(rot.results()[:covered], res_plain) for k in a)
     log(f"ordered: {ord_ms:.4f} ms/step, kernel {ord_k_us:.1f} us, bits of plain {ord_same}")
 
     # ---- precision against the in-order FP64 path (PLAIN), every batch
     cat = lambda rs, f: np.concatenate([r[f] for r in rs])  # noqa: E731
     prec = {}
-    for name, rs in (("fp64", res_fp64), ("fp32c", res_fp32c)):
+    for name, rs in (("fp64", res_fp64[:covered]), ("fp32c", res_fp32c[:covered])):
         rel = {f: np.abs(cat(rs, f) - cat(res_plain, f)) / np.abs(cat(res_plain, f))
                for f in ("mit", "cfd")}
         prec[name] = {"max_rel": {f: float(v.max()) for f, v in rel.items()},
@@ -754,6 +801,10 @@ def main() -> int:
                       "counts_identical": bool(
                           (cat(rs, "occ_count") == cat(res_plain, "occ_count")).all() and
                           (cat(rs, "site_count") == cat(res_plain, "site_count")).all())}
+
+    # ---- heavy-tail workloads: the mixed batch against its parts
+    tail = (tail_leg(issl, hiprt, ix, hx, batches, kinds, args)
+            if args.workload.endswith("_heavy") else None)
 
     # ---- request latency: 1,000-guide requests, host buffers, one at a time (FP64)
     g0 = batches[0]
@@ -805,7 +856,10 @@ def main() -> int:
         f"mean batch {serving['mean_batch_guides']:.0f} guides")
 
     # ---- 100,000 guides in one launch (FP64) on the same index
-    large = large_batch_leg(issl, hiprt, ix, hx, args, threads) if args.large_steps > 0 else None
+    # (both legs below use the default workload's index)
+    c1_index = args.workload == "genome100m_10x10"
+    large = (large_batch_leg(issl, hiprt, ix, hx, args, threads)
+             if args.large_steps > 0 and c1_index else None)
 
     # ---- CPU baseline: the oracle's OpenMP scorer on the host cores
     cpu_base = None
@@ -838,7 +892,7 @@ def main() -> int:
                     "host": cpu, "plain_gpu_bit_exact": cpu_match}
         log(f"CPU: {cpu_rate:,.0f} guides/s (GPU plain bit-exact: {cpu_match})")
 
-    c4 = serving_c4_leg(issl, hx, args, threads) if args.c4 else None
+    c4 = serving_c4_leg(issl, hx, args, threads) if args.c4 and c1_index else None
 
     line.update({
         "cpu_baseline": cpu_base,
@@ -850,7 +904,7 @@ def main() -> int:
                     "ms_per_step": round(ord_ms, 4), "kernel": "k_tiles (in order)",
                     "kernel_us": round(ord_k_us, 2),
                     "achieved_GBps": round(bytes_ref / (ord_k_us * 1e-6) / 1e9, 1),
-                    "same_bits_as_plain": bool(ord_same),
+                    "same_bits_as_plain": bool(ord_same), "batches_compared": covered,
                     "plain_speedup": round(plain_ms / ord_ms, 3),
                     "mode": "ORDERED: FP64 hits added in the reference's order, one wave "
                             "per guide in the persistent tile kernel"},
@@ -872,6 +926,7 @@ def main() -> int:
         "serving": dict(serving, request_guides=1000, precision="fp32c",
                         mode="request loop, Poisson arrivals, host buffers in/out"),
         "serving_c4": c4,
+        "tail": tail,
         "large_batch": large,
         "index_build": build,
         "streamed": streamed,

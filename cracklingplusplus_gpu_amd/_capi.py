@@ -67,7 +67,7 @@ class issl_synth_cfg(Structure):
                 ("contiguous_masks", c_int32), ("guide_batches", c_int32),
                 ("n_guides_site", c_uint64), ("n_guides_unique", c_uint64),
                 ("n_guides_planted", c_uint64), ("n_guides_repeat", c_uint64),
-                ("n_guides_random", c_uint64), ("seed", c_uint64)]
+                ("n_guides_random", c_uint64), ("seed", c_uint64), ("guide_sub_p", c_double)]
 
 
 class issl_server_config(Structure):

@@ -537,13 +537,6 @@ __device__ __forceinline__ double readlane_f64(double v, uint32_t src) {
 // lib/ISSLScoringModule.cpp:231-401 adds and stops.  Sums, counts and the
 // exit point are the reference's bit for bit; after the exit the wave stops
 // streaming the guide.  (FP64 and the tile queue only.)
-// Resident waves per SIMD the register allocation aims for: 5 at depths 4
-// and 8, 4 at depth 16.
-template <int DEPTH>
-constexpr int tile_waves_per_simd() {
-    return DEPTH <= 8 ? 5 : 4;
-}
-
 template <int PREC, int DEPTH, bool QUEUE, bool RES, bool ORD = false>
 __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     static_assert(!ORD || (PREC == ISSL_PREC_FP64 && QUEUE), "in-order tiles are FP64 and queued");
@@ -2001,6 +1994,8 @@ int launch_ordered(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     if (!ordered_fits(ix)) return fail(ISSL_ERR_STATE, "index too long for the in-order tiles");
     issl_tuning tn = tn_in;
     if (tn.chunk <= 0) tn = default_tuning();
+    tn.depth = ix->ord_depth;
+    tn.grid_per_cu = ix->ord_grid;
     // One tile per guide: the tile covers the longest possible guide.
     const uint32_t ord_chunk =
         static_cast<uint32_t>(std::max<uint64_t>(64, (ix->p_max + 63) / 64 * 64));

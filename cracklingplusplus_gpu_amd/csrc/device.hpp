@@ -206,6 +206,10 @@ struct issl_index {
     uint64_t device_bytes = 0;
     issl_tuning tuning{};
     bool tuned = false;
+    // Load depth and workgroups per CU of the ORDERED form's tile kernel
+    // (one tile per guide): issl_autotune times them beside the BATCHED
+    // shapes, issl_set_tuning sets them with the BATCHED ones.
+    int32_t ord_depth = 4, ord_grid = 6;
     // Kernel timing: event pairs around the dominant kernel of each launch.
     int timing = 0;                 // 0 off; else dispatches per event pair
     std::vector<hipEvent_t> ev_pool;

@@ -175,7 +175,8 @@ extern "C" int issl_synth(const issl_synth_cfg *cfg, uint64_t *sigs, uint32_t *o
     if (cfg->n_slices && !masks) return fail(ISSL_ERR_ARG, "issl_synth: null masks");
     if (N == 0 || N >= (1ULL << 32)) return fail(ISSL_ERR_ARG, "issl_synth: n_sites out of range");
     if (cfg->n_guides_planted > N) return fail(ISSL_ERR_ARG, "issl_synth: too many planted guides");
-    if (cfg->repeat_frac < 0 || cfg->repeat_frac > 1 || cfg->gc < 0 || cfg->gc > 1)
+    if (cfg->repeat_frac < 0 || cfg->repeat_frac > 1 || cfg->gc < 0 || cfg->gc > 1 ||
+        cfg->guide_sub_p < 0 || cfg->guide_sub_p > 1)
         return fail(ISSL_ERR_ARG, "issl_synth: fraction out of range");
     if (cfg->slice_width > 16 || (cfg->contiguous_masks && cfg->n_slices * cfg->slice_width > 20) ||
         cfg->slice_width > 20)
@@ -184,6 +185,7 @@ extern "C" int issl_synth(const issl_synth_cfg *cfg, uint64_t *sigs, uint32_t *o
     const uint64_t seed = cfg->seed;
     const Base base(cfg->gc);
     const uint64_t tsub = prob_to_u64(cfg->repeat_sub_p);
+    const uint64_t tsub_guide = cfg->guide_sub_p > 0 ? prob_to_u64(cfg->guide_sub_p) : tsub;
 
     // Repeat seeds and planted guides.
     const uint64_t nseeds = cfg->repeat_frac > 0 ? std::max<uint64_t>(1, cfg->n_repeat_seeds) : 0;
@@ -301,7 +303,7 @@ extern "C" int issl_synth(const issl_synth_cfg *cfg, uint64_t *sigs, uint32_t *o
         }
         for (uint64_t i = 0; i < cfg->n_guides_repeat; ++i, ++g) {
             Rng r(seed, kStreamGuideRepeat, b * cfg->n_guides_repeat + i);
-            guides[g] = nseeds ? substitute(rseed[r.below(nseeds)], tsub, r) : base.sig(r);
+            guides[g] = nseeds ? substitute(rseed[r.below(nseeds)], tsub_guide, r) : base.sig(r);
             kind[g] = 3;
         }
         for (uint64_t i = 0; i < cfg->n_guides_random; ++i, ++g) {

@@ -13,7 +13,10 @@ Conventions fixed here:
     (10 masks cannot cover all 4,845 4-subsets of 20 positions; the
     Schoenheim bound for such a covering is 30 masks);
   * planted guides are uniform 20-mers; one variant of each with 1-4
-    substitutions is added to the sites.
+    substitutions is added to the sites;
+  * genome100m_10x10_heavy is the labelled heavy-tail variant: the stated
+    generator of the 100 M-site configuration gives repeat-neighbourhood
+    guides only about 4x the median posting volume, this one 100-1000x.
 """
 from __future__ import annotations
 
@@ -21,7 +24,7 @@ from . import synth
 
 BASE = dict(gc=0.5, repeat_frac=0.0, n_repeat_seeds=0, repeat_sub_p=0.1, occ_p_unique=0.8,
             occ_p_repeat=0.5, contiguous_masks=1, guide_batches=0, n_guides_site=0, n_guides_unique=0,
-            n_guides_planted=0, n_guides_repeat=0, n_guides_random=0)
+            n_guides_planted=0, n_guides_repeat=0, n_guides_random=0, guide_sub_p=0.0)
 
 WORKLOADS = {
     # 1 M uniform sites (+200 planted variants), 5 contiguous 4-nt slices.
@@ -40,6 +43,15 @@ WORKLOADS = {
                                    n_slices=10, slice_width=10, contiguous_masks=0,
                                    n_guides_unique=70_000, n_guides_repeat=30_000,
                                    seed=0xC1C1),
+    # A heavy tail on a 100 M-site index: 4 % of the sites from two large repeat
+    # families (2 M members each, Binomial(20, 0.2) substitutions), and 10,000
+    # guides of which 1 % sit close to a family's seed (Binomial(20, 0.05)):
+    # their lists hold 100-1000x the median posting volume.
+    "genome100m_10x10_heavy": dict(BASE, n_sites=100_000_000, gc=0.41, repeat_frac=0.04,
+                                   n_repeat_seeds=2, repeat_sub_p=0.2, guide_sub_p=0.05,
+                                   occ_p_repeat=0.5, n_slices=10, slice_width=10,
+                                   contiguous_masks=0, n_guides_unique=9_900,
+                                   n_guides_repeat=100, seed=0xC1C1),
     # 300 M sites, 5 % from 10,000 repeat seeds, the 10x10 slices (same mask seed).
     "genome300m_10x10": dict(BASE, n_sites=300_000_000, gc=0.41, repeat_frac=0.05,
                              n_repeat_seeds=10_000, repeat_sub_p=0.1, occ_p_repeat=0.5,

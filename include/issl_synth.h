@@ -45,6 +45,8 @@ typedef struct {
     uint64_t n_guides_repeat;  /* repeat seed + Binomial(20, repeat_sub_p) subs */
     uint64_t n_guides_random;  /* i.i.d. at gc */
     uint64_t seed;
+    double   guide_sub_p;      /* per-position substitution prob. of repeat-neighbourhood
+                                  guides (0: repeat_sub_p, as for the family's sites) */
 } issl_synth_cfg;
 
 /* Generate one workload.  Buffers: sigs[n_sites] and occ[n_sites] (ids in

@@ -13,6 +13,9 @@ full size, on an MI355X:
   counts against a brute force over all 300 M sites.
 * Both indexes resident in one server: mixed-size requests against either
   index, each result bit-identical to scoring its guides directly.
+* A heavy tail (genome100m_10x10_heavy): 1 % of 10,000 guides visit
+  100-1000x the median posting volume; the exact forms against the oracle on
+  a sample holding heavy guides, BATCHED against ORDERED for every guide.
 
 Reference: lib/ISSLScoringModule.cpp:204-462.
 """
@@ -191,3 +194,46 @@ def test_c4_server_two_indexes_bit_identical(c1_index, c3):
         st = srv.stats()
         assert st.requests_failed == 0 and st.requests_done == 200
         assert st.peak_inflight >= 2
+
+
+# ---------------------------------------------------------------- heavy tail
+
+@pytest.fixture(scope="module")
+def heavy(gpu_available):
+    print(f"[gpu-heavy] building genome100m_10x10_heavy ({threads()} threads)", flush=True)
+    sigs, occ, masks, guides, kinds = workloads.make("genome100m_10x10_heavy", threads=threads())
+    hx = issl.build_lists(sigs, occ, masks, threads=threads())
+    del sigs, occ
+    print("[gpu-heavy] uploading", flush=True)
+    ix = issl.Index.from_arrays(hx)
+    yield hx, ix, guides, kinds
+    ix.close()
+
+
+def test_heavy_tail_is_heavy(heavy):
+    """The repeat-neighbourhood guides visit 100-1000x the median posting volume."""
+    hx, ix, guides, kinds = heavy
+    P = np.zeros(guides.shape[0], dtype=np.int64)
+    base = 0
+    for m in hx.masks.tolist():
+        pos = [j for j in range(20) if (m >> j) & 1]
+        key = np.zeros(guides.shape[0], dtype=np.int64)
+        for j, p in enumerate(pos):
+            key |= ((guides >> np.uint64(2 * p)) & np.uint64(3)).astype(np.int64) << (2 * j)
+        P += hx.list_sizes[base + key].astype(np.int64)
+        base += 1 << (2 * len(pos))
+    med = np.median(P)
+    ratio = np.median(P[kinds == 3]) / med
+    print(f"[gpu-heavy] repeat guides at {ratio:.0f}x the median postings (max {P.max() / med:.0f}x)")
+    assert 100 <= ratio <= 1000
+
+
+def test_heavy_exact_forms_vs_oracle(heavy):
+    hx, ix, guides, kinds = heavy
+    idx = np.sort(np.concatenate([np.nonzero(kinds == 3)[0][:8], np.nonzero(kinds == 1)[0][:24]]))
+    exact_vs_oracle(hx, ix, guides[idx], "gpu-heavy")
+
+
+def test_heavy_batched_vs_ordered(heavy):
+    hx, ix, guides, kinds = heavy
+    batched_vs_ordered(ix, guides, "gpu-heavy")
