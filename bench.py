@@ -443,20 +443,27 @@ def tail_leg(issl, hiprt, ix, hx, batches, kinds_all, args):
     control = pool[:n_ctl]
     out.update({"control_guides": n_ctl, "control_postings": int(Pc[n_ctl - 1]),
                 "mixed_postings": int(P.sum())})
-    times = {}
+    times, steps = {}, {}
     for name, g in (("mixed", batch), ("heavy_only", batch[heavy]), ("unique_only", batch[~heavy]),
                     ("control", control)):
         rot = Rotation(hiprt, [g])
-        timed(hiprt, rot, ix, 2, 2, **kw)
+        steps[name] = timed(hiprt, rot, ix, args.steps, 2, **kw)
         times[name], _ = kernel_us(hiprt, rot, ix, args.steps, **kw)
     out.update({f"{k}_kernel_us": round(v, 2) for k, v in times.items()})
+    out.update({f"{k}_step_ms": round(v, 4) for k, v in steps.items()})
+    out["mixed_over_control_step"] = round(steps["mixed"] / steps["control"], 3)
+    # Hits (distinct sites scored) carry the stage-A/B work, which postings do not count.
+    hits = {name: int(ix.score(g, **kw).site_count.sum()) for name, g in (("mixed", batch), ("control", control))}
+    out.update({f"{k}_hits": v for k, v in hits.items()})
     out["mixed_over_parts"] = round(times["mixed"] / (times["heavy_only"] + times["unique_only"]), 3)
     out["mixed_over_control"] = round(times["mixed"] / times["control"], 3)
     log(f"tail: heavy guides at {out['heavy_postings_p50_max_over_median']}x the median postings, "
         f"{out['heavy_share_of_postings']:.0%} of the postings; mixed {times['mixed']:.1f} us, "
         f"heavy alone {times['heavy_only']:.1f} us, unique alone {times['unique_only']:.1f} us; "
         f"control ({n_ctl:,} tail-free guides, {out['control_postings']:,} postings) "
-        f"{times['control']:.1f} us: mixed/control {out['mixed_over_control']}")
+        f"{times['control']:.1f} us: mixed/control {out['mixed_over_control']} (steps "
+        f"{steps['mixed']:.4f} / {steps['control']:.4f} ms = {out['mixed_over_control_step']}); "
+        f"hits mixed {hits['mixed']:,}, control {hits['control']:,}")
     return out
 
 

@@ -22,7 +22,7 @@ import numpy as np
 from . import _capi
 from ._capi import (FLAG_DISTINCT, FLAG_GRAPH, FORM_BATCHED, FORM_ORDERED, FORM_PLAIN, METHOD_AVG_MIT_CFD, METHOD_CFD,
                     METHOD_MIT, METHOD_MIT_AND_CFD, METHOD_MIT_OR_CFD, PREC_FP32C, PREC_FP64,
-                    IsslError, check, lib)
+                    SPLIT_NEVER, IsslError, check, lib)
 
 __all__ = [
     "Index", "IndexArrays", "Results", "IsslError", "encode", "decode", "build_lists",
@@ -417,10 +417,16 @@ class Index:
         check(lib().issl_score_device(self._h, d_guides.ptr, n, byref(p), byref(res), stream))
 
     def set_tuning(self, chunk: int = 1024, depth: int = 8, grid_per_cu: int = 6,
-                   direct_depth: int = 8, heavy_kpost: int = 0) -> None:
+                   direct_depth: int = 8, heavy_kpost: int = 0, ord_depth: Optional[int] = None,
+                   ord_grid: Optional[int] = None, ord_split: int = 0) -> None:
         """heavy_kpost: guides above heavy_kpost*1024 postings use the tile kernel,
-        the rest one wave per guide (0: every guide uses tiles; HEAVY_NEVER: none)."""
-        t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, direct_depth, heavy_kpost, 0.0)
+        the rest one wave per guide (0: every guide uses tiles; HEAVY_NEVER: none).
+        ord_depth / ord_grid: the ORDERED form's shape (default: depth, grid_per_cu);
+        ord_split: ORDERED guides above this many postings are split over tiles
+        (0: the default threshold; SPLIT_NEVER: never)."""
+        t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, direct_depth, heavy_kpost, 0.0,
+                              depth if ord_depth is None else ord_depth,
+                              grid_per_cu if ord_grid is None else ord_grid, ord_split, 0)
         check(lib().issl_set_tuning(self._h, byref(t)))
 
 

@@ -53,10 +53,13 @@ class issl_tuning(Structure):
     _fields_ = [("chunk", c_int32), ("block", c_int32), ("depth", c_int32),
                 ("grid_per_cu", c_int32), ("direct_depth", c_int32),
                 ("heavy_kpost", c_int32),
-                ("us_per_guide", c_double)]
+                ("us_per_guide", c_double),
+                ("ord_depth", c_int32), ("ord_grid", c_int32), ("ord_split", c_int32),
+                ("reserved", c_int32)]
 
 
 HEAVY_NEVER = 0x7FFFFFFF
+SPLIT_NEVER = 0x7FFFFFFF
 
 
 class issl_synth_cfg(Structure):

@@ -77,7 +77,10 @@ constexpr uint32_t kCtrStride = 16;
 // sits at tctr[i * kCtrStride].
 constexpr uint32_t kSubQueueBase = 2 * kRegions * kCtrStride;   // taken counters
 constexpr uint32_t kSubQueueCtr = kCtrStride * kSubQueues;      // taken -> finished
-constexpr int kQueueCtrs = static_cast<int>(kSubQueueBase + 2 * kSubQueueCtr);
+// In-order split: the term entries handed out (64-bit) and the split guides.
+constexpr uint32_t kTermCtr = kSubQueueBase + 2 * kSubQueueCtr;
+constexpr uint32_t kSplitCtr = kTermCtr + kCtrStride;
+constexpr int kQueueCtrs = static_cast<int>(kSplitCtr + kCtrStride);
 // Guides a region takes at most: planner block b (kPlanWaves waves of 64 / S
 // guides each) appends to region b % kRegions; a region of the first list
 // holds their tiles (at most kMaxTiles per guide), of the second one tile per
@@ -108,6 +111,16 @@ struct BatchWs {
     uint32_t *tlist;
     uint32_t *tctr;
     TilePart *part;      // [kMaxTiles][n] partial of tile (g, k) at slot k*n + g
+    // In-order split (ORDERED guides with more than the split threshold of
+    // postings, while term entries last): the guide's tiles run in parallel
+    // and write each hit's terms, in the guide's visit order, to entries
+    // gterm[g] + k*C_g + i (i-th hit of tile k); k_ordered_terms then adds them
+    // one at a time.  slist holds the split guides, counted at tctr[kSplitCtr].
+    double2 *term_mc;    // [term_cap] (MIT term, CFD term) of a hit
+    uint2 *term_oi;      // [term_cap] (occurrences, site id) of a hit
+    uint32_t *gterm;     // [n] first entry of a split guide
+    uint32_t *slist;     // [n] split guides
+    uint32_t term_cap;   // entries (0: no split)
 };
 
 // Candidate ring entries: position within the tile (tiles are shorter than
@@ -338,7 +351,18 @@ struct PlanArgs {
     int s0, s1;                   // slice window: lists outside it are empty
     const GuideState *state_in;   // guides that took the exit visit nothing
     BatchWs ws;
+    // In-order split (0: off): guides with more than split_at postings get
+    // tiles of split_chunk postings or longer (at most kMaxTiles) and term
+    // entries, while ws.term_cap lasts.
+    uint32_t split_at;
+    uint32_t split_chunk;
 };
+
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int off) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl_up(static_cast<int>(v), off));
+    const uint32_t hi = static_cast<uint32_t>(__shfl_up(static_cast<int>(v >> 32), off));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
 
 // One lane per (guide, slice): a wave plans G = 64 / S guides, each on S
 // consecutive lanes, and a workgroup kPlanWaves waves.  Every lane forms its
@@ -357,6 +381,8 @@ __host__ __device__ inline uint32_t plan_guides_per_wave(int S) {
 __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
     __shared__ uint32_t s_tot[kPlanWaves][2];
     __shared__ uint32_t s_base[2];
+    __shared__ uint64_t s_want[kPlanWaves];
+    __shared__ uint64_t s_tbase;
     const int S = a.S;
     const int SS = S > 0 ? S : 1;
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
@@ -397,8 +423,41 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
     uint64_t c = (static_cast<uint64_t>(P) + kMaxTiles - 1) / kMaxTiles;
     c = (c + 63) / 64 * 64;
     if (c < a.chunk) c = a.chunk;
-    const uint32_t K = static_cast<uint32_t>((P + c - 1) / c);
+    uint32_t K = static_cast<uint32_t>((P + c - 1) / c);
     const bool lead = in && s == 0;
+    if (a.split_at) {
+        // In-order split: P term entries per long guide, one 64-bit atomic per
+        // workgroup; a guide whose entries would pass term_cap stays one tile.
+        const uint64_t want = (lead && P > a.split_at) ? P : 0;
+        uint64_t incl = want;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t v = shfl_up64(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (lane == 63) s_want[wave] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t t = 0;
+            for (int w = 0; w < kPlanWaves; ++w) t += s_want[w];
+            s_tbase = t ? atomicAdd(reinterpret_cast<unsigned long long *>(a.ws.tctr + kTermCtr),
+                                    static_cast<unsigned long long>(t))
+                        : 0;
+        }
+        __syncthreads();
+        uint64_t start = s_tbase + incl - want;
+        for (int w = 0; w < wave; ++w) start += s_want[w];
+        uint64_t cs = (static_cast<uint64_t>(P) + kMaxTiles - 1) / kMaxTiles;
+        cs = (cs + 63) / 64 * 64;
+        if (cs < a.split_chunk) cs = a.split_chunk;
+        const uint32_t Ks = static_cast<uint32_t>((P + cs - 1) / cs);
+        if (want && Ks > 1 && start + want <= a.ws.term_cap) {
+            c = cs;
+            K = Ks;
+            a.ws.gterm[g] = static_cast<uint32_t>(start);
+            a.ws.slist[atomicAdd(a.ws.tctr + kSplitCtr, 1u)] = static_cast<uint32_t>(g);
+        }
+    }
     if (lead) {
         a.ws.pre[g * (S + 1) + S] = P;
         a.ws.gchunk[g] = static_cast<uint32_t>(c);
@@ -736,6 +795,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         uint32_t pre;
         uint64_t dl;
         uint32_t gr;   // RES: the guide's residual codes for slice `lane` | lane << 24
+        uint32_t tb;   // ORD, split guide: the tile's first term entry
     };
     auto plan_load = [&](uint32_t v, Plan &m) {
         m.g = v % n;
@@ -748,6 +808,9 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         m.pre = 0;
         m.dl = 0;
         m.gr = 0;
+        m.tb = 0;
+        // (gterm is written for split guides only, and read for K > 1 only)
+        if (ORD && a.ws.term_cap) m.tb = a.ws.gterm[m.g] + m.k * m.C;
         if (lane < S) {
             m.pre = a.ws.pre[row + lane];
             m.dl = a.ws.delta[static_cast<size_t>(m.g) * S + lane];
@@ -798,9 +861,13 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         uint64_t occ_sum = 0;
         uint32_t sites = 0;   // hits of this tile in this lane (< 2^26)
         // ORD: the guide's running sums (wave-uniform) and whether the exit
-        // has been taken (then nothing more of the guide is scored).
+        // has been taken (then nothing more of the guide is scored).  A tile
+        // of a split guide (K > 1) writes its hits' terms instead, and its
+        // hit count goes to the partial.
         double run_m = 0.0, run_c = 0.0;
         bool stop = false;
+        const bool tmode = ORD && mc.K > 1;
+        uint32_t hcnt = 0;   // tmode: hits of the tile so far
 
         // Candidates (dist <= 4) wait in the ring in position order.  They are
         // checked 64 at a time against the first-seen rule, one per lane; the
@@ -896,7 +963,17 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     ac += tc;
                 }
             }
-            if constexpr (ORD) {
+            if (ORD && tmode) {
+                // Split guide: the terms in hit order, for k_ordered_terms.
+                if (static_cast<uint32_t>(lane) < cnt) {
+                    const uint32_t i = mc.tb + hcnt + static_cast<uint32_t>(lane);
+                    a.ws.term_mc[i] = make_double2(static_cast<double>(h_tm), static_cast<double>(h_tc));
+                    a.ws.term_oi[i] = make_uint2(h_occ, h_id);
+                    occ_sum += h_occ;
+                    sites += 1;
+                }
+                hcnt += cnt;
+            } else if constexpr (ORD) {
                 // The hits in order, one at a time: hit i's terms sit at
                 // s_ord[i], and lane 0 adds them to the running sums (the
                 // reads run ahead of the dependent adds); the sums and the
@@ -1177,6 +1254,8 @@ struct FinArgs {
     uint64_t *distinct;           // mark sites up to the exit (early exit armed)
     int skip_untiled;             // guides without tiles were finished by k_stream
     int finish_single;            // guides with one tile were finished by k_tiles
+    int skip_multi;               // guides with several tiles are split guides of the
+                                  // in-order form, finished by k_ordered_terms
     const GuideState *state_in;   // carried state (null: start from zero)
     GuideState *state_out;        // null: write the results
     BatchWs ws;
@@ -1186,6 +1265,11 @@ struct FinArgs {
 // tile kernel has finished with them).
 __device__ __forceinline__ void reset_queue(const BatchWs &ws) {
     if (ws.tctr && blockIdx.x == 0 && threadIdx.x < 2 * kRegions) ws.tctr[threadIdx.x * kCtrStride] = 0;
+    if (ws.tctr && blockIdx.x == 0 && threadIdx.x == 0) {
+        ws.tctr[kTermCtr] = 0;
+        ws.tctr[kTermCtr + 1] = 0;
+        ws.tctr[kSplitCtr] = 0;
+    }
 }
 
 __global__ void k_zero_queue(uint32_t *tctr) {
@@ -1363,6 +1447,7 @@ __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
     const uint32_t K = a.ws.gtiles[g];
     if (K == 0 && a.skip_untiled) return;
     if (K == 1 && a.finish_single) return;  // finished by the wave that scored its tile
+    if (K > 1 && a.skip_multi) return;      // finished by k_ordered_terms
     double dm = 0, dc = 0;
     uint64_t occ_sum = 0, sites = 0;
     bool exited = false;
@@ -1392,6 +1477,149 @@ __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
         return;
     }
     write_guide(a.sc, a.out, g, dm, dc, occ_sum, sites);
+}
+
+// In-order split: one wave per split guide adds its hits' terms in the order
+// the reference visits them (tile 0's hits, then tile 1's, ...: slices
+// ascending, postings in list order) into FP64 sums, one at a time
+// (lib/ISSLScoringModule.cpp:365-401; guides are split only with the exit
+// off, so every hit counts); it also sums occurrences and sites and marks
+// distinct sites.  Lanes fetch
+// kOrdTermBatch entries at a time into registers and then LDS, the next
+// batch's loads in flight while the wave adds the current one (add_in_order).
+constexpr int kOrdTermPer = 8;                    // entries per lane per batch
+constexpr int kOrdTermBatch = 64 * kOrdTermPer;
+
+// Adds src[0 .. n) to (m, c) one term at a time, in order.  Every lane of
+// the wave runs it with the same values (the terms are broadcast LDS reads),
+// so the loop is uniform; sixteen terms are read before they are added.
+__device__ __forceinline__ void add_in_order(const double2 *src, uint32_t n, double &m, double &c) {
+    constexpr int U = 16;
+    double mm = m, cc = c;
+    uint32_t t = 0;
+    for (; t + U <= n; t += U) {
+        double vm[U], vc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double2 v = src[t + u];
+            vm[u] = v.x;
+            vc[u] = v.y;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            mm += vm[u];
+            cc += vc[u];
+        }
+    }
+    for (; t < n; ++t) {
+        const double2 v = src[t];
+        mm += v.x;
+        cc += v.y;
+    }
+    m = mm;
+    c = cc;
+}
+
+__global__ __launch_bounds__(256) void k_ordered_terms(FinArgs a) {
+    __shared__ double2 s_t[4][kOrdTermBatch];
+    __shared__ uint2 s_o[4][kOrdTermBatch];
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const uint32_t nsplit = a.ws.tctr[kSplitCtr];
+    const uint32_t W = gridDim.x * 4;
+    for (uint32_t i = blockIdx.x * 4 + static_cast<uint32_t>(wave); i < nsplit; i += W) {
+        const uint64_t g = a.ws.slist[i];
+        const uint32_t K = a.ws.gtiles[g], C = a.ws.gchunk[g], tb = a.ws.gterm[g];
+        // Tile hit counts, lane k holding tile k's (K <= kMaxTiles = 64), so
+        // the cursor below moves without loads that would wait behind the
+        // batch loads in flight.
+        const uint32_t cnt_l = static_cast<uint32_t>(lane) < K
+                                   ? static_cast<uint32_t>(a.ws.part[static_cast<uint64_t>(lane) * a.n + g].sites)
+                                   : 0u;
+        auto count_of = [&](uint32_t kk) {
+            return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt_l), static_cast<int>(kk)));
+        };
+        // Cursor: entry j of tile k, whose hits number cnt.
+        uint32_t k = 0, j = 0;
+        uint32_t cnt = count_of(0);
+        while (j >= cnt && k + 1 < K) {
+            ++k;
+            cnt = count_of(k);
+        }
+        // (scalar arrays: arrays of vector types are kept in scratch memory)
+        double tmv[kOrdTermPer], tcv[kOrdTermPer];
+        uint32_t ov[kOrdTermPer], iv[kOrdTermPer];
+        uint32_t nb = 0;     // entries of the batch in registers
+        bool have = false;   // registers hold a batch
+        double m = 0.0, c = 0.0;
+        uint64_t occ = 0;
+        uint32_t sites = 0;
+        for (;;) {
+            // The batch in registers goes to LDS; the next one is fetched and
+            // its loads stay in flight while the wave adds this one.
+            uint32_t ncur = 0;
+            if (have) {
+#pragma unroll
+                for (int r = 0; r < kOrdTermPer; ++r) {
+                    const uint32_t e = static_cast<uint32_t>(r * 64 + lane);
+                    if (e < nb) {
+                        s_t[wave][e] = make_double2(tmv[r], tcv[r]);
+                        s_o[wave][e] = make_uint2(ov[r], iv[r]);
+                    }
+                }
+                ncur = nb;
+            }
+            nb = j < cnt ? std::min<uint32_t>(kOrdTermBatch, cnt - j) : 0u;
+            const uint32_t base = tb + k * C + j;
+            // Every lane loads without a branch (a load under a branch is
+            // waited for at once): entries past the batch re-read its last
+            // one, and an empty batch reads the guide's first entry (a split
+            // guide has more than 64 entries).
+#pragma unroll
+            for (int r = 0; r < kOrdTermPer; ++r) {
+                const uint32_t e = static_cast<uint32_t>(r * 64 + lane);
+                const uint32_t at = nb ? base + std::min(e, nb - 1) : tb;
+                const double2 t2 = a.ws.term_mc[at];
+                const uint2 o2 = a.ws.term_oi[at];
+                tmv[r] = t2.x;
+                tcv[r] = t2.y;
+                ov[r] = o2.x;
+                iv[r] = o2.y;
+            }
+            j += nb;
+            while (j >= cnt && k + 1 < K) {
+                ++k;
+                j = 0;
+                cnt = count_of(k);
+            }
+            if (!have) {
+                have = true;
+                if (nb == 0) break;   // no hits at all
+                continue;
+            }
+            __builtin_amdgcn_wave_barrier();
+            add_in_order(s_t[wave], ncur, m, c);
+#pragma unroll
+            for (int r = 0; r < kOrdTermPer; ++r) {
+                const uint32_t e = static_cast<uint32_t>(r * 64 + lane);
+                if (e < ncur) {
+                    const uint2 v = s_o[wave][e];
+                    occ += v.x;
+                    sites += 1;
+                    if (a.distinct)
+                        atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (v.y >> 6),
+                                 1ULL << (v.y & 63));
+                }
+            }
+            __builtin_amdgcn_wave_barrier();   // before the next batch overwrites LDS
+            if (nb == 0) break;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            occ += shfl64(occ, lane ^ off);
+            sites += static_cast<uint32_t>(__shfl(static_cast<int>(sites), lane ^ off));
+        }
+        if (lane == 0) write_guide(a.sc, a.out, g, m, c, occ, sites);
+    }
 }
 
 // ------------------------------------------------------------ direct
@@ -1669,10 +1897,11 @@ namespace {
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-// Sizes of the workspace's arrays for n guides over S slices.
+// Sizes of the workspace's arrays for n guides over S slices, with term_cap
+// term entries for the in-order split (0: none).
 struct WsSizes {
-    size_t pre, gres, delta, u32, part, list, ctr, total;
-    WsSizes(uint64_t n, int S, bool res) {
+    size_t pre, gres, delta, u32, part, list, ctr, term_mc, term_oi, split, total;
+    WsSizes(uint64_t n, int S, bool res, uint64_t term_cap = 0) {
         pre = align256(n * (S + 1) * 4);
         gres = res ? align256(n * std::max(S, 1) * 4) : 0;
         delta = align256(n * std::max(S, 1) * 8);
@@ -1680,14 +1909,17 @@ struct WsSizes {
         part = align256(n * kMaxTiles * sizeof(TilePart));
         list = align256(kRegions * region_guides(n, S) * (kMaxTiles + 1) * 4);
         ctr = align256(kQueueCtrs * 4);
-        total = ctr + pre + delta + 2 * u32 + part + list + gres;
+        term_mc = align256(term_cap * sizeof(double2));
+        term_oi = align256(term_cap * sizeof(uint2));
+        split = term_cap ? 2 * u32 : 0;   // gterm, slist
+        total = ctr + pre + delta + 2 * u32 + part + list + gres + term_mc + term_oi + split;
     }
 };
 
 // fresh: the queue counters are not known to be zero (the buffer was
 // (re)allocated, or is the caller's).
-int carve(Workspace &ws, uint64_t n, int S, bool res, BatchWs *b, bool *fresh) {
-    const WsSizes z(n, S, res);
+int carve(Workspace &ws, uint64_t n, int S, bool res, uint64_t term_cap, BatchWs *b, bool *fresh) {
+    const WsSizes z(n, S, res, term_cap);
     char *p;
     if (ws.ext) {
         if (z.total > ws.ext_bytes)
@@ -1722,6 +1954,15 @@ int carve(Workspace &ws, uint64_t n, int S, bool res, BatchWs *b, bool *fresh) {
     b->tlist = reinterpret_cast<uint32_t *>(p);
     p += z.list;
     b->gres = res ? reinterpret_cast<uint32_t *>(p) : nullptr;
+    p += z.gres;
+    b->term_cap = static_cast<uint32_t>(term_cap);
+    b->term_mc = term_cap ? reinterpret_cast<double2 *>(p) : nullptr;
+    p += z.term_mc;
+    b->term_oi = term_cap ? reinterpret_cast<uint2 *>(p) : nullptr;
+    p += z.term_oi;
+    b->gterm = term_cap ? reinterpret_cast<uint32_t *>(p) : nullptr;
+    p += term_cap ? z.u32 : 0;
+    b->slist = term_cap ? reinterpret_cast<uint32_t *>(p) : nullptr;
     return ISSL_OK;
 }
 
@@ -1823,7 +2064,7 @@ DevResults offset_results(const DevResults &o, uint64_t i) {
 int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                  int precision, const issl_tuning &tn, const DevResults &out, uint64_t *distinct,
                  Workspace &ws, hipStream_t st, const Window &win, bool ordered = false,
-                 uint32_t ord_chunk = 0) {
+                 uint32_t ord_chunk = 0, uint32_t split_at = 0, uint64_t term_cap = 0) {
     const int S = static_cast<int>(ix->S);
     const bool carry = win.state_in || win.state_out;
     if (ordered && (carry || precision != ISSL_PREC_FP64))
@@ -1835,7 +2076,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     const bool res = win.res != nullptr && !direct;
     BatchWs b;
     bool fresh = false;
-    int rc = carve(ws, n, S, res, &b, &fresh);
+    const bool split = ordered && split_at && term_cap;
+    int rc = carve(ws, n, S, res, split ? term_cap : 0, &b, &fresh);
     if (rc != ISSL_OK) return rc;
     // The counters are zero unless the buffer is new or an earlier sequence
     // stopped before its k_finalize was queued.
@@ -1890,6 +2132,8 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
         const uint32_t plan_chunk = ordered ? ord_chunk : static_cast<uint32_t>(tn.chunk);
         PlanArgs pa{d_guides, n, ix->d_lstart, ix->d_meta, S, ix->n, plan_chunk,
                     static_cast<uint32_t>(tn.chunk) / 2, win.s0, win.s1, win.state_in, b};
+        pa.split_at = split ? split_at : 0;
+        pa.split_chunk = static_cast<uint32_t>(tn.chunk);
         const uint64_t G = kPlanWaves * plan_guides_per_wave(S);
         hipLaunchKernelGGL(k_plan, dim3(static_cast<uint32_t>((n + G - 1) / G)),
                            dim3(kPlanBlock), 0, st, pa);
@@ -1948,9 +2192,17 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     fa.distinct = (distinct && sc.use_cut) ? distinct : nullptr;
     fa.skip_untiled = direct ? 1 : 0;
     fa.finish_single = finish_single;
+    fa.skip_multi = split ? 1 : 0;
     fa.state_in = win.state_in;
     fa.state_out = win.state_out;
     fa.ws = b;
+    if (split) {
+        // The split guides, one wave each (their count is on the device).
+        FinArgs fo = fa;
+        fo.distinct = distinct;
+        const uint64_t waves = std::min<uint64_t>(n, static_cast<uint64_t>(cu_count(ix->device)) * 16);
+        hipLaunchKernelGGL(k_ordered_terms, dim3(static_cast<uint32_t>((waves + 3) / 4)), dim3(256), 0, st, fo);
+    }
     // (in order: the tile kernel finished every guide with postings; this
     // writes those without any and sets the queue counters back to zero)
     if (!sc.use_cut || ordered)
@@ -1976,6 +2228,9 @@ issl_tuning default_tuning() {
     t.grid_per_cu = 6;
     t.direct_depth = 8;
     t.heavy_kpost = 0;  // every guide on the tile kernel
+    t.ord_depth = 4;
+    t.ord_grid = 6;
+    t.ord_split = kOrdSplitDefault;
     return t;
 }
 
@@ -1999,6 +2254,15 @@ int launch_ordered(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     // One tile per guide: the tile covers the longest possible guide.
     const uint32_t ord_chunk =
         static_cast<uint32_t>(std::max<uint64_t>(64, (ix->p_max + 63) / 64 * 64));
+    // Guides longer than ord_split are split over tiles of tn.chunk postings
+    // when the index can hold such guides, with term entries for at most
+    // kOrdTermCap hits per launch (the rest stay one tile).  Not with the exit
+    // armed: a split guide streams all its tiles, while one wave stops at the
+    // exit, which most long guides reach early.  The allocation-free
+    // workspace splits nothing.
+    const uint32_t split_at =
+        ix->ord_split == kSplitNever ? 0u : static_cast<uint32_t>(ix->ord_split);
+    const bool can_split = split_at && !sc.use_cut && !ws.ext && ix->p_max > split_at;
     Window w;
     w.s0 = 0;
     w.s1 = static_cast<int>(ix->S);
@@ -2008,8 +2272,9 @@ int launch_ordered(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     w.res_shift = ix->res_shift;
     for (uint64_t i = 0; i < n; i += kMaxBatch) {
         const uint64_t m = std::min<uint64_t>(kMaxBatch, n - i);
+        const uint64_t cap = can_split ? std::min<uint64_t>(kOrdTermCap, m * ix->p_max) : 0;
         const int rc = launch_batch(ix, d_guides + i, m, sc, ISSL_PREC_FP64, tn, offset_results(out, i),
-                                    distinct, ws, st, w, true, ord_chunk);
+                                    distinct, ws, st, w, true, ord_chunk, split_at, cap);
         if (rc != ISSL_OK) return rc;
     }
     return ISSL_OK;

@@ -48,6 +48,11 @@ __device__ __forceinline__ uint32_t slice_key(uint64_t sig, uint64_t fold) {
 
 // heavy_kpost value meaning "no guide goes to the tile kernel".
 constexpr int32_t kHeavyNever = ISSL_HEAVY_NEVER;
+// ORDERED: guides with more postings than this are split over tiles
+// (issl_tuning.ord_split); term entries the workspace holds for them at most.
+constexpr int32_t kSplitNever = ISSL_SPLIT_NEVER;
+constexpr int32_t kOrdSplitDefault = 65536;
+constexpr uint64_t kOrdTermCap = 1ULL << 25;
 
 // Site word of the signature-inline layout: signature in bits 0..39,
 // occurrences in bits 40..63; kOccEscape means "read occ from the posting".
@@ -209,7 +214,7 @@ struct issl_index {
     // Load depth and workgroups per CU of the ORDERED form's tile kernel
     // (one tile per guide): issl_autotune times them beside the BATCHED
     // shapes, issl_set_tuning sets them with the BATCHED ones.
-    int32_t ord_depth = 4, ord_grid = 6;
+    int32_t ord_depth = 4, ord_grid = 6, ord_split = issl::kOrdSplitDefault;
     // Kernel timing: event pairs around the dominant kernel of each launch.
     int timing = 0;                 // 0 off; else dispatches per event pair
     std::vector<hipEvent_t> ev_pool;

@@ -36,7 +36,7 @@ std::string shape_key(const issl_index *ix, uint64_t n_sample, int precision) {
     for (char &c : arch)
         if (c == ' ' || c == ':') c = '_';
     std::ostringstream k;
-    k << "t5/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
+    k << "t6/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
     for (const SliceMeta &m : ix->meta_h) k << m.width << ',';
     int q = 0;
     while ((n_sample >> q) > 1) ++q;
@@ -45,9 +45,9 @@ std::string shape_key(const issl_index *ix, uint64_t n_sample, int precision) {
 }
 
 // A cache line: key, the tuning's fields, the site-word layout (1:
-// list-residual words), then the ORDERED form's depth and workgroups per CU.
-bool cache_lookup(const char *path, const std::string &key, issl_tuning *t, int *res,
-                  int *ord_depth, int *ord_grid) {
+// list-residual words), then the ORDERED form's depth, workgroups per CU and
+// split threshold.
+bool cache_lookup(const char *path, const std::string &key, issl_tuning *t, int *res) {
     if (!path) return false;
     std::ifstream in(path);
     std::string line;
@@ -56,14 +56,12 @@ bool cache_lookup(const char *path, const std::string &key, issl_tuning *t, int 
         std::istringstream ls(line);
         std::string k;
         issl_tuning v{};
-        int r = 0, od = 0, og = 0;
+        int r = 0;
         if (ls >> k >> v.chunk >> v.block >> v.depth >> v.grid_per_cu >> v.direct_depth >>
-                v.heavy_kpost >> v.us_per_guide >> r >> od >> og &&
+                v.heavy_kpost >> v.us_per_guide >> r >> v.ord_depth >> v.ord_grid >> v.ord_split &&
             k == key) {
             *t = v;
             *res = r;
-            *ord_depth = od;
-            *ord_grid = og;
             found = true;  // the last entry for a key wins
         }
     }
@@ -75,6 +73,10 @@ bool cache_lookup(const char *path, const std::string &key, issl_tuning *t, int 
 extern "C" int issl_get_tuning(issl_index *ix, issl_tuning *out) {
     if (!ix || !out) return fail(ISSL_ERR_ARG, "null argument");
     *out = ix->tuning.chunk > 0 ? ix->tuning : default_tuning();
+    out->ord_depth = ix->ord_depth;
+    out->ord_grid = ix->ord_grid;
+    out->ord_split = ix->ord_split;
+    out->reserved = 0;
     return ok();
 }
 
@@ -86,12 +88,18 @@ extern "C" int issl_set_tuning(issl_index *ix, const issl_tuning *in) {
         !(in->direct_depth == 4 || in->direct_depth == 8 || in->direct_depth == 16) ||
         in->heavy_kpost < 0)
         return fail(ISSL_ERR_ARG, "tuning out of range");
+    const int od = in->ord_depth, og = in->ord_grid, os = in->ord_split;
+    if (!(od == 0 || od == 4 || od == 8 || od == 16) || og < 0 || og > 32 ||
+        !(os == 0 || os >= 64))
+        return fail(ISSL_ERR_ARG, "ORDERED tuning out of range");
     std::lock_guard<std::mutex> lk(ix->mu);
     ix->tuning = *in;
     ix->tuning.block = 256;
     ix->tuned = true;
-    ix->ord_depth = in->depth;
-    ix->ord_grid = in->grid_per_cu;
+    const issl_tuning def = default_tuning();
+    ix->ord_depth = od ? od : def.ord_depth;
+    ix->ord_grid = og ? og : def.ord_grid;
+    ix->ord_split = os ? os : def.ord_split;
     return ok();
 }
 
@@ -105,15 +113,16 @@ extern "C" int issl_autotune_precision(issl_index *ix, const uint64_t *sample, u
         return fail(ISSL_ERR_ARG, "unknown precision");
     const std::string key = shape_key(ix, n, precision);
     issl_tuning cached{};
-    int cached_res = 0, cached_od = 0, cached_og = 0;
-    if (cache_lookup(cache_path, key, &cached, &cached_res, &cached_od, &cached_og)) {
+    int cached_res = 0;
+    if (cache_lookup(cache_path, key, &cached, &cached_res)) {
         int rc = issl_set_tuning(ix, &cached);
         if (rc == ISSL_OK) {
             std::lock_guard<std::mutex> lk(ix->mu);
             ix->res_use = cached_res != 0;
-            if (cached_od == 4 || cached_od == 8 || cached_od == 16) ix->ord_depth = cached_od;
-            if (cached_og >= 1 && cached_og <= 32) ix->ord_grid = cached_og;
             *best = ix->tuning;
+            best->ord_depth = ix->ord_depth;
+            best->ord_grid = ix->ord_grid;
+            best->ord_split = ix->ord_split;
             return ok();
         }
     }
@@ -212,45 +221,71 @@ extern "C" int issl_autotune_precision(issl_index *ix, const uint64_t *sample, u
             }
     ix->tuning = top;
     ix->tuned = true;
-    // (3) the ORDERED form's tile kernel (one tile, so one wave, per guide,
-    // hits added in order): load depth and occupancy, on the chosen layout
+    // (3) the ORDERED form's tile kernel (one wave per guide, hits added in
+    // order): load depth and occupancy, on the chosen layout, then the split
+    // threshold for long guides when the index can hold guides that long
     if (precision == ISSL_PREC_FP64 && ordered_fits(ix)) {
+        auto time_ordered = [&](float *out_ms) -> int {
+            float ms_best = 1e30f;
+            for (int r = 0; r < 2 + kTrials; ++r) {
+                ISSL_HIP(hipEventRecord(e0, st));
+                for (int k = 0; k < kPerTrial; ++k) {
+                    const int r0 = launch_ordered(ix, dg, n, sc, top, o, nullptr, ix->ws, st);
+                    if (r0 != ISSL_OK) return r0;
+                }
+                ISSL_HIP(hipEventRecord(e1, st));
+                ISSL_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                ISSL_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (r >= 2 && ms < ms_best) ms_best = ms;   // two warm-up trials
+            }
+            *out_ms = ms_best;
+            return ISSL_OK;
+        };
         float best_ms = 1e30f;
         int bd = ix->ord_depth, bg = ix->ord_grid;
         for (int d : depths)
             for (int gpc : grids) {
                 ix->ord_depth = d;
                 ix->ord_grid = gpc;
-                float ms_best = 1e30f;
-                for (int r = 0; r < 2 + kTrials; ++r) {
-                    ISSL_HIP(hipEventRecord(e0, st));
-                    for (int k = 0; k < kPerTrial; ++k)
-                        if ((rc = launch_ordered(ix, dg, n, sc, top, o, nullptr, ix->ws, st)) != ISSL_OK)
-                            return rc;
-                    ISSL_HIP(hipEventRecord(e1, st));
-                    ISSL_HIP(hipEventSynchronize(e1));
-                    float ms = 0;
-                    ISSL_HIP(hipEventElapsedTime(&ms, e0, e1));
-                    if (r >= 2 && ms < ms_best) ms_best = ms;   // two warm-up trials
-                }
-                if (ms_best < best_ms) {
-                    best_ms = ms_best;
+                float ms = 0;
+                if ((rc = time_ordered(&ms)) != ISSL_OK) return rc;
+                if (ms < best_ms) {
+                    best_ms = ms;
                     bd = d;
                     bg = gpc;
                 }
             }
         ix->ord_depth = bd;
         ix->ord_grid = bg;
+        const int splits[] = {kOrdSplitDefault, 16384, kSplitNever};
+        int bs = ix->ord_split;
+        best_ms = 1e30f;
+        for (int sp : splits) {
+            if (sp != kOrdSplitDefault && ix->p_max <= 16384) continue;   // no guide that long
+            ix->ord_split = sp;
+            float ms = 0;
+            if ((rc = time_ordered(&ms)) != ISSL_OK) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                bs = sp;
+            }
+        }
+        ix->ord_split = bs;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *best = top;
+    best->ord_depth = ix->ord_depth;
+    best->ord_grid = ix->ord_grid;
+    best->ord_split = ix->ord_split;
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%s %d %d %d %d %d %d %.6g %d %d %d\n", key.c_str(), top.chunk, top.block,
+            fprintf(f, "%s %d %d %d %d %d %d %.6g %d %d %d %d\n", key.c_str(), top.chunk, top.block,
                     top.depth, top.grid_per_cu, top.direct_depth, top.heavy_kpost,
-                    top.us_per_guide, ix->res_use ? 1 : 0, ix->ord_depth, ix->ord_grid);
+                    top.us_per_guide, ix->res_use ? 1 : 0, ix->ord_depth, ix->ord_grid,
+                    ix->ord_split);
             fclose(f);
         }
     }

@@ -277,9 +277,19 @@ typedef struct {
                              one wave per guide; 0 = all guides use tiles,
                              ISSL_HEAVY_NEVER = none do */
     double  us_per_guide; /* measured */
+    /* ISSL_FORM_ORDERED's tile kernel (0 in any of the three: the default) */
+    int32_t ord_depth;    /* postings in flight per lane: 4, 8 or 16 */
+    int32_t ord_grid;     /* persistent workgroups per CU: 1..32 */
+    int32_t ord_split;    /* with the exit off (threshold 0), guides with more
+                             postings than this (>= 64) are split over tiles
+                             of `chunk` postings that run in parallel and
+                             write their hits' terms, which one wave then adds
+                             in order; ISSL_SPLIT_NEVER = one tile per guide */
+    int32_t reserved;
 } issl_tuning;
 
 #define ISSL_HEAVY_NEVER 0x7FFFFFFF
+#define ISSL_SPLIT_NEVER 0x7FFFFFFF
 
 /* Time the BATCHED kernel's launch shapes, and the site-word layouts the
  * index holds (issl_index_set_residual), on a sample of guides for this index

@@ -237,3 +237,36 @@ def test_heavy_exact_forms_vs_oracle(heavy):
 def test_heavy_batched_vs_ordered(heavy):
     hx, ix, guides, kinds = heavy
     batched_vs_ordered(ix, guides, "gpu-heavy")
+
+
+def split_vs_unsplit(ix, g, split_at, tag):
+    """ORDERED with long guides split over tiles against one tile per guide:
+    the same bits for every guide, threshold off (split) and on (where the
+    form keeps one tile per guide)."""
+    try:
+        for t in (0.0, 75.0):
+            ix.set_tuning(chunk=4096, depth=8, grid_per_cu=8, ord_depth=8, ord_grid=6,
+                          ord_split=issl.SPLIT_NEVER)
+            print(f"[{tag}] ordered, one tile per guide, threshold {t}", flush=True)
+            one = ix.score(g, method=AND, threshold=t, form=issl.FORM_ORDERED)
+            ix.set_tuning(chunk=4096, depth=8, grid_per_cu=8, ord_depth=8, ord_grid=6,
+                          ord_split=split_at)
+            print(f"[{tag}] ordered, split above {split_at} postings, threshold {t}", flush=True)
+            sp = ix.score(g, method=AND, threshold=t, form=issl.FORM_ORDERED)
+            assert same_bits(sp, one), (tag, t)
+    finally:
+        ix.set_tuning()
+
+
+def test_heavy_ordered_split_bits(heavy):
+    """The heavy guides (over 100,000 postings) split, the rest one tile."""
+    hx, ix, guides, kinds = heavy
+    split_vs_unsplit(ix, guides, 65536, "gpu-heavy")
+
+
+def test_c2_ordered_split_bits(c2):
+    """Every C2 guide over 1,024 postings asks for term entries: the 100,000
+    guides need more than a launch holds, so some are split and the rest fall
+    back to one tile."""
+    hx, ix, guides, kinds = c2
+    split_vs_unsplit(ix, guides, 1024, "gpu-c2")

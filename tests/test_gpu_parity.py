@@ -159,6 +159,41 @@ def test_ordered_shapes_and_layouts(cases, name, shape):
         ix.set_tuning()
 
 
+@pytest.mark.parametrize("name", ["tiny4k_5x4", "mid50k_10x10", "small200k_8x8", "uniform1m_5x4"])
+def test_ordered_split(cases, name):
+    """ORDERED with the split threshold at 64 postings: with the exit off every
+    longer guide is split over 64-posting tiles that run in parallel and write
+    their hits' terms, which one wave adds in order; with the exit armed the
+    guides stay one tile each.  The reference's bits at every golden
+    threshold, both site-word layouts, counts and distinct sites."""
+    hx, ix, gd = cases[name]
+    g = gd["guides"]
+    built, used, _ = ix.residual()
+    try:
+        ix.set_tuning(chunk=64, depth=8, grid_per_cu=4, heavy_kpost=0, ord_split=64)
+        assert ix.tuning().ord_split == 64
+        for res in ([False, True] if built else [False]):
+            ix.set_residual(res)
+            for m, t in itertools.product(METHODS, GOLDEN_THRESHOLDS):
+                r = ix.score(g, method=m, threshold=t, form=issl.FORM_ORDERED)
+                key = f"m{m}_t{tkey(t)}"
+                what = f"{name} split res {res} {key}"
+                assert (bits(r.mit) == bits(gd[key + "_mit"])).all(), what
+                assert (bits(r.cfd) == bits(gd[key + "_cfd"])).all(), what
+                assert (r.passed == gd[key + "_pass"]).all(), what
+            o = orc.score(hx, g, 2, 75.0, distinct=True)
+            for t in (75.0, 0.0):
+                ot = o if t == 75.0 else orc.score(hx, g, 2, 0.0, distinct=True)
+                r = ix.score(g, method=2, threshold=t, form=issl.FORM_ORDERED)
+                assert (r.occ_count == ot["occ_count"]).all() and (r.site_count == ot["site_count"]).all()
+                ix.distinct_reset()
+                ix.score(g, method=2, threshold=t, form=issl.FORM_ORDERED, flags=issl.FLAG_DISTINCT)
+                assert ix.distinct_count() == sum(bin(int(w)).count("1") for w in ot["distinct"]), t
+    finally:
+        ix.set_residual(used)
+        ix.set_tuning()
+
+
 BATCHED_MODES = {
     # every guide through the persistent tile kernel, three tile shapes
     "tiles": dict(chunk=1024, depth=4, grid_per_cu=4, heavy_kpost=0),
