@@ -50,7 +50,18 @@
 
 namespace issl {
 
-constexpr int kMaxTiles = 64;        // tiles per guide at most
+// Tiles per guide at most: kMaxTiles in launches of up to kManyTilesGuides
+// guides, kFewTiles in larger ones (the partials array holds n * cap
+// entries).  A long guide's tiles are at least P / cap postings long.
+constexpr int kMaxTiles = 256;
+constexpr int kFewTiles = 64;
+constexpr uint64_t kManyTilesGuides = 32768;
+__host__ __device__ inline uint32_t tiles_cap(uint64_t n) {
+    return n <= kManyTilesGuides ? static_cast<uint32_t>(kMaxTiles) : static_cast<uint32_t>(kFewTiles);
+}
+// In-order split guides have at most kSplitTiles tiles (k_ordered_terms
+// holds their hit counts one per lane).
+constexpr int kSplitTiles = 64;
 constexpr int kTileBlock = 256;      // threads per persistent workgroup
 constexpr int kTileWaves = kTileBlock / 64;
 
@@ -62,8 +73,8 @@ struct TilePart {
     uint64_t sites;
 };
 
-// Guides per launch: slots n * kMaxTiles stay below 2^32 and the partials
-// (kMaxTiles * n * 32 B) stay modest.
+// Guides per launch: slots n * tiles_cap(n) stay below 2^32 and the partials
+// (tiles_cap(n) * n * 32 B) stay modest.
 constexpr uint64_t kMaxBatch = 1ULL << 18;
 
 // Tile queue: the planner appends to kRegions regions with a counter each,
@@ -83,7 +94,7 @@ constexpr uint32_t kSplitCtr = kTermCtr + kCtrStride;
 constexpr int kQueueCtrs = static_cast<int>(kSplitCtr + kCtrStride);
 // Guides a region takes at most: planner block b (kPlanWaves waves of 64 / S
 // guides each) appends to region b % kRegions; a region of the first list
-// holds their tiles (at most kMaxTiles per guide), of the second one tile per
+// holds their tiles (at most tmax per guide), of the second one tile per
 // guide.
 constexpr uint64_t kPlanWavesPerBlock = 4;
 __host__ __device__ inline uint64_t region_guides(uint64_t n, int S) {
@@ -110,7 +121,8 @@ struct BatchWs {
     // sets back to zero.
     uint32_t *tlist;
     uint32_t *tctr;
-    TilePart *part;      // [kMaxTiles][n] partial of tile (g, k) at slot k*n + g
+    TilePart *part;      // [tmax][n] partial of tile (g, k) at slot k*n + g
+    uint32_t tmax;       // tiles per guide at most in this launch: tiles_cap(n)
     // In-order split (ORDERED guides with more than the split threshold of
     // postings, while term entries last): the guide's tiles run in parallel
     // and write each hit's terms, in the guide's visit order, to entries
@@ -352,7 +364,7 @@ struct PlanArgs {
     const GuideState *state_in;   // guides that took the exit visit nothing
     BatchWs ws;
     // In-order split (0: off): guides with more than split_at postings get
-    // tiles of split_chunk postings or longer (at most kMaxTiles) and term
+    // tiles of split_chunk postings or longer (at most kSplitTiles) and term
     // entries, while ws.term_cap lasts.
     uint32_t split_at;
     uint32_t split_chunk;
@@ -420,7 +432,8 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         a.ws.delta[g * S + s] = len ? static_cast<uint64_t>(s - a.s0) * a.n_sites + b - run : 0;
         if (a.ws.gres) a.ws.gres[g * S + s] = gr;
     }
-    uint64_t c = (static_cast<uint64_t>(P) + kMaxTiles - 1) / kMaxTiles;
+    const uint32_t tmax = a.ws.tmax;
+    uint64_t c = (static_cast<uint64_t>(P) + tmax - 1) / tmax;
     c = (c + 63) / 64 * 64;
     if (c < a.chunk) c = a.chunk;
     uint32_t K = static_cast<uint32_t>((P + c - 1) / c);
@@ -447,7 +460,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         __syncthreads();
         uint64_t start = s_tbase + incl - want;
         for (int w = 0; w < wave; ++w) start += s_want[w];
-        uint64_t cs = (static_cast<uint64_t>(P) + kMaxTiles - 1) / kMaxTiles;
+        uint64_t cs = (static_cast<uint64_t>(P) + kSplitTiles - 1) / kSplitTiles;
         cs = (cs + 63) / 64 * 64;
         if (cs < a.split_chunk) cs = a.split_chunk;
         const uint32_t Ks = static_cast<uint32_t>((P + cs - 1) / cs);
@@ -503,13 +516,13 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
             base_b += s_tot[w][1];
         }
         const uint64_t rg = region_guides(a.n, S);
-        uint32_t *const la = a.ws.tlist + r * rg * kMaxTiles;
-        uint32_t *const lb = a.ws.tlist + kRegions * rg * kMaxTiles + r * rg;
+        uint32_t *const la = a.ws.tlist + r * rg * tmax;
+        uint32_t *const lb = a.ws.tlist + kRegions * rg * tmax + r * rg;
         const uint32_t n32 = static_cast<uint32_t>(a.n);
         // A region holds all its blocks' tiles; the bound only guards the
         // list against counters that were not zero.
         for (uint32_t k = 0; k < ka; ++k)
-            if (base_a + k < rg * kMaxTiles) la[base_a + k] = k * n32 + static_cast<uint32_t>(g);
+            if (base_a + k < rg * tmax) la[base_a + k] = k * n32 + static_cast<uint32_t>(g);
         if (kb && base_b < rg) lb[base_b] = static_cast<uint32_t>(g);
     }
 }
@@ -550,7 +563,7 @@ struct TileArgs {
     const SliceMeta *meta;
     const void *mit;        // float or double by precision
     const void *cfd;
-    uint32_t n;             // guides; n * kMaxTiles < 2^32
+    uint32_t n;             // guides; n * ws.tmax < 2^32
     int S;
     int mark_distinct;      // mark distinct sites here (no early exit armed)
     int calc_mit, calc_cfd;
@@ -690,7 +703,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
     const Real cfd_pam = s_cfd[320 + 10];   // NGG
     const int S = a.S;
     const uint32_t n = a.n;
-    const uint32_t V = n * kMaxTiles;
+    const uint32_t V = n * a.ws.tmax;
     const uint32_t W = gridDim.x * kTileWaves;
     const uint32_t w = blockIdx.x * kTileWaves + wave;
 
@@ -710,7 +723,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
         rb = region_guides(n, a.S);
         // (the bounds only guard the list against counters that were not zero)
         const uint32_t ca = lane < static_cast<int>(kRegions)
-                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[lane * kCtrStride], rb * kMaxTiles)) : 0;
+                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[lane * kCtrStride], rb * a.ws.tmax)) : 0;
         const uint32_t cb = lane < static_cast<int>(kRegions)
                                 ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[(kRegions + lane) * kCtrStride], rb)) : 0;
         uint32_t ia = ca, ib = cb;
@@ -735,14 +748,14 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
             const uint64_t m = __ballot(lane < static_cast<int>(kRegions) && pre_a <= c);
             const int r = 63 - __builtin_clzll(m);
             const uint32_t off = static_cast<uint32_t>(c) - static_cast<uint32_t>(__shfl(static_cast<int>(pre_a), r));
-            return a.ws.tlist[r * rb * kMaxTiles + off];
+            return a.ws.tlist[r * rb * a.ws.tmax + off];
         }
         c -= tot_a;
         if (c < tot_b) {
             const uint64_t m = __ballot(lane < static_cast<int>(kRegions) && pre_b <= c);
             const int r = 63 - __builtin_clzll(m);
             const uint32_t off = static_cast<uint32_t>(c) - static_cast<uint32_t>(__shfl(static_cast<int>(pre_b), r));
-            return a.ws.tlist[kRegions * rb * kMaxTiles + r * rb + off];
+            return a.ws.tlist[kRegions * rb * a.ws.tmax + r * rb + off];
         }
         return kNoSlot;
     };
@@ -1297,7 +1310,9 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     const uint32_t K = a.ws.gtiles[g];
     if (K == 0 && a.skip_untiled) return;
     if (K == 1 && a.finish_single) return;  // finished by the wave that scored its tile
+    // Lane l holds the partial of tile k0 + l, 64 tiles at a time.
     TilePart mine{};
+    uint32_t k0 = 0;
     if (lane < static_cast<int>(K)) mine = a.ws.part[static_cast<uint64_t>(lane) * a.n + g];
 
     // Sums so far (FP64): zero, or the state carried from earlier slice
@@ -1328,11 +1343,17 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
 
     int exit_tile = -1;
     for (uint32_t k = 0; k < K; ++k) {
-        const uint64_t o = shfl64(mine.occ, k);
-        const uint64_t si = shfl64(mine.sites, k);
+        if (k - k0 == 64) {
+            k0 = k;
+            mine = TilePart{};
+            if (k0 + lane < K) mine = a.ws.part[static_cast<uint64_t>(k0 + lane) * a.n + g];
+        }
+        const int kl = static_cast<int>(k - k0);
+        const uint64_t o = shfl64(mine.occ, kl);
+        const uint64_t si = shfl64(mine.sites, kl);
         const double qm = dm, qc = dc;
-        dm += __shfl(mine.m, k);
-        dc += __shfl(mine.c, k);
+        dm += __shfl(mine.m, kl);
+        dc += __shfl(mine.c, kl);
         if (!(a.sc.use_cut && bexit(a.sc.method, dm, dc, a.sc.cut))) {
             occ_sum += o;
             sites += si;
@@ -1529,7 +1550,7 @@ __global__ __launch_bounds__(256) void k_ordered_terms(FinArgs a) {
     for (uint32_t i = blockIdx.x * 4 + static_cast<uint32_t>(wave); i < nsplit; i += W) {
         const uint64_t g = a.ws.slist[i];
         const uint32_t K = a.ws.gtiles[g], C = a.ws.gchunk[g], tb = a.ws.gterm[g];
-        // Tile hit counts, lane k holding tile k's (K <= kMaxTiles = 64), so
+        // Tile hit counts, lane k holding tile k's (K <= kSplitTiles = 64), so
         // the cursor below moves without loads that would wait behind the
         // batch loads in flight.
         const uint32_t cnt_l = static_cast<uint32_t>(lane) < K
@@ -1722,7 +1743,7 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
                 }
                 if (lane == 0) {
                     a.ws.pre[gg * (S + 1) + S] = static_cast<uint32_t>(P);
-                    uint64_t c = (P + kMaxTiles - 1) / kMaxTiles;
+                    uint64_t c = (P + a.ws.tmax - 1) / a.ws.tmax;
                     c = (c + 63) / 64 * 64;
                     if (c < a.chunk) c = a.chunk;
                     a.ws.gchunk[gg] = static_cast<uint32_t>(c);
@@ -1901,13 +1922,13 @@ size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 // term entries for the in-order split (0: none).
 struct WsSizes {
     size_t pre, gres, delta, u32, part, list, ctr, term_mc, term_oi, split, total;
-    WsSizes(uint64_t n, int S, bool res, uint64_t term_cap = 0) {
+    WsSizes(uint64_t n, int S, bool res, uint32_t tmax, uint64_t term_cap = 0) {
         pre = align256(n * (S + 1) * 4);
         gres = res ? align256(n * std::max(S, 1) * 4) : 0;
         delta = align256(n * std::max(S, 1) * 8);
         u32 = align256(n * 4);
-        part = align256(n * kMaxTiles * sizeof(TilePart));
-        list = align256(kRegions * region_guides(n, S) * (kMaxTiles + 1) * 4);
+        part = align256(n * tmax * sizeof(TilePart));
+        list = align256(kRegions * region_guides(n, S) * (tmax + 1) * 4);
         ctr = align256(kQueueCtrs * 4);
         term_mc = align256(term_cap * sizeof(double2));
         term_oi = align256(term_cap * sizeof(uint2));
@@ -1918,8 +1939,15 @@ struct WsSizes {
 
 // fresh: the queue counters are not known to be zero (the buffer was
 // (re)allocated, or is the caller's).
+// A caller's workspace (ws.ext, the allocation-free entry point) keeps the
+// smaller tile cap, so that a fixed size holds more guides.
+uint32_t ws_tiles_cap(const Workspace &ws, uint64_t n) {
+    return ws.ext ? static_cast<uint32_t>(kFewTiles) : tiles_cap(n);
+}
+
 int carve(Workspace &ws, uint64_t n, int S, bool res, uint64_t term_cap, BatchWs *b, bool *fresh) {
-    const WsSizes z(n, S, res, term_cap);
+    const uint32_t tmax = ws_tiles_cap(ws, n);
+    const WsSizes z(n, S, res, tmax, term_cap);
     char *p;
     if (ws.ext) {
         if (z.total > ws.ext_bytes)
@@ -1953,6 +1981,7 @@ int carve(Workspace &ws, uint64_t n, int S, bool res, uint64_t term_cap, BatchWs
     p += z.part;
     b->tlist = reinterpret_cast<uint32_t *>(p);
     p += z.list;
+    b->tmax = tmax;
     b->gres = res ? reinterpret_cast<uint32_t *>(p) : nullptr;
     p += z.gres;
     b->term_cap = static_cast<uint32_t>(term_cap);
@@ -2161,7 +2190,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     ta.ws = b;
     // Persistent grid: grid_per_cu workgroups per CU, no more waves than slots.
     uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
-    grid = std::min<uint64_t>(grid, (n * kMaxTiles + kTileWaves - 1) / kTileWaves);
+    grid = std::min<uint64_t>(grid, (n * b.tmax + kTileWaves - 1) / kTileWaves);
     auto tiles_launch = [&](hipEvent_t e0, hipEvent_t e1) {
         if (ordered) return launch_ordered_depth(ta, tn.depth, static_cast<int>(grid), st, e0, e1);
         return fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st, e0, e1)
@@ -2218,7 +2247,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
 
 }  // namespace
 
-size_t batch_ws_bytes(uint64_t n, int S, bool res) { return WsSizes(n, S, res).total; }
+size_t batch_ws_bytes(uint64_t n, int S, bool res) { return WsSizes(n, S, res, kFewTiles).total; }
 
 issl_tuning default_tuning() {
     issl_tuning t{};

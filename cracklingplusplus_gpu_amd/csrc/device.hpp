@@ -244,7 +244,8 @@ int launch_plain(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
                  const DevResults &out, uint64_t *distinct, const uint64_t *cand_off,
                  uint32_t *cand_ids, hipStream_t stream, const Window *win = nullptr);
 // Bytes of the BATCHED / ORDERED workspace one launch sequence carves for n
-// guides over S slices (res: with the list-residual codes).
+// guides over S slices (res: with the list-residual codes) in a caller's
+// workspace (Workspace::ext), which holds no in-order split terms.
 size_t batch_ws_bytes(uint64_t n, int S, bool res);
 int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const ScoreScalars &sc,
                    int precision, int chunk, const issl_tuning &tn, const DevResults &out,
