@@ -602,6 +602,26 @@ __device__ __forceinline__ double readlane_f64(double v, uint32_t src) {
     return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
 }
 
+// Adds p[0], p[2], p[4], ... (n terms, every other double) to `a`, one at a
+// time in order, U terms read before they are added.  Two lanes run it
+// together on the (MIT, CFD) pairs of an LDS array of double2, lane 0 from .x
+// and lane 1 from .y: one FP64 addition per term and chain, where one lane
+// holding both sums issues two.
+template <int U>
+__device__ __forceinline__ void add_chain(const double *p, uint32_t n, double &a) {
+    double acc = a;
+    uint32_t t = 0;
+    for (; t + U <= n; t += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = p[2 * (t + u)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += v[u];
+    }
+    for (; t < n; ++t) acc += p[2 * t];
+    a = acc;
+}
+
 // ORD: every guide is a single tile (the planner's tile length covers the
 // longest guide), and each batch of hits is added to two wave-uniform FP64
 // sums one hit at a time, in hit-ring order -- the guide's slice-major
@@ -995,9 +1015,10 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                     s_ord[wave][lane] = make_double2(static_cast<double>(h_tm), static_cast<double>(h_tc));
                 __builtin_amdgcn_wave_barrier();
                 uint32_t acc = cnt, ex = 0;
-                if (lane == 0) {
-                    double m = run_m, c = run_c;
-                    if (a.sc.use_cut) {
+                if (a.sc.use_cut) {
+                    // The exit test needs both sums after every hit: lane 0.
+                    if (lane == 0) {
+                        double m = run_m, c = run_c;
                         for (uint32_t i = 0; i < cnt; ++i) {
                             const double2 t = s_ord[wave][i];
                             m += t.x;
@@ -1008,20 +1029,21 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                                 break;
                             }
                         }
-                    } else {
-#pragma unroll 8
-                        for (uint32_t i = 0; i < cnt; ++i) {
-                            const double2 t = s_ord[wave][i];
-                            m += t.x;
-                            c += t.y;
-                        }
+                        run_m = m;
+                        run_c = c;
                     }
-                    run_m = m;
-                    run_c = c;
+                    __builtin_amdgcn_wave_barrier();
+                    run_m = readlane_f64(run_m, 0);
+                    run_c = readlane_f64(run_c, 0);
+                } else {
+                    // No exit: lane 0 adds the MIT terms and lane 1 the CFD
+                    // terms, one addition per hit each (add_chain).
+                    double ch = lane == 1 ? run_c : run_m;
+                    if (lane < 2) add_chain<8>(reinterpret_cast<const double *>(s_ord[wave]) + lane, cnt, ch);
+                    __builtin_amdgcn_wave_barrier();
+                    run_m = readlane_f64(ch, 0);
+                    run_c = readlane_f64(ch, 1);
                 }
-                __builtin_amdgcn_wave_barrier();
-                run_m = readlane_f64(run_m, 0);
-                run_c = readlane_f64(run_c, 0);
                 acc = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(acc)));
                 stop = __builtin_amdgcn_readfirstlane(static_cast<int>(ex)) != 0;
                 // Counts and distinct sites of the hits up to the exit.
@@ -1507,39 +1529,9 @@ __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
 // off, so every hit counts); it also sums occurrences and sites and marks
 // distinct sites.  Lanes fetch
 // kOrdTermBatch entries at a time into registers and then LDS, the next
-// batch's loads in flight while the wave adds the current one (add_in_order).
+// batch's loads in flight while lanes 0 and 1 add the current one (add_chain).
 constexpr int kOrdTermPer = 8;                    // entries per lane per batch
 constexpr int kOrdTermBatch = 64 * kOrdTermPer;
-
-// Adds src[0 .. n) to (m, c) one term at a time, in order.  Every lane of
-// the wave runs it with the same values (the terms are broadcast LDS reads),
-// so the loop is uniform; sixteen terms are read before they are added.
-__device__ __forceinline__ void add_in_order(const double2 *src, uint32_t n, double &m, double &c) {
-    constexpr int U = 16;
-    double mm = m, cc = c;
-    uint32_t t = 0;
-    for (; t + U <= n; t += U) {
-        double vm[U], vc[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const double2 v = src[t + u];
-            vm[u] = v.x;
-            vc[u] = v.y;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            mm += vm[u];
-            cc += vc[u];
-        }
-    }
-    for (; t < n; ++t) {
-        const double2 v = src[t];
-        mm += v.x;
-        cc += v.y;
-    }
-    m = mm;
-    c = cc;
-}
 
 __global__ __launch_bounds__(256) void k_ordered_terms(FinArgs a) {
     __shared__ double2 s_t[4][kOrdTermBatch];
@@ -1576,7 +1568,7 @@ __global__ __launch_bounds__(256) void k_ordered_terms(FinArgs a) {
         uint32_t sites = 0;
         for (;;) {
             // The batch in registers goes to LDS; the next one is fetched and
-            // its loads stay in flight while the wave adds this one.
+            // its loads stay in flight while lanes 0 and 1 add this one.
             uint32_t ncur = 0;
             if (have) {
 #pragma unroll
@@ -1618,7 +1610,11 @@ __global__ __launch_bounds__(256) void k_ordered_terms(FinArgs a) {
                 continue;
             }
             __builtin_amdgcn_wave_barrier();
-            add_in_order(s_t[wave], ncur, m, c);
+            double ch = lane == 1 ? c : m;
+            if (lane < 2) add_chain<16>(reinterpret_cast<const double *>(s_t[wave]) + lane, ncur, ch);
+            __builtin_amdgcn_wave_barrier();
+            m = readlane_f64(ch, 0);
+            c = readlane_f64(ch, 1);
 #pragma unroll
             for (int r = 0; r < kOrdTermPer; ++r) {
                 const uint32_t e = static_cast<uint32_t>(r * 64 + lane);
