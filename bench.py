@@ -797,6 +797,18 @@ def main() -> int:
                    for a, b in zip(rot.results()[:covered], res_plain) for k in a)
     log(f"ordered: {ord_ms:.4f} ms/step, kernel {ord_k_us:.1f} us, bits of plain {ord_same}")
 
+    # ---- threshold 75 (the module's usual setting): ORDERED and PLAIN take the
+    # reference's early exit, hit by hit
+    kw75 = dict(kw, threshold=75.0)
+    ord75_ms = timed(hiprt, rot, ix, args.steps, args.warmup, **dict(kw75, form=issl.FORM_ORDERED))
+    ord75 = rot.results()[:covered]
+    plain75_ms = timed(hiprt, rot, ix, args.plain_steps, 1, **dict(kw75, form=issl.FORM_PLAIN))
+    ord75_same = all(np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
+                     for a, b in zip(ord75, rot.results()[:covered]) for k in a)
+    bat75_ms = timed(hiprt, rot, ix, args.steps, args.warmup, **kw75)
+    log(f"threshold 75: ordered {ord75_ms:.4f} ms/step (bits of plain {ord75_same}), "
+        f"plain {plain75_ms:.4f}, batched {bat75_ms:.4f}")
+
     # ---- precision against the in-order FP64 path (PLAIN), every batch
     cat = lambda rs, f: np.concatenate([r[f] for r in rs])  # noqa: E731
     prec = {}
@@ -915,6 +927,14 @@ def main() -> int:
                     "plain_speedup": round(plain_ms / ord_ms, 3),
                     "mode": "ORDERED: FP64 hits added in the reference's order, one wave "
                             "per guide in the persistent tile kernel"},
+        "threshold_75": {"ordered": {"value": round(Q / (ord75_ms * 1e-3), 1),
+                                     "ms_per_step": round(ord75_ms, 4),
+                                     "same_bits_as_plain": bool(ord75_same)},
+                         "plain": {"value": round(Q / (plain75_ms * 1e-3), 1),
+                                   "ms_per_step": round(plain75_ms, 4)},
+                         "batched_fp64": {"value": round(Q / (bat75_ms * 1e-3), 1),
+                                          "ms_per_step": round(bat75_ms, 4)},
+                         "mode": "MIT+CFD at threshold 75: the exit is armed (cut 33.3)"},
         "plain": {"value": round(Q / (plain_ms * 1e-3), 1), "unit": "guides/s",
                   "ms_per_step": round(plain_ms, 4), "kernel_us": round(plain_k_us, 2),
                   "achieved_GBps": round(bytes_ref / (plain_k_us * 1e-6) / 1e9, 1),

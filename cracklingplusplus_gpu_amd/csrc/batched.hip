@@ -622,6 +622,29 @@ __device__ __forceinline__ void add_chain(const double *p, uint32_t n, double &a
     a = acc;
 }
 
+// add_chain that also leaves the running sum after each term in place of the
+// term: p[2t] becomes a + p[0] + p[2] + ... + p[2t], added in that order.
+template <int U>
+__device__ __forceinline__ void add_chain_prefix(double *p, uint32_t n, double &a) {
+    double acc = a;
+    uint32_t t = 0;
+    for (; t + U <= n; t += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = p[2 * (t + u)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc += v[u];
+            p[2 * (t + u)] = acc;
+        }
+    }
+    for (; t < n; ++t) {
+        acc += p[2 * t];
+        p[2 * t] = acc;
+    }
+    a = acc;
+}
+
 // ORD: every guide is a single tile (the planner's tile length covers the
 // longest guide), and each batch of hits is added to two wave-uniform FP64
 // sums one hit at a time, in hit-ring order -- the guide's slice-major
@@ -1016,25 +1039,29 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(TileArgs a) {
                 __builtin_amdgcn_wave_barrier();
                 uint32_t acc = cnt, ex = 0;
                 if (a.sc.use_cut) {
-                    // The exit test needs both sums after every hit: lane 0.
-                    if (lane == 0) {
-                        double m = run_m, c = run_c;
-                        for (uint32_t i = 0; i < cnt; ++i) {
-                            const double2 t = s_ord[wave][i];
-                            m += t.x;
-                            c += t.y;
-                            if (bexit(a.sc.method, m, c, a.sc.cut)) {
-                                acc = i + 1;
-                                ex = 1;
-                                break;
-                            }
-                        }
-                        run_m = m;
-                        run_c = c;
-                    }
+                    // Lanes 0 and 1 leave the running MIT and CFD sums after
+                    // each hit in place of its terms; lane i then tests the
+                    // exit on the sums after hit i, and the first hit that
+                    // passes is where the one-at-a-time loop stops, with
+                    // those sums.
+                    double *const pd = reinterpret_cast<double *>(s_ord[wave]);
+                    double ch = lane == 1 ? run_c : run_m;
+                    if (lane < 2) add_chain_prefix<8>(pd + lane, cnt, ch);
                     __builtin_amdgcn_wave_barrier();
-                    run_m = readlane_f64(run_m, 0);
-                    run_c = readlane_f64(run_c, 0);
+                    bool passes = false;
+                    if (static_cast<uint32_t>(lane) < cnt)
+                        passes = bexit(a.sc.method, pd[2 * lane], pd[2 * lane + 1], a.sc.cut);
+                    const uint64_t bx = __ballot(passes);
+                    if (bx) {
+                        const uint32_t f = static_cast<uint32_t>(__builtin_ctzll(bx));
+                        acc = f + 1;
+                        ex = 1;
+                        run_m = pd[2 * f];
+                        run_c = pd[2 * f + 1];
+                    } else {
+                        run_m = readlane_f64(ch, 0);
+                        run_c = readlane_f64(ch, 1);
+                    }
                 } else {
                     // No exit: lane 0 adds the MIT terms and lane 1 the CFD
                     // terms, one addition per hit each (add_chain).
