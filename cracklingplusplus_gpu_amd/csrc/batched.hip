@@ -603,10 +603,10 @@ __device__ __forceinline__ double readlane_f64(double v, uint32_t src) {
 }
 
 // Adds p[0], p[2], p[4], ... (n terms, every other double) to `a`, one at a
-// time in order, U terms read before they are added.  Run on the (MIT, CFD)
-// pairs of an LDS array of double2 from .x by even lanes and from .y by odd
-// lanes, it advances both sums with one FP64 addition per term, where one
-// lane holding both sums issues two.
+// time in order, U terms read before they are added.  Lanes 0 and 1 run it
+// together on the (MIT, CFD) pairs of an LDS array of double2, lane 0 from .x
+// and lane 1 from .y: one FP64 addition per term and chain, where one lane
+// holding both sums issues two.
 template <int U>
 __device__ __forceinline__ void add_chain(const double *p, uint32_t n, double &a) {
     double acc = a;
@@ -1063,12 +1063,11 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
                         run_c = readlane_f64(ch, 1);
                     }
                 } else {
-                    // No exit: even lanes add the MIT terms and odd lanes the
-                    // CFD terms, one addition per hit each (add_chain); every
-                    // lane runs the loop, so its control stays scalar.
-                    const int par = lane & 1;
-                    double ch = par ? run_c : run_m;
-                    add_chain<8>(reinterpret_cast<const double *>(s_ord[wave]) + par, cnt, ch);
+                    // No exit: lane 0 adds the MIT terms and lane 1 the CFD
+                    // terms, one addition per hit each (add_chain).
+                    double ch = lane == 1 ? run_c : run_m;
+                    if (lane < 2) add_chain<8>(reinterpret_cast<const double *>(s_ord[wave]) + lane, cnt, ch);
+                    __builtin_amdgcn_wave_barrier();
                     run_m = readlane_f64(ch, 0);
                     run_c = readlane_f64(ch, 1);
                 }
@@ -1557,7 +1556,7 @@ __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
 // off, so every hit counts); it also sums occurrences and sites and marks
 // distinct sites.  Lanes fetch
 // kOrdTermBatch entries at a time into registers and then LDS, the next
-// batch's loads in flight while the wave adds the current one (add_chain).
+// batch's loads in flight while lanes 0 and 1 add the current one (add_chain).
 constexpr int kOrdTermPer = 8;                    // entries per lane per batch
 constexpr int kOrdTermBatch = 64 * kOrdTermPer;
 
@@ -1596,7 +1595,7 @@ __global__ __launch_bounds__(256) void k_ordered_terms(FinArgs a) {
         uint32_t sites = 0;
         for (;;) {
             // The batch in registers goes to LDS; the next one is fetched and
-            // its loads stay in flight while the wave adds this one.
+            // its loads stay in flight while lanes 0 and 1 add this one.
             uint32_t ncur = 0;
             if (have) {
 #pragma unroll
@@ -1638,11 +1637,10 @@ __global__ __launch_bounds__(256) void k_ordered_terms(FinArgs a) {
                 continue;
             }
             __builtin_amdgcn_wave_barrier();
-            // Even lanes add the MIT terms, odd lanes the CFD terms (the loop
-            // is the same on every lane, so its control stays scalar).
-            const int par = lane & 1;
-            double ch = par ? c : m;
-            add_chain<16>(reinterpret_cast<const double *>(s_t[wave]) + par, ncur, ch);
+            // Lane 0 adds the MIT terms, lane 1 the CFD terms.
+            double ch = lane == 1 ? c : m;
+            if (lane < 2) add_chain<16>(reinterpret_cast<const double *>(s_t[wave]) + lane, ncur, ch);
+            __builtin_amdgcn_wave_barrier();
             m = readlane_f64(ch, 0);
             c = readlane_f64(ch, 1);
 #pragma unroll
