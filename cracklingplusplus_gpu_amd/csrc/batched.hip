@@ -43,8 +43,10 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "device.hpp"
 
@@ -84,17 +86,21 @@ constexpr uint64_t kMaxBatch = 1ULL << 18;
 constexpr uint32_t kRegions = 32;
 constexpr uint32_t kSubQueues = 256;
 constexpr uint32_t kCtrStride = 16;
-// Region counter i (i < 2 kRegions: first list's regions, then the second's)
-// sits at tctr[i * kCtrStride].
-constexpr uint32_t kSubQueueBase = 2 * kRegions * kCtrStride;   // taken counters
+// The queue is kLists lists, longest tiles first: list 0 holds the tiles of
+// several-tile guides and the single tiles longer than half a chunk; list j
+// (1 <= j < kLists) the single tiles of (h / 2^j, h / 2^(j-1)] postings for
+// h = half a chunk, the last list also the shorter ones.  Region counter
+// (list j, region r) sits at tctr[(j * kRegions + r) * kCtrStride].
+constexpr uint32_t kLists = 6;
+constexpr uint32_t kSubQueueBase = kLists * kRegions * kCtrStride;   // taken counters
 constexpr uint32_t kSubQueueCtr = kCtrStride * kSubQueues;      // taken -> finished
 // In-order split: the term entries handed out (64-bit) and the split guides.
 constexpr uint32_t kTermCtr = kSubQueueBase + 2 * kSubQueueCtr;
 constexpr uint32_t kSplitCtr = kTermCtr + kCtrStride;
 constexpr int kQueueCtrs = static_cast<int>(kSplitCtr + kCtrStride);
 // Guides a region takes at most: planner block b (kPlanWaves waves of 64 / S
-// guides each) appends to region b % kRegions; a region of the first list
-// holds their tiles (at most tmax per guide), of the second one tile per
+// guides each) appends to region b % kRegions; a region of list 0 holds
+// their tiles (at most tmax per guide), of the other lists one tile per
 // guide.
 constexpr uint64_t kPlanWavesPerBlock = 4;
 __host__ __device__ inline uint64_t region_guides(uint64_t n, int S) {
@@ -111,12 +117,11 @@ struct BatchWs {
     // layout), with s in bits 24..29; null: the signature-inline layout.
     uint32_t *gres;
     // Tile queue (null: the slot schedule).  Planner workgroup b appends the
-    // slots of its guides with several tiles or a single tile longer than half
-    // a chunk to region b % kRegions of the first list and of its other
-    // single-tile guides to the same region of the second, counting them in
-    // tctr[r * kCtrStride] and tctr[(kRegions + r) * kCtrStride] (set back to
-    // zero by k_finalize).  The queue is the first list's regions in order,
-    // then the second's.  The tile kernel splits it into sub-queues, each with
+    // slots of its guides to region b % kRegions of the list of their size
+    // class (kLists), counting them in that region's counter (set back to
+    // zero by k_finalize).  The queue is list 0's regions in order, then list
+    // 1's, and so on: list 0 at tlist, kRegions regions of rb * tmax entries;
+    // list j >= 1 after it, kRegions regions of rb entries each.  The tile kernel splits it into sub-queues, each with
     // a taken and a finished counter (kSubQueueCtr apart), which its last wave
     // sets back to zero.
     uint32_t *tlist;
@@ -391,8 +396,8 @@ __host__ __device__ inline uint32_t plan_guides_per_wave(int S) {
 }
 
 __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
-    __shared__ uint32_t s_tot[kPlanWaves][2];
-    __shared__ uint32_t s_base[2];
+    __shared__ uint32_t s_tot[kPlanWaves][kLists];
+    __shared__ uint32_t s_base[kLists];
     __shared__ uint64_t s_want[kPlanWaves];
     __shared__ uint64_t s_tbase;
     const int S = a.S;
@@ -478,52 +483,63 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
     }
     if (a.ws.tlist) {
         // Append the workgroup's tiles to the queue: one atomic per list per
-        // workgroup, on the counter of its region.  The first list takes the
-        // tiles likely to cost most (several-tile guides, and single tiles
-        // longer than first_min, half a chunk), so that they start early and
-        // the cheap ones fill the end.
+        // workgroup, on the counter of its region.  Tiles go to the list of
+        // their size class, longest first (the queue's order), so that the
+        // costly ones start early and the cheap ones fill the end: list 0
+        // takes several-tile guides and single tiles longer than first_min
+        // (half a chunk), list j the single tiles of (first_min / 2^j,
+        // first_min / 2^(j-1)] postings, the last list all shorter ones.
         const bool first = K > 1 || (K == 1 && P > a.first_min);
-        const uint32_t ka = (lead && first) ? K : 0, kb = (lead && !first && K == 1) ? 1 : 0;
-        uint32_t ia = ka, ib = kb;
+        const bool single = lead && !first && K == 1;
+        uint32_t cls = 0;
+        if (!first) {
+            cls = 1;
+            for (uint32_t t = a.first_min >> 1; cls < kLists - 1 && P <= t; t >>= 1) ++cls;
+        }
+        const uint32_t ka = (lead && first) ? K : 0;
+        uint32_t ia = ka;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t va = static_cast<uint32_t>(__shfl(static_cast<int>(ia), lane >= off ? lane - off : lane));
-            const uint32_t vb = static_cast<uint32_t>(__shfl(static_cast<int>(ib), lane >= off ? lane - off : lane));
-            if (lane >= off) {
-                ia += va;
-                ib += vb;
-            }
+            if (lane >= off) ia += va;
         }
-        if (lane == 63) {
-            s_tot[wave][0] = ia;
-            s_tot[wave][1] = ib;
+        // Single tiles: one per lane of the class's ballot.
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t j = 1; j < kLists; ++j) {
+            const uint64_t m = __ballot(single && cls == j);
+            if (single && cls == j)
+                rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+            if (lane == 0) s_tot[wave][j] = static_cast<uint32_t>(__popcll(m));
         }
+        if (lane == 63) s_tot[wave][0] = ia;
         __syncthreads();
         const uint32_t r = blockIdx.x % kRegions;
-        if (threadIdx.x == 0) {
-            uint32_t ta = 0, tb = 0;
-            for (int w = 0; w < kPlanWaves; ++w) {
-                ta += s_tot[w][0];
-                tb += s_tot[w][1];
-            }
-            s_base[0] = ta ? atomicAdd(a.ws.tctr + r * kCtrStride, ta) : 0u;
-            s_base[1] = tb ? atomicAdd(a.ws.tctr + (kRegions + r) * kCtrStride, tb) : 0u;
+        if (threadIdx.x < kLists) {
+            const uint32_t j = threadIdx.x;
+            uint32_t t = 0;
+            for (int w = 0; w < kPlanWaves; ++w) t += s_tot[w][j];
+            s_base[j] = t ? atomicAdd(a.ws.tctr + (j * kRegions + r) * kCtrStride, t) : 0u;
         }
         __syncthreads();
-        uint32_t base_a = s_base[0] + ia - ka, base_b = s_base[1] + ib - kb;
-        for (int w = 0; w < wave; ++w) {
-            base_a += s_tot[w][0];
-            base_b += s_tot[w][1];
-        }
         const uint64_t rg = region_guides(a.n, S);
-        uint32_t *const la = a.ws.tlist + r * rg * tmax;
-        uint32_t *const lb = a.ws.tlist + kRegions * rg * tmax + r * rg;
         const uint32_t n32 = static_cast<uint32_t>(a.n);
-        // A region holds all its blocks' tiles; the bound only guards the
-        // list against counters that were not zero.
-        for (uint32_t k = 0; k < ka; ++k)
-            if (base_a + k < rg * tmax) la[base_a + k] = k * n32 + static_cast<uint32_t>(g);
-        if (kb && base_b < rg) lb[base_b] = static_cast<uint32_t>(g);
+        // A region holds all its blocks' tiles; the bounds only guard the
+        // lists against counters that were not zero.
+        if (ka) {
+            uint32_t base = s_base[0] + ia - ka;
+            for (int w = 0; w < wave; ++w) base += s_tot[w][0];
+            uint32_t *const la = a.ws.tlist + r * rg * tmax;
+            for (uint32_t k = 0; k < ka; ++k)
+                if (base + k < rg * tmax) la[base + k] = k * n32 + static_cast<uint32_t>(g);
+        }
+        if (single) {
+            uint32_t base = s_base[cls] + rank;
+            for (int w = 0; w < wave; ++w) base += s_tot[w][cls];
+            uint32_t *const lj = a.ws.tlist + kRegions * rg * tmax + ((cls - 1) * kRegions + r) * rg;
+            if (base < rg) lj[base] = static_cast<uint32_t>(g);
+        }
     }
 }
 
@@ -561,16 +577,14 @@ struct TileArgs {
     uint32_t res_shift;     // their first occurrence bit
     const uint64_t *post;   // postings (escape occ, ids)
     const SliceMeta *meta;
-    const void *mit;        // float or double by precision
+    const double *mitf;     // MIT closed-form factors (kMitFactorEntries)
     const void *cfd;
     uint32_t n;             // guides; n * ws.tmax < 2^32
     int S;
     int mark_distinct;      // mark distinct sites here (no early exit armed)
     int calc_mit, calc_cfd;
     uint64_t *distinct;
-    ScoreScalars sc;        // a guide with one tile and no exit armed is finished here
-    DevResults out;         // (when finish_single is set)
-    int finish_single;
+    ScoreScalars sc;        // the exit test of the in-order form
     BatchWs ws;
 };
 
@@ -653,7 +667,7 @@ __device__ __forceinline__ void add_chain_prefix(double *p, uint32_t n, double &
 // exit point are the reference's bit for bit; after the exit the wave stops
 // streaming the guide.  (FP64 and the tile queue only.)
 template <int PREC, int DEPTH, bool QUEUE, bool RES, bool ORD = false>
-__global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_tiles(TileArgs a) {
+__global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 1) void k_tiles(TileArgs a) {
     static_assert(!ORD || (PREC == ISSL_PREC_FP64 && QUEUE), "in-order tiles are FP64 and queued");
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
     // Site word as streamed and ringed; the hit ring holds the word (or, in
@@ -662,7 +676,14 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
     __shared__ Real s_cfd[kCfdEntries];
     __shared__ uint64_t s_fold[kMaxSlices];
     __shared__ uint64_t s_posmask[kNoPos + 1];
-    __shared__ uint32_t s_rank[3][RES ? 128 : kRankTerms];
+    // MIT closed-form factors: 1 - weight by position, (t2, t3) by count and
+    // span (kMitOneMinus + 2 (20 k + span)).
+    __shared__ double s_om[kMitOneMinus];
+    __shared__ double2 s_t23[5 * 20];
+    // Tile queue: each list's regions' exclusive prefixes, and the lists'
+    // first entries (s_qbase[kLists]: the queue's length).
+    __shared__ uint32_t s_qpre[QUEUE ? kLists : 1][kRegions];
+    __shared__ uint32_t s_qbase[kLists + 1];
     // The current tile's CFD factors by (position, guide ^ site there).
     __shared__ Real s_cq[kTileWaves][(kNoPos + 1) * 4];
     // Arrays sized by the slice count S, in dynamic LDS (tiles_lds_bytes):
@@ -701,22 +722,37 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
     auto delta_of = [&](int b) { return d_delta + (wave * 2 + b) * SD; };
     auto gres_of = [&](int b) { return d_gres + (wave * 2 + b) * SD; };
     const Real *cfdg = static_cast<const Real *>(a.cfd);
-    const Real *mitg = static_cast<const Real *>(a.mit);
+    // Queue schedule: sub-queue sq (of nq) is served by the workgroups b with
+    // b % nq == sq, sq_waves waves in all, and holds queue entries sq,
+    // sq + nq, sq + 2 nq, ...  The wave of rank r among them takes entry r
+    // first; every later tile comes from a ticket t (the sub-queue's taken
+    // counter), entry sq_waves + t, taken when the wave has streamed its
+    // current tile: the queue is ordered longest first, and a wave takes its
+    // next tile only when it is nearly free, so the short tiles at the end
+    // go to the waves that finish first.  The queue regions' counts are
+    // requested before the tables are built, so their round trips overlap.
+    const uint32_t nq = std::min<uint32_t>(kSubQueues, gridDim.x);
+    const uint32_t sq = blockIdx.x % nq;
+    const uint32_t sq_waves = (gridDim.x - sq + nq - 1) / nq * kTileWaves;
+    const uint32_t sq_rank = (blockIdx.x / nq) * kTileWaves + static_cast<uint32_t>(wave);
+    uint32_t *const sq_taken = QUEUE ? a.ws.tctr + kSubQueueBase + sq * kCtrStride : nullptr;
+    uint32_t *const sq_done = QUEUE ? sq_taken + kSubQueueCtr : nullptr;
+    uint32_t tk = 0;          // lane 0: the pending ticket
+    uint32_t cq[kLists];      // wave 0, lane r < kRegions: region r's count in each list
+#pragma unroll
+    for (uint32_t j = 0; j < kLists; ++j) cq[j] = 0;
+    if constexpr (QUEUE) {
+        if (wave == 0 && lane < static_cast<int>(kRegions)) {
+#pragma unroll
+            for (uint32_t j = 0; j < kLists; ++j) cq[j] = a.ws.tctr[(j * kRegions + lane) * kCtrStride];
+        }
+    }
     for (int i = tid; i < kCfdEntries; i += kTileBlock) s_cfd[i] = cfdg[i];
     if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
+    if (tid < kMitOneMinus) s_om[tid] = a.mitf[tid];
+    if (tid < 5 * 20)
+        s_t23[tid] = make_double2(a.mitf[kMitOneMinus + 2 * tid], a.mitf[kMitOneMinus + 2 * tid + 1]);
     if constexpr (RES) {
-        // MIT rank terms by CFD-table index 4p + x (the offsets of mit_rank).
-        for (int i = tid; i < 3 * 128; i += kTileBlock) {
-            const uint32_t k = static_cast<uint32_t>(i) / 128;
-            const uint32_t p = (static_cast<uint32_t>(i) % 128) >> 2;
-            uint32_t v = 0;
-            if (p < 20) {
-                if (k == 0) v = 20u + p * (p - 1) / 2;
-                else if (k == 1) v = 190u + p * (p - 1) * (p - 2) / 6;
-                else v = 1140u + p * (p - 1) * (p - 2) * (p - 3) / 24;
-            }
-            s_rank[k][i % 128] = v;
-        }
         if (tid < a.S) {
             const uint64_t f = a.meta[tid].fold;
             uint32_t j = 0;
@@ -726,8 +762,27 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
             for (; j < static_cast<uint32_t>(kResMaxCodes); ++j)
                 d_rabs[tid * kResMaxCodes + j] = static_cast<uint8_t>(kNoPos);
         }
-    } else {
-        build_rank_terms(s_rank, tid, kTileBlock);
+    }
+    if constexpr (QUEUE) {
+        // (the bounds only guard the lists against counters that were not zero)
+        if (wave == 0) {
+            const uint64_t rbq = region_guides(a.n, a.S);
+            uint32_t base = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kLists; ++j) {
+                const uint32_t c = static_cast<uint32_t>(std::min<uint64_t>(cq[j], j == 0 ? rbq * a.ws.tmax : rbq));
+                uint32_t x = c;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(x), lane >= off ? lane - off : lane));
+                    if (lane >= off) x += v;
+                }
+                if (lane < static_cast<int>(kRegions)) s_qpre[j][lane] = x - c;
+                if (lane == 0) s_qbase[j] = base;
+                base += static_cast<uint32_t>(__shfl(static_cast<int>(x), 63));
+            }
+            if (lane == 0) s_qbase[kLists] = base;
+        }
     }
     __syncthreads();
     build_posmask(s_posmask, s_fold, a.S, tid);
@@ -750,65 +805,36 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
     const uint32_t W = gridDim.x * kTileWaves;
     const uint32_t w = blockIdx.x * kTileWaves + wave;
 
-    // Queue schedule: a wave takes the next tile of its sub-queue when it
-    // starts one, so that the waves finish together whatever each tile
-    // costs.  Sub-queue j (of Q) is served by the workgroups b with
-    // b % Q == j and holds queue entries j, j + Q, j + 2Q, ...: its share of
-    // the first list's tiles first, then of the second's.  Slot
-    // schedule (guide streams wrote the plans): the slots of this wave are
-    // w + j*W; 64 of them are checked per load and the real ones (k < K_g)
-    // are taken in increasing order.
-    //
-    // The queue regions' counts, one per lane, and their exclusive prefixes.
-    uint32_t pre_a = 0, pre_b = 0, tot_a = 0, tot_b = 0;
-    uint64_t rb = 0;
-    if constexpr (QUEUE) {
-        rb = region_guides(n, a.S);
-        // (the bounds only guard the list against counters that were not zero)
-        const uint32_t ca = lane < static_cast<int>(kRegions)
-                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[lane * kCtrStride], rb * a.ws.tmax)) : 0;
-        const uint32_t cb = lane < static_cast<int>(kRegions)
-                                ? static_cast<uint32_t>(std::min<uint64_t>(a.ws.tctr[(kRegions + lane) * kCtrStride], rb)) : 0;
-        uint32_t ia = ca, ib = cb;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t va = static_cast<uint32_t>(__shfl(static_cast<int>(ia), lane >= off ? lane - off : lane));
-            const uint32_t vb = static_cast<uint32_t>(__shfl(static_cast<int>(ib), lane >= off ? lane - off : lane));
-            if (lane >= off) {
-                ia += va;
-                ib += vb;
-            }
-        }
-        pre_a = ia - ca;
-        pre_b = ib - cb;
-        tot_a = static_cast<uint32_t>(__shfl(static_cast<int>(ia), 63));
-        tot_b = static_cast<uint32_t>(__shfl(static_cast<int>(ib), 63));
-    }
-    // Queue entry c: the region holding it is the last one starting at or
-    // before c.
-    auto entry = [&](uint64_t c) -> uint32_t {
-        if (c < tot_a) {
-            const uint64_t m = __ballot(lane < static_cast<int>(kRegions) && pre_a <= c);
+    // Queue schedule: a wave asks for its next tile when it has streamed
+    // the current one, so that the waves finish together whatever each tile
+    // costs; the queue holds the tiles longest first (kLists size classes).
+    // The request (ticket, then the queue entry, then the plan) overlaps the
+    // tile's last candidates and its reduction.  Slot schedule (guide streams
+    // wrote the plans): the slots of this wave are w + j*W; 64 of them are
+    // checked per load and the real ones (k < K_g) are taken in increasing
+    // order.
+    const uint64_t rb = QUEUE ? region_guides(n, a.S) : 0;
+    // Queue entry c: its list is the last one starting at or before c, its
+    // region the last of that list starting at or before c.  Where its slot
+    // is stored, or null past the queue's end.
+    auto entry_at = [&](uint64_t c) -> const uint32_t * {
+        if constexpr (QUEUE) {
+            if (c >= s_qbase[kLists]) return nullptr;
+            uint32_t j = 0;
+            while (j + 1 < kLists && s_qbase[j + 1] <= c) ++j;
+            const uint32_t cj = static_cast<uint32_t>(c) - s_qbase[j];
+            const uint32_t pv = lane < static_cast<int>(kRegions) ? s_qpre[j][lane] : 0xFFFFFFFFu;
+            const uint64_t m = __ballot(pv <= cj);
             const int r = 63 - __builtin_clzll(m);
-            const uint32_t off = static_cast<uint32_t>(c) - static_cast<uint32_t>(__shfl(static_cast<int>(pre_a), r));
-            return a.ws.tlist[r * rb * a.ws.tmax + off];
+            const uint32_t off = cj - static_cast<uint32_t>(__shfl(static_cast<int>(pv), r));
+            if (j == 0) return a.ws.tlist + r * rb * a.ws.tmax + off;
+            return a.ws.tlist + kRegions * rb * a.ws.tmax + ((j - 1) * kRegions + r) * rb + off;
         }
-        c -= tot_a;
-        if (c < tot_b) {
-            const uint64_t m = __ballot(lane < static_cast<int>(kRegions) && pre_b <= c);
-            const int r = 63 - __builtin_clzll(m);
-            const uint32_t off = static_cast<uint32_t>(c) - static_cast<uint32_t>(__shfl(static_cast<int>(pre_b), r));
-            return a.ws.tlist[kRegions * rb * a.ws.tmax + r * rb + off];
-        }
-        return kNoSlot;
+        return nullptr;
     };
-    const uint32_t nq = std::min<uint32_t>(kSubQueues, gridDim.x);
-    const uint32_t sq = blockIdx.x % nq;
-    const uint32_t sq_waves = (gridDim.x - sq + nq - 1) / nq * kTileWaves;
-    uint32_t *const sq_taken = QUEUE ? a.ws.tctr + kSubQueueBase + sq * kCtrStride : nullptr;
-    uint32_t *const sq_done = QUEUE ? sq_taken + kSubQueueCtr : nullptr;
-    // A wave leaving the kernel: the last one of its sub-queue's waves sets
-    // the sub-queue's counters back to zero for the next dispatch.
+    // A wave leaving the kernel (with no ticket pending): the last one of its
+    // sub-queue's waves sets the sub-queue's counters back to zero for the
+    // next dispatch.
     auto leave = [&]() {
         if constexpr (QUEUE) {
             if (lane == 0 && atomicAdd(sq_done, 1u) == sq_waves - 1) {
@@ -819,14 +845,8 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
     };
     uint32_t j0 = 0, vl = 0;
     uint64_t smask = 0;
+    // Slot schedule: the next slot of this wave.
     auto next_slot = [&]() -> uint32_t {
-        if constexpr (QUEUE) {
-            uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(sq_taken, 1u);
-            t = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(t)));
-            const uint32_t v = entry(sq + static_cast<uint64_t>(t) * nq);
-            return v < V ? v : kNoSlot;   // a slot outside the slot space ends the wave
-        }
         while (smask == 0) {
             if (static_cast<uint64_t>(j0) * W + w >= V) return kNoSlot;
             const uint64_t vv = static_cast<uint64_t>(j0 + lane) * W + w;
@@ -883,16 +903,46 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
         __builtin_amdgcn_wave_barrier();
     };
 
-    uint32_t vc = next_slot();
+    // The next tile's request: stage 0, the ticket tk is in flight; 1, the
+    // queue entry tv is; 2, the next slot vn is known and its plan mn is in
+    // flight (or vn is kNoSlot, or no request was made).
+    Plan mc, mn;
+    uint32_t vc = kNoSlot, vn = kNoSlot, tv = kNoSlot;
+    int stage = 2;
+    auto request_step = [&]() {
+        if constexpr (QUEUE) {
+            if (stage == 0) {
+                const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(tk)));
+                const uint32_t *const e = entry_at(sq + static_cast<uint64_t>(sq_waves + t) * nq);
+                tv = e ? *e : kNoSlot;
+                stage = 1;
+            } else if (stage == 1) {
+                // (the entry is the same in every lane: keep it scalar)
+                const uint32_t v = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(tv)));
+                vn = v < V ? v : kNoSlot;   // a slot outside the slot space ends the wave
+                if (vn != kNoSlot) plan_load(vn, mn);
+                stage = 2;
+            }
+        }
+    };
+    if constexpr (QUEUE) {
+        // The first tile is queue entry sq_rank.
+        const uint32_t *const e = entry_at(sq + static_cast<uint64_t>(sq_rank) * nq);
+        const uint32_t v = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(e ? *e : kNoSlot)));
+        vc = v < V ? v : kNoSlot;
+    } else {
+        vc = next_slot();
+    }
     if (vc == kNoSlot) {  // uniform per wave; no workgroup barrier follows
         leave();
         return;
     }
-    Plan mc, mn;
     plan_load(vc, mc);
     plan_commit(mc, 0);
-    uint32_t vn = next_slot();
-    if (vn != kNoSlot) plan_load(vn, mn);
+    if constexpr (!QUEUE) {
+        vn = next_slot();
+        if (vn != kNoSlot) plan_load(vn, mn);
+    }
     int buf = 0;
 
     for (;;) {
@@ -981,14 +1031,23 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
                 asm("" : "+v"(occ));
                 const Real ro = to_real<Real>(occ);
                 Real tm = 0, tc = 0, mv = 0;
-                // The MIT load goes first so the CFD work covers its latency;
-                // an exact match (dist 0, all positions kNoPos) reads a
-                // valid entry that is then not used.
                 if (a.calc_mit) {
-                    if constexpr (RES)
-                        mv = mitg[(m2[0] >> 2) + s_rank[0][m2[1]] + s_rank[1][m2[2]] + s_rank[2][m2[3]]];
-                    else
-                        mv = mitg[(m2[0] >> 1) + s_rank[0][m2[1]] + s_rank[1][m2[2]] + s_rank[2][m2[3]]];
+                    // The MIT value from the closed form's factors, in its
+                    // order: ((t1 * t2) * t3) * 100 with t1 the product of
+                    // 1 - weight over the mismatch positions (absent ones,
+                    // position 31 or 32, give 1.0), t2 and t3 by the count
+                    // and the span of the positions
+                    // (ISSLCreateIndex.cpp:235-261); the same bits as the
+                    // table the reference loads.  (An exact match reads the
+                    // k = 0 entry, which is not used.)
+                    constexpr int SH = RES ? 2 : 1;
+                    const uint32_t p0 = m2[0] >> SH, p1 = m2[1] >> SH, p2 = m2[2] >> SH, p3 = m2[3] >> SH;
+                    double t1 = s_om[p0] * s_om[p1];
+                    t1 = t1 * s_om[p2];
+                    t1 = t1 * s_om[p3];
+                    const uint32_t last = dist >= 4 ? p3 : dist == 3 ? p2 : dist == 2 ? p1 : p0;
+                    const double2 f = s_t23[dist * 20 + static_cast<int>(last - p0)];
+                    mv = static_cast<Real>(((t1 * f.x) * f.y) * 100.0);
                 }
                 if (a.calc_cfd) {
                     // PAM factor, then one factor per mismatch in position
@@ -1251,8 +1310,14 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
             round(base, std::true_type{});
         }
         if (base < a1 && !(ORD && stop)) round(base, std::false_type{});
+        if constexpr (QUEUE) {
+            // the ticket for the next tile
+            if (lane == 0) tk = atomicAdd(sq_taken, 1u);
+            stage = 0;
+        }
         while (tail != head) drain(std::min<uint32_t>(64, tail - head));
         if (htail != hhead) score_hits(htail - hhead);
+        request_step();   // the ticket's answer, and the queue entry's request
         // Lane partials as doubles (a compensated pair converts exactly),
         // combined in a fixed butterfly: every lane ends with the same sums.
         double dm, dc;
@@ -1276,6 +1341,7 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
             dm = run_m;
             dc = run_c;
         }
+        request_step();   // the entry, and the plan's request
         if (lane == 0) {
             TilePart tp;
             tp.m = dm;
@@ -1283,9 +1349,6 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
             tp.occ = occ_sum;
             tp.sites = sites;
             a.ws.part[vc] = tp;
-            // A guide with a single tile is complete: its sum is this partial
-            // (what the finalizer would form: 0 + partial, exactly).
-            if (mc.K == 1 && a.finish_single) write_guide(a.sc, a.out, mc.g, dm, dc, occ_sum, sites);
         }
         if (vn == kNoSlot) {
             leave();
@@ -1295,8 +1358,12 @@ __global__ __launch_bounds__(kTileBlock, (ORD && DEPTH == 8) ? 4 : 1) void k_til
         plan_commit(mn, buf);
         vc = vn;
         mc = mn;
-        vn = next_slot();
-        if (vn != kNoSlot) plan_load(vn, mn);
+        if constexpr (QUEUE) {
+            vn = kNoSlot;
+        } else {
+            vn = next_slot();
+            if (vn != kNoSlot) plan_load(vn, mn);
+        }
     }
 }
 
@@ -1315,7 +1382,6 @@ struct FinArgs {
     DevResults out;
     uint64_t *distinct;           // mark sites up to the exit (early exit armed)
     int skip_untiled;             // guides without tiles were finished by k_stream
-    int finish_single;            // guides with one tile were finished by k_tiles
     int skip_multi;               // guides with several tiles are split guides of the
                                   // in-order form, finished by k_ordered_terms
     const GuideState *state_in;   // carried state (null: start from zero)
@@ -1326,7 +1392,8 @@ struct FinArgs {
 // The tile queue's counters back to zero for the next launch sequence (the
 // tile kernel has finished with them).
 __device__ __forceinline__ void reset_queue(const BatchWs &ws) {
-    if (ws.tctr && blockIdx.x == 0 && threadIdx.x < 2 * kRegions) ws.tctr[threadIdx.x * kCtrStride] = 0;
+    if (ws.tctr && blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < kLists * kRegions; i += blockDim.x) ws.tctr[i * kCtrStride] = 0;
     if (ws.tctr && blockIdx.x == 0 && threadIdx.x == 0) {
         ws.tctr[kTermCtr] = 0;
         ws.tctr[kTermCtr + 1] = 0;
@@ -1358,7 +1425,6 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     const int S = a.S;
     const uint32_t K = a.ws.gtiles[g];
     if (K == 0 && a.skip_untiled) return;
-    if (K == 1 && a.finish_single) return;  // finished by the wave that scored its tile
     // Lane l holds the partial of tile k0 + l, 64 tiles at a time.
     TilePart mine{};
     uint32_t k0 = 0;
@@ -1516,7 +1582,6 @@ __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
     if (g >= a.n) return;
     const uint32_t K = a.ws.gtiles[g];
     if (K == 0 && a.skip_untiled) return;
-    if (K == 1 && a.finish_single) return;  // finished by the wave that scored its tile
     if (K > 1 && a.skip_multi) return;      // finished by k_ordered_terms
     double dm = 0, dc = 0;
     uint64_t occ_sum = 0, sites = 0;
@@ -1952,7 +2017,7 @@ struct WsSizes {
         delta = align256(n * std::max(S, 1) * 8);
         u32 = align256(n * 4);
         part = align256(n * tmax * sizeof(TilePart));
-        list = align256(kRegions * region_guides(n, S) * (tmax + 1) * 4);
+        list = align256(kRegions * region_guides(n, S) * (tmax + kLists - 1) * 4);
         ctr = align256(kQueueCtrs * 4);
         term_mc = align256(term_cap * sizeof(double2));
         term_oi = align256(term_cap * sizeof(uint2));
@@ -2029,48 +2094,77 @@ size_t tiles_lds_bytes(int S, bool res) {
     return (b + 15) / 16 * 16;
 }
 
+// Workgroups of a tile kernel that one CU holds at once, for its block size
+// and dynamic LDS (cached per kernel and LDS size).
+int resident_blocks(const void *fn, size_t lds) {
+    struct Entry {
+        const void *fn;
+        size_t lds;
+        int blocks;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Entry &e : cache)
+        if (e.fn == fn && e.lds == lds) return e.blocks;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kTileBlock, lds) != hipSuccess || nb <= 0)
+        nb = 1;
+    cache.push_back(Entry{fn, lds, nb});
+    return nb;
+}
+
+// A persistent tile kernel's launch: at most as many workgroups as the CUs
+// hold at once (a wave's first tile is fixed by its rank, so every wave must
+// be able to start while the others run).
+template <typename Kernel>
+void launch_resident(Kernel kern, const TileArgs &ta, int grid, int cus, size_t lds, hipStream_t st,
+                     hipEvent_t e0, hipEvent_t e1) {
+    const int cap = cus * resident_blocks(reinterpret_cast<const void *>(kern), lds);
+    hipExtLaunchKernelGGL(kern, dim3(std::max(1, std::min(grid, cap))), dim3(kTileBlock), lds, st, e0,
+                          e1, 0, ta);
+}
+
 // e0/e1: timing events attached to the dispatch (null when timing is off).
 // The RES kernels need the tile queue and the planner's guide residuals.
 template <int PREC>
-int launch_tiles_depth(const TileArgs &ta, int depth, int grid, hipStream_t st, hipEvent_t e0,
-                       hipEvent_t e1) {
-    const dim3 g(grid), b(kTileBlock);
+int launch_tiles_depth(const TileArgs &ta, int depth, int grid, int cus, hipStream_t st,
+                       hipEvent_t e0, hipEvent_t e1) {
     if (depth != 4 && depth != 8 && depth != 16)
         return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
     const size_t lds_res = tiles_lds_bytes(ta.S, true), lds_inl = tiles_lds_bytes(ta.S, false);
     if (ta.res && ta.ws.tlist && ta.ws.gres) {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
-        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
-        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
+        if (depth == 4) launch_resident(k_tiles<PREC, 4, true, true>, ta, grid, cus, lds_res, st, e0, e1);
+        if (depth == 8) launch_resident(k_tiles<PREC, 8, true, true>, ta, grid, cus, lds_res, st, e0, e1);
+        if (depth == 16) launch_resident(k_tiles<PREC, 16, true, true>, ta, grid, cus, lds_res, st, e0, e1);
     } else if (ta.ws.tlist) {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, true, false>), g, b, lds_inl, st, e0, e1, 0, ta);
-        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, true, false>), g, b, lds_inl, st, e0, e1, 0, ta);
-        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, true, false>), g, b, lds_inl, st, e0, e1, 0, ta);
+        if (depth == 4) launch_resident(k_tiles<PREC, 4, true, false>, ta, grid, cus, lds_inl, st, e0, e1);
+        if (depth == 8) launch_resident(k_tiles<PREC, 8, true, false>, ta, grid, cus, lds_inl, st, e0, e1);
+        if (depth == 16) launch_resident(k_tiles<PREC, 16, true, false>, ta, grid, cus, lds_inl, st, e0, e1);
     } else {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<PREC, 4, false, false>), g, b, lds_inl, st, e0, e1, 0, ta);
-        if (depth == 8) hipExtLaunchKernelGGL((k_tiles<PREC, 8, false, false>), g, b, lds_inl, st, e0, e1, 0, ta);
-        if (depth == 16) hipExtLaunchKernelGGL((k_tiles<PREC, 16, false, false>), g, b, lds_inl, st, e0, e1, 0, ta);
+        if (depth == 4) launch_resident(k_tiles<PREC, 4, false, false>, ta, grid, cus, lds_inl, st, e0, e1);
+        if (depth == 8) launch_resident(k_tiles<PREC, 8, false, false>, ta, grid, cus, lds_inl, st, e0, e1);
+        if (depth == 16) launch_resident(k_tiles<PREC, 16, false, false>, ta, grid, cus, lds_inl, st, e0, e1);
     }
     return ISSL_OK;
 }
 
 // The in-order tile kernel (FP64, queued), either site-word layout.
-int launch_ordered_depth(const TileArgs &ta, int depth, int grid, hipStream_t st, hipEvent_t e0,
-                         hipEvent_t e1) {
-    const dim3 g(grid), b(kTileBlock);
+int launch_ordered_depth(const TileArgs &ta, int depth, int grid, int cus, hipStream_t st,
+                         hipEvent_t e0, hipEvent_t e1) {
     constexpr int P = ISSL_PREC_FP64;
     if (!ta.ws.tlist) return fail(ISSL_ERR_STATE, "in-order tiles need the tile queue");
+    if (depth != 4 && depth != 8 && depth != 16)
+        return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
     const size_t lds_res = tiles_lds_bytes(ta.S, true), lds_inl = tiles_lds_bytes(ta.S, false);
     if (ta.res && ta.ws.gres) {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<P, 4, true, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
-        else if (depth == 8) hipExtLaunchKernelGGL((k_tiles<P, 8, true, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
-        else if (depth == 16) hipExtLaunchKernelGGL((k_tiles<P, 16, true, true, true>), g, b, lds_res, st, e0, e1, 0, ta);
-        else return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
+        if (depth == 4) launch_resident(k_tiles<P, 4, true, true, true>, ta, grid, cus, lds_res, st, e0, e1);
+        if (depth == 8) launch_resident(k_tiles<P, 8, true, true, true>, ta, grid, cus, lds_res, st, e0, e1);
+        if (depth == 16) launch_resident(k_tiles<P, 16, true, true, true>, ta, grid, cus, lds_res, st, e0, e1);
     } else {
-        if (depth == 4) hipExtLaunchKernelGGL((k_tiles<P, 4, true, false, true>), g, b, lds_inl, st, e0, e1, 0, ta);
-        else if (depth == 8) hipExtLaunchKernelGGL((k_tiles<P, 8, true, false, true>), g, b, lds_inl, st, e0, e1, 0, ta);
-        else if (depth == 16) hipExtLaunchKernelGGL((k_tiles<P, 16, true, false, true>), g, b, lds_inl, st, e0, e1, 0, ta);
-        else return fail(ISSL_ERR_ARG, "tuning depth must be 4, 8 or 16");
+        if (depth == 4) launch_resident(k_tiles<P, 4, true, false, true>, ta, grid, cus, lds_inl, st, e0, e1);
+        if (depth == 8) launch_resident(k_tiles<P, 8, true, false, true>, ta, grid, cus, lds_inl, st, e0, e1);
+        if (depth == 16) launch_resident(k_tiles<P, 16, true, false, true>, ta, grid, cus, lds_inl, st, e0, e1);
     }
     return ISSL_OK;
 }
@@ -2139,10 +2233,6 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     const bool fp32 = precision == ISSL_PREC_FP32C;
     const void *mit = fp32 ? static_cast<const void *>(ix->d_mit32) : static_cast<const void *>(ix->d_mit64);
     const void *cfd = fp32 ? static_cast<const void *>(ix->d_cfd32) : static_cast<const void *>(ix->d_cfd64);
-    // A guide's single tile finishes it when no exit is armed (its sum is the
-    // tile's partial) and always in the in-order form (the tile kernel takes
-    // the exit itself).
-    const int finish_single = (!carry && (!sc.use_cut || ordered)) ? 1 : 0;
     const bool tiles = !direct || tn.heavy_kpost != kHeavyNever;
     const uint32_t gw = static_cast<uint32_t>((n + 3) / 4);
 
@@ -2199,7 +2289,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     ta.res_shift = win.res_shift;
     ta.post = win.post;
     ta.meta = ix->d_meta;
-    ta.mit = mit;
+    ta.mitf = ix->d_mitf;
     ta.cfd = cfd;
     ta.n = static_cast<uint32_t>(n);
     ta.S = S;
@@ -2209,16 +2299,15 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     ta.calc_cfd = sc.calc_cfd;
     ta.distinct = distinct;
     ta.sc = sc;
-    ta.out = out;
-    ta.finish_single = finish_single;
     ta.ws = b;
     // Persistent grid: grid_per_cu workgroups per CU, no more waves than slots.
     uint64_t grid = static_cast<uint64_t>(cu_count(ix->device)) * tn.grid_per_cu;
     grid = std::min<uint64_t>(grid, (n * b.tmax + kTileWaves - 1) / kTileWaves);
     auto tiles_launch = [&](hipEvent_t e0, hipEvent_t e1) {
-        if (ordered) return launch_ordered_depth(ta, tn.depth, static_cast<int>(grid), st, e0, e1);
-        return fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), st, e0, e1)
-                    : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), st, e0, e1);
+        const int cus = cu_count(ix->device);
+        if (ordered) return launch_ordered_depth(ta, tn.depth, static_cast<int>(grid), cus, st, e0, e1);
+        return fp32 ? launch_tiles_depth<ISSL_PREC_FP32C>(ta, tn.depth, static_cast<int>(grid), cus, st, e0, e1)
+                    : launch_tiles_depth<ISSL_PREC_FP64>(ta, tn.depth, static_cast<int>(grid), cus, st, e0, e1);
     };
     if (direct) {
         // Long guides go to the tile kernel: one event pair spans both kernels.
@@ -2244,7 +2333,6 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     fa.out = out;
     fa.distinct = (distinct && sc.use_cut) ? distinct : nullptr;
     fa.skip_untiled = direct ? 1 : 0;
-    fa.finish_single = finish_single;
     fa.skip_multi = split ? 1 : 0;
     fa.state_in = win.state_in;
     fa.state_out = win.state_out;
@@ -2256,8 +2344,9 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
         const uint64_t waves = std::min<uint64_t>(n, static_cast<uint64_t>(cu_count(ix->device)) * 16);
         hipLaunchKernelGGL(k_ordered_terms, dim3(static_cast<uint32_t>((waves + 3) / 4)), dim3(256), 0, st, fo);
     }
-    // (in order: the tile kernel finished every guide with postings; this
-    // writes those without any and sets the queue counters back to zero)
+    // (in order: the tile kernel took the exit; the finalizer writes each
+    // guide from its tile's partial, or from nothing, and sets the queue
+    // counters back to zero)
     if (!sc.use_cut || ordered)
         hipLaunchKernelGGL(k_finalize_sum, dim3(static_cast<uint32_t>((n + 63) / 64)), dim3(64), 0, st, fa);
     else if (fp32)

@@ -24,6 +24,11 @@ inline uint64_t fold_of(uint64_t x) { return ((x >> 1) & kFoldLo) | (x & kFoldLo
 constexpr int kMitEntries = 6195;
 // First combinadic rank of each mismatch count k (index k = 1..4).
 constexpr int kMitBase[5] = {0, 0, 20, 210, 1350};
+// MIT factor table (mit_factor_table): kMitOneMinus values 1 - weight[p] by
+// position (1.0 from position 20 on), then for k = 0..4 and span = 0..19 the
+// pair (t2, t3) of the closed form at [kMitOneMinus + 2 (20 k + span)].
+constexpr int kMitOneMinus = 33;
+constexpr int kMitFactorEntries = kMitOneMinus + 2 * 5 * 20;
 
 // Slice widths are limited to 16 positions: the reference builds keys in a
 // uint32 (ISSLCreateIndex.cpp:458).
@@ -47,6 +52,10 @@ const char *last_error_cstr();
 // matching fold masks; filled from the closed form on first use.
 const double *mit_by_rank();
 const uint64_t *mit_keys_by_rank();
+// The MIT closed form as factors (kMitFactorEntries values), and whether
+// they reproduce every value of mit_by_rank() bit for bit.
+const double *mit_factor_table();
+bool mit_factors_exact();
 // CFD tables: 320 position penalties, 16 PAM penalties.
 const double *cfd_pos_table();
 const double *cfd_pam_table();

@@ -521,7 +521,7 @@ void free_index(issl_index *ix) {
     if (ix->stream) (void)hipStreamSynchronize(ix->stream);
     stream_free(ix);
     void *ptrs[] = {ix->d_sig, ix->d_post, ix->d_inline, ix->d_res, ix->d_lstart, ix->d_meta,
-                    ix->d_mit64, ix->d_mit32, ix->d_cfd64, ix->d_cfd32, ix->d_distinct};
+                    ix->d_mit64, ix->d_mit32, ix->d_mitf, ix->d_cfd64, ix->d_cfd32, ix->d_distinct};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ix->ws.guides.release();
@@ -712,6 +712,11 @@ int alloc_index(issl_index *ix, uint64_t lstart_count, bool postings) {
     if ((rc = dev_alloc(ix, &ix->d_meta, std::max<uint64_t>(n_slices, 1))) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix, &ix->d_mit64, kMitEntries)) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix, &ix->d_mit32, kMitEntries)) != ISSL_OK) return rc;
+    // The tile kernels form MIT values from these factors: they must give the
+    // table's values bit for bit (checked once on the host).
+    if (!mit_factors_exact())
+        return fail(ISSL_ERR_STATE, "MIT factor table does not reproduce the MIT closed form");
+    if ((rc = dev_alloc(ix, &ix->d_mitf, kMitFactorEntries)) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix, &ix->d_cfd64, kCfdTable)) != ISSL_OK) return rc;
     if ((rc = dev_alloc(ix, &ix->d_cfd32, kCfdTable)) != ISSL_OK) return rc;
     ix->distinct_words = n_sites / 64 + 1;
@@ -727,6 +732,7 @@ int alloc_index(issl_index *ix, uint64_t lstart_count, bool postings) {
     for (int i = 0; i < kCfdTable; ++i) c32[i] = static_cast<float>(c64[i]);
     ISSL_HIP(hipMemcpy(ix->d_mit64, mit_by_rank(), kMitEntries * 8, hipMemcpyHostToDevice));
     ISSL_HIP(hipMemcpy(ix->d_mit32, m32.data(), kMitEntries * 4, hipMemcpyHostToDevice));
+    ISSL_HIP(hipMemcpy(ix->d_mitf, mit_factor_table(), kMitFactorEntries * 8, hipMemcpyHostToDevice));
     ISSL_HIP(hipMemcpy(ix->d_cfd64, c64.data(), kCfdTable * 8, hipMemcpyHostToDevice));
     ISSL_HIP(hipMemcpy(ix->d_cfd32, c32.data(), kCfdTable * 4, hipMemcpyHostToDevice));
     ISSL_HIP(hipMemset(ix->d_distinct, 0, ix->distinct_words * 8));

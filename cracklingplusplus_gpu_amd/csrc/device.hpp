@@ -204,6 +204,7 @@ struct issl_index {
     issl::SliceMeta *d_meta = nullptr;
     double *d_mit64 = nullptr;      // [6195] by combinadic rank
     float *d_mit32 = nullptr;
+    double *d_mitf = nullptr;       // [kMitFactorEntries] the closed form's factors
     double *d_cfd64 = nullptr;      // [320 pos | 16 pam | 1.0]
     float *d_cfd32 = nullptr;
     uint64_t *d_distinct = nullptr; // [n/64 + 1]

@@ -78,6 +78,52 @@ const MitTable &mit_table() {
     return t;
 }
 
+// The closed form as factors the kernels multiply: one minus the position
+// weight (1.0 past position 19, so absent mismatches multiply exactly), and
+// per mismatch count k and span (last minus first position) the second and
+// third terms.  mit_closed_form's t1 starts at 1.0 and multiplies the
+// factors in position order, so om[p0]*om[p1]*om[p2]*om[p3] is the same
+// rounded value; t2 depends on the positions only through the span (the sum
+// of the gaps is the span, exactly), t3 only on k.  mit_factors_exact()
+// checks that ((t1 * t2) * t3) * 100 from these factors equals every table
+// value bit for bit.
+struct MitFactors {
+    double f[kMitFactorEntries];
+    bool exact = true;
+    MitFactors() {
+        for (int p = 0; p < kMitOneMinus; ++p) f[p] = p < 20 ? 1.0 - kMitWeight[p] : 1.0;
+        double *t23 = f + kMitOneMinus;
+        for (int k = 0; k <= 4; ++k)
+            for (int span = 0; span < 20; ++span) {
+                double t2 = 0.0, t3 = 0.0;
+                if (k >= 1) {
+                    const double gap = k == 1 ? 19.0 : static_cast<double>(span) / static_cast<double>(k - 1);
+                    t2 = 1.0 / ((19.0 - gap) / 19.0 * 4.0 + 1);
+                    t3 = 1.0 / static_cast<double>(static_cast<uint64_t>(k) * k);
+                }
+                t23[2 * (k * 20 + span)] = t2;
+                t23[2 * (k * 20 + span) + 1] = t3;
+            }
+        const MitTable &t = mit_table();
+        for (int r = 0; r < kMitEntries; ++r) {
+            int pos[4] = {32, 32, 32, 32}, k = 0;
+            for (int j = 0; j < 20; ++j)
+                if ((t.key[r] >> (2 * j)) & 1ULL) pos[k++] = j;
+            double t1 = f[pos[0]] * f[pos[1]];
+            t1 = t1 * f[pos[2]];
+            t1 = t1 * f[pos[3]];
+            const int span = pos[k - 1] - pos[0];
+            const double v = ((t1 * t23[2 * (k * 20 + span)]) * t23[2 * (k * 20 + span) + 1]) * 100.0;
+            if (std::memcmp(&v, &t.val[r], sizeof v) != 0) exact = false;
+        }
+    }
+};
+
+const MitFactors &mit_factors() {
+    static const MitFactors t;
+    return t;
+}
+
 // One row per position; four groups of four: guide nucleotide, then the
 // reverse complement of the site nucleotide (index (pos<<4)|(g<<2)|rc(o)).
 const double kCfdPos[320] = {
@@ -114,6 +160,8 @@ const double kCfdPam[16] = {
 }  // namespace
 
 const double *mit_by_rank() { return mit_table().val; }
+const double *mit_factor_table() { return mit_factors().f; }
+bool mit_factors_exact() { return mit_factors().exact; }
 const uint64_t *mit_keys_by_rank() { return mit_table().key; }
 const double *cfd_pos_table() { return kCfdPos; }
 const double *cfd_pam_table() { return kCfdPam; }

@@ -49,6 +49,34 @@ def test_product_tables_equal_oracle_and_reference_fixture():
     assert (tab[:, 0] == k).all() and (tab[:, 1] == v.view(np.uint64)).all()
 
 
+def test_mit_factored_form_equals_reference_table():
+    """The tile kernels form each MIT value as ((t1 * t2) * t3) * 100 with t1 the
+    product of (1 - weight) over the mismatch positions (1.0 for absent ones),
+    t2 looked up by (count, last - first position) and t3 by count.  Computed
+    here with IEEE doubles in that order, it must equal every entry of the
+    reference fixture's table (ISSLCreateIndex.cpp:235-261 wrote it)."""
+    w = [0.0, 0.0, 0.014, 0.0, 0.0, 0.395, 0.317, 0.0, 0.389, 0.079, 0.445, 0.508,
+         0.613, 0.851, 0.732, 0.828, 0.615, 0.804, 0.685, 0.583]
+    om = [1.0 - x for x in w] + [1.0] * 13
+    raw = np.fromfile(os.path.join(GOLDEN, "ref_test_genome_offTargets_indexed.issl"),
+                      dtype=np.uint64)
+    tab = raw[6:6 + 2 * 6195].reshape(-1, 2)
+    for key, bits in tab.tolist():
+        pos = [j for j in range(20) if (key >> (2 * j)) & 1]
+        k = len(pos)
+        assert 1 <= k <= 4
+        span = pos[-1] - pos[0]
+        gap = 19.0 if k == 1 else float(span) / float(k - 1)
+        t2 = 1.0 / ((19.0 - gap) / 19.0 * 4.0 + 1)
+        t3 = 1.0 / float(k * k)
+        p = pos + [32] * (4 - k)
+        t1 = om[p[0]] * om[p[1]]
+        t1 = t1 * om[p[2]]
+        t1 = t1 * om[p[3]]
+        v = ((t1 * t2) * t3) * 100.0
+        assert np.float64(v).view(np.uint64) == bits, (pos, v)
+
+
 def test_builder_reproduces_reference_fixture_index():
     """Sorted site text -> (sig, occ) -> slice lists equals the reference's test index."""
     legacy = golden_index("legacy39_5x4")
