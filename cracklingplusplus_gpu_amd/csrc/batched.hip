@@ -419,7 +419,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         e = a.lstart[lb + key + 1];
     }
     // (formed while the list bounds are in flight)
-    const uint32_t gr = a.ws.gres && sl_in ? residual_code(q, fold) | (static_cast<uint32_t>(s) << 24) : 0u;
+    const uint32_t gr = a.ws.gres && sl_in ? residual_code_kept(q, a.meta[s].keep) | (static_cast<uint32_t>(s) << 24) : 0u;
     const bool live = in && (!a.state_in || (a.state_in[g].sites & kExited) == 0);
     const uint32_t len = (sl_in && live && s >= a.s0 && s < a.s1) ? e - b : 0u;
     // Inclusive scan of the lengths over the guide's lanes (lane - off is the
@@ -437,11 +437,13 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         a.ws.delta[g * S + s] = len ? static_cast<uint64_t>(s - a.s0) * a.n_sites + b - run : 0;
         if (a.ws.gres) a.ws.gres[g * S + s] = gr;
     }
+    // (P < 2^31: the launch refuses indexes whose guides could visit more,
+    // so the tile arithmetic holds in 32 bits)
     const uint32_t tmax = a.ws.tmax;
-    uint64_t c = (static_cast<uint64_t>(P) + tmax - 1) / tmax;
-    c = (c + 63) / 64 * 64;
+    uint32_t c = (P + tmax - 1) / tmax;
+    c = (c + 63u) & ~63u;
     if (c < a.chunk) c = a.chunk;
-    uint32_t K = static_cast<uint32_t>((P + c - 1) / c);
+    uint32_t K = (P + c - 1) / c;
     const bool lead = in && s == 0;
     if (a.split_at) {
         // In-order split: P term entries per long guide, one 64-bit atomic per
@@ -465,10 +467,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         __syncthreads();
         uint64_t start = s_tbase + incl - want;
         for (int w = 0; w < wave; ++w) start += s_want[w];
-        uint64_t cs = (static_cast<uint64_t>(P) + kSplitTiles - 1) / kSplitTiles;
-        cs = (cs + 63) / 64 * 64;
+        uint32_t cs = (P + kSplitTiles - 1) / kSplitTiles;
+        cs = (cs + 63u) & ~63u;
         if (cs < a.split_chunk) cs = a.split_chunk;
-        const uint32_t Ks = static_cast<uint32_t>((P + cs - 1) / cs);
+        const uint32_t Ks = (P + cs - 1) / cs;
         if (want && Ks > 1 && start + want <= a.ws.term_cap) {
             c = cs;
             K = Ks;
@@ -527,12 +529,19 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(PlanArgs a) {
         const uint32_t n32 = static_cast<uint32_t>(a.n);
         // A region holds all its blocks' tiles; the bounds only guard the
         // lists against counters that were not zero.
-        if (ka) {
+        {
+            // list 0: each guide's tiles written by the whole wave, guide by guide
             uint32_t base = s_base[0] + ia - ka;
             for (int w = 0; w < wave; ++w) base += s_tot[w][0];
             uint32_t *const la = a.ws.tlist + r * rg * tmax;
-            for (uint32_t k = 0; k < ka; ++k)
-                if (base + k < rg * tmax) la[base + k] = k * n32 + static_cast<uint32_t>(g);
+            for (uint64_t m = __ballot(ka != 0); m; m &= m - 1) {
+                const int l = __builtin_ctzll(m);
+                const uint32_t kl = static_cast<uint32_t>(__shfl(static_cast<int>(ka), l));
+                const uint32_t bl = static_cast<uint32_t>(__shfl(static_cast<int>(base), l));
+                const uint32_t gl = static_cast<uint32_t>(__shfl(static_cast<int>(g), l));
+                for (uint32_t k = static_cast<uint32_t>(lane); k < kl; k += 64)
+                    if (bl + k < rg * tmax) la[bl + k] = k * n32 + gl;
+            }
         }
         if (single) {
             uint32_t base = s_base[cls] + rank;
@@ -1576,42 +1585,97 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
 // k_finalize without an early exit armed: no rescan can happen, so one lane
 // per guide adds the tile partials in tile order, the same additions the
 // wave form makes.
+constexpr uint32_t kFinSerialTiles = 8;   // more tiles: the guide's wave adds them
+
 __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
     reset_queue(a.ws);
     const uint64_t g = blockIdx.x * 64ULL + threadIdx.x;
-    if (g >= a.n) return;
-    const uint32_t K = a.ws.gtiles[g];
-    if (K == 0 && a.skip_untiled) return;
-    if (K > 1 && a.skip_multi) return;      // finished by k_ordered_terms
-    double dm = 0, dc = 0;
-    uint64_t occ_sum = 0, sites = 0;
-    bool exited = false;
-    if (a.state_in) {
-        const GuideState st = a.state_in[g];
-        dm = st.m;
-        dc = st.c;
-        occ_sum = st.occ;
-        sites = st.sites & ~kExited;
-        exited = (st.sites & kExited) != 0;
+    const int lane = static_cast<int>(threadIdx.x);
+    uint32_t K = 0;
+    bool mine = false;
+    if (g < a.n) {
+        K = a.ws.gtiles[g];
+        mine = !(K == 0 && a.skip_untiled) && !(K > 1 && a.skip_multi);   // (split: k_ordered_terms)
     }
-#pragma unroll 4
-    for (uint32_t k = 0; k < K; ++k) {
-        const TilePart p = a.ws.part[static_cast<uint64_t>(k) * a.n + g];
-        dm += p.m;
-        dc += p.c;
-        occ_sum += p.occ;
-        sites += p.sites;
+    // Sums of guide gg over its K tiles, in tile order after its carried state.
+    auto finish = [&](uint64_t gg, double dm, double dc, uint64_t occ_sum, uint64_t sites, bool exited) {
+        if (a.state_out) {
+            GuideState st;
+            st.m = dm;
+            st.c = dc;
+            st.occ = occ_sum;
+            st.sites = sites | (exited ? kExited : 0);
+            a.state_out[gg] = st;
+            return;
+        }
+        write_guide(a.sc, a.out, gg, dm, dc, occ_sum, sites);
+    };
+    auto start = [&](uint64_t gg, double &dm, double &dc, uint64_t &occ_sum, uint64_t &sites, bool &exited) {
+        dm = 0;
+        dc = 0;
+        occ_sum = 0;
+        sites = 0;
+        exited = false;
+        if (a.state_in) {
+            const GuideState st = a.state_in[gg];
+            dm = st.m;
+            dc = st.c;
+            occ_sum = st.occ;
+            sites = st.sites & ~kExited;
+            exited = (st.sites & kExited) != 0;
+        }
+    };
+    // One lane per guide with few tiles.
+    if (mine && K <= kFinSerialTiles) {
+        double dm, dc;
+        uint64_t occ_sum, sites;
+        bool exited;
+        start(g, dm, dc, occ_sum, sites, exited);
+        for (uint32_t k = 0; k < K; ++k) {
+            const TilePart p = a.ws.part[static_cast<uint64_t>(k) * a.n + g];
+            dm += p.m;
+            dc += p.c;
+            occ_sum += p.occ;
+            sites += p.sites;
+        }
+        finish(g, dm, dc, occ_sum, sites, exited);
     }
-    if (a.state_out) {
-        GuideState st;
-        st.m = dm;
-        st.c = dc;
-        st.occ = occ_sum;
-        st.sites = sites | (exited ? kExited : 0);
-        a.state_out[g] = st;
-        return;
+    // Guides with many tiles: the wave loads 64 partials at a time into LDS,
+    // lane 0 adds the MIT sums and lane 1 the CFD sums one by one, in tile
+    // order (the same additions, add_chain), and the counts are summed over
+    // the lanes (integers: any order).
+    __shared__ double2 s_p[64];
+    for (uint64_t m = __ballot(mine && K > kFinSerialTiles); m; m &= m - 1) {
+        const int l = __builtin_ctzll(m);
+        const uint64_t gg = blockIdx.x * 64ULL + static_cast<uint64_t>(l);
+        const uint32_t KK = static_cast<uint32_t>(__shfl(static_cast<int>(K), l));
+        double dm, dc;
+        uint64_t occ_sum, sites;
+        bool exited;
+        start(gg, dm, dc, occ_sum, sites, exited);
+        double ch = lane == 1 ? dc : dm;
+        uint64_t po = 0, ps = 0;
+        for (uint32_t k0 = 0; k0 < KK; k0 += 64) {
+            TilePart p{0.0, 0.0, 0, 0};
+            if (k0 + static_cast<uint32_t>(lane) < KK)
+                p = a.ws.part[static_cast<uint64_t>(k0 + static_cast<uint32_t>(lane)) * a.n + gg];
+            s_p[lane] = make_double2(p.m, p.c);
+            po += p.occ;
+            ps += p.sites;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < 2) add_chain<8>(reinterpret_cast<const double *>(s_p) + lane, std::min<uint32_t>(64, KK - k0), ch);
+            __builtin_amdgcn_wave_barrier();
+        }
+        dm = readlane_f64(ch, 0);
+        dc = readlane_f64(ch, 1);
+        for (int off = 32; off >= 1; off >>= 1) {
+            po += shfl64(po, lane ^ off);
+            ps += shfl64(ps, lane ^ off);
+        }
+        occ_sum += po;
+        sites += ps;
+        if (lane == 0) finish(gg, dm, dc, occ_sum, sites, exited);
     }
-    write_guide(a.sc, a.out, g, dm, dc, occ_sum, sites);
 }
 
 // In-order split: one wave per split guide adds its hits' terms in the order

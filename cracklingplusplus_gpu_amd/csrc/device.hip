@@ -681,6 +681,8 @@ int init_meta(issl_index *ix, uint64_t n_sites, uint64_t seq_len, uint64_t n_sli
             m.pos[j] = pos[j];
             m.fold |= 1ULL << (2 * pos[j]);
         }
+        for (int p = 0; p < kResPositions; ++p)
+            if (!((m.fold >> (2 * p)) & 1ULL)) m.keep |= 1u << p;
         const uint64_t nk = 1ULL << (2 * w);
         if (base + nk + 1 > 0xFFFFFFFFULL)
             return fail(ISSL_ERR_FORMAT, "too many lists for 32-bit list offsets");

@@ -30,6 +30,7 @@ struct SliceMeta {
     uint64_t fold;
     uint32_t lbase;
     uint32_t width;
+    uint32_t keep;   // bit p set for each position p < 20 the slice does not fix
     uint8_t pos[16];
 };
 
@@ -82,6 +83,17 @@ __host__ __device__ __forceinline__ uint32_t residual_code(uint64_t sig, uint64_
         const uint32_t keep = static_cast<uint32_t>(~(fold >> (2 * p))) & 1u;
         r |= (static_cast<uint32_t>(sig >> (2 * p)) & 3u & (0u - keep)) << (2 * j);
         j += keep;
+    }
+    return r;
+}
+
+// residual_code from the slice's kept positions (SliceMeta::keep): one step
+// per kept position instead of one per position; the same value.
+__device__ __forceinline__ uint32_t residual_code_kept(uint64_t sig, uint32_t keep) {
+    uint32_t r = 0, j = 0;
+    for (uint32_t m = keep; m; m &= m - 1, j += 2) {
+        const uint32_t p = static_cast<uint32_t>(__builtin_ctz(m));
+        r |= (static_cast<uint32_t>(sig >> (2 * p)) & 3u) << j;
     }
     return r;
 }
