@@ -341,11 +341,10 @@ __device__ __forceinline__ double xor_lane_f64(double v) {
 // One butterfly step of the tile reduction: every lane adds its partner's
 // partial (a + b and b + a agree, so all lanes hold the same sums after).
 template <int M>
-__device__ __forceinline__ void tile_reduce_step(double &m, double &c, uint64_t &occ, uint32_t &sites) {
+__device__ __forceinline__ void tile_reduce_step(double &m, double &c, uint64_t &occ) {
     m += xor_lane_f64<M>(m);
     c += xor_lane_f64<M>(c);
     occ += xor_lane64<M>(occ);
-    sites += xor_lane<M>(sites);
 }
 
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
@@ -803,6 +802,9 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
         }
         __syncthreads();
     }
+    // CFD factors of absent mismatches (positions 20..kNoPos): 1.0 in every
+    // tile (the tiles rewrite positions 0..19 only).
+    for (int i = 20 * 4 + lane; i < static_cast<int>(kNoPos + 1) * 4; i += 64) s_cq[wave][i] = Real(1);
     // RES: the code field's fold bits (bit 2j per residual position j) and the
     // occurrence escape.
     const uint32_t res_fold = RES ? (0x55555555u & ((1u << a.res_shift) - 1u)) : 0u;
@@ -959,22 +961,19 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
         const uint32_t a0 = mc.k * mc.C;
         const uint32_t a1 = std::min(mc.P, a0 + mc.C);
         // Entry 4p + x: the factor of a mismatch at position p where guide ^
-        // site = x (lib/ISSLScoringModule.cpp:308-361); 1 at kNoPos.
-        for (int i = lane; i < static_cast<int>(kNoPos + 1) * 4; i += 64) {
+        // site = x (lib/ISSLScoringModule.cpp:308-361), for p < 20 (the
+        // entries of positions 20..kNoPos hold 1.0 from the kernel's start).
+        for (int i = lane; i < 20 * 4; i += 64) {
             const uint32_t pp = static_cast<uint32_t>(i) >> 2, x = static_cast<uint32_t>(i) & 3u;
-            Real v = Real(1);
-            if (pp < 20) {
-                const uint32_t g = static_cast<uint32_t>(q >> (2 * pp)) & 3u;
-                v = s_cfd[(pp << 4) | (g << 2) | ((g ^ x) ^ 3u)];
-            }
-            s_cq[wave][i] = v;
+            const uint32_t g = static_cast<uint32_t>(q >> (2 * pp)) & 3u;
+            s_cq[wave][i] = s_cfd[(pp << 4) | (g << 2) | ((g ^ x) ^ 3u)];
         }
         __builtin_amdgcn_wave_barrier();
 
         Real am = 0, ac = 0;
         F2 fm{0.f, 0.f}, fc{0.f, 0.f};
         uint64_t occ_sum = 0;
-        uint32_t sites = 0;   // hits of this tile in this lane (< 2^26)
+        uint32_t sites = 0;   // hits of this tile (wave-uniform; < 2^26)
         // ORD: the guide's running sums (wave-uniform) and whether the exit
         // has been taken (then nothing more of the guide is scored).  A tile
         // of a split guide (K > 1) writes its hits' terms instead, and its
@@ -998,6 +997,8 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
                     hhead += cnt;
                     return;
                 }
+            } else {
+                sites += cnt;
             }
             uint32_t h_occ = 0, h_id = 0;
             Real h_tm = 0, h_tc = 0;
@@ -1030,10 +1031,7 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
                         atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (h_id >> 6),
                                  1ULL << (h_id & 63));
                 }
-                if constexpr (!ORD) {
-                    occ_sum += occ;
-                    sites += 1;
-                }
+                if constexpr (!ORD) occ_sum += occ;
                 h_occ = occ;
                 // Keep occ a 32-bit value: otherwise the compiler widens it
                 // back to the 64-bit word it was cut from and converts that.
@@ -1094,8 +1092,8 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
                     a.ws.term_mc[i] = make_double2(static_cast<double>(h_tm), static_cast<double>(h_tc));
                     a.ws.term_oi[i] = make_uint2(h_occ, h_id);
                     occ_sum += h_occ;
-                    sites += 1;
                 }
+                sites += cnt;
                 hcnt += cnt;
             } else if constexpr (ORD) {
                 // The hits in order, one at a time: hit i's terms sit at
@@ -1142,9 +1140,9 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
                 acc = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(acc)));
                 stop = __builtin_amdgcn_readfirstlane(static_cast<int>(ex)) != 0;
                 // Counts and distinct sites of the hits up to the exit.
+                sites += acc;
                 if (static_cast<uint32_t>(lane) < acc) {
                     occ_sum += h_occ;
-                    sites += 1;
                     if (a.mark_distinct)
                         atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) + (h_id >> 6),
                                  1ULL << (h_id & 63));
@@ -1340,12 +1338,12 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
             dm = am;
             dc = ac;
         }
-        tile_reduce_step<32>(dm, dc, occ_sum, sites);
-        tile_reduce_step<16>(dm, dc, occ_sum, sites);
-        tile_reduce_step<8>(dm, dc, occ_sum, sites);
-        tile_reduce_step<4>(dm, dc, occ_sum, sites);
-        tile_reduce_step<2>(dm, dc, occ_sum, sites);
-        tile_reduce_step<1>(dm, dc, occ_sum, sites);
+        tile_reduce_step<32>(dm, dc, occ_sum);
+        tile_reduce_step<16>(dm, dc, occ_sum);
+        tile_reduce_step<8>(dm, dc, occ_sum);
+        tile_reduce_step<4>(dm, dc, occ_sum);
+        tile_reduce_step<2>(dm, dc, occ_sum);
+        tile_reduce_step<1>(dm, dc, occ_sum);
         if constexpr (ORD) {
             dm = run_m;
             dc = run_c;
