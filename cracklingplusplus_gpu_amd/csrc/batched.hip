@@ -709,13 +709,15 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
     //                              kResMaxCodes, as the layout needs every
     //                              slice to have kResMinWidth or more positions)
     extern __shared__ __align__(16) char s_dyn[];
-    // Candidate ring: tile-relative position | slice << kRelBits, and the site word.
-    __shared__ uint32_t s_rpos[kTileWaves][kRing];
-    __shared__ Word s_rword[kTileWaves][kRing];
-    // Hit ring: first-seen candidates with packed mismatch positions and distance.
-    __shared__ uint32_t s_hpos[kTileWaves][kHitRing];
-    __shared__ Word s_hword[kTileWaves][kHitRing];
-    __shared__ uint32_t s_hmis[kTileWaves][kHitRing];
+    // Candidate ring: tile-relative position | slice << kRelBits, and the site
+    // word (RES: both in one 8-byte entry, written and read at once).
+    __shared__ uint32_t s_rpos[kTileWaves][RES ? 1 : kRing];
+    __shared__ Word s_rword[kTileWaves][RES ? 1 : kRing];
+    __shared__ uint2 s_rring[kTileWaves][RES ? kRing : 1];
+    // Hit ring: first-seen candidates, one 16-byte entry each: the candidate
+    // ring's position entry, the packed mismatch positions and distance, and
+    // the word (RES: the clipped occurrence count).
+    __shared__ uint4 s_hring[kTileWaves][kHitRing];
     // ORD: a batch's hit terms (MIT, CFD) in hit order, for the in-order adds.
     __shared__ double2 s_ord[ORD ? kTileWaves : 1][ORD ? 64 : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1004,9 +1006,12 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
             Real h_tm = 0, h_tc = 0;
             if (static_cast<uint32_t>(lane) < cnt) {
                 const uint32_t e = (hhead + lane) & (kHitRing - 1);
-                const uint32_t ent = s_hpos[wave][e];
-                const Word wv = s_hword[wave][e];
-                const uint32_t pk = s_hmis[wave][e];
+                const uint4 he = s_hring[wave][e];
+                const uint32_t ent = he.x;
+                const uint32_t pk = he.y;
+                Word wv;
+                if constexpr (RES) wv = he.z;
+                else wv = (static_cast<uint64_t>(he.w) << 32) | he.z;
                 // 2p (or, RES, 4p + x) of each mismatch
                 const uint32_t m2[4] = {pk & 127u, (pk >> 7) & 127u, (pk >> 14) & 127u,
                                         (pk >> 21) & 127u};
@@ -1164,26 +1169,48 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
             Word wv = 0;
             if (static_cast<uint32_t>(lane) < cnt) {
                 const uint32_t e = (head + lane) & (kRing - 1);
-                ent = s_rpos[wave][e];
-                wv = s_rword[wave][e];
+                if constexpr (RES) {
+                    const uint2 r = s_rring[wave][e];
+                    ent = r.x;
+                    wv = r.y;
+                } else {
+                    ent = s_rpos[wave][e];
+                    wv = s_rword[wave][e];
+                }
                 const int sl = static_cast<int>(ent >> kRelBits);
                 if constexpr (RES) {
                     // Residual fold: bit 2j for each differing residual
                     // position j; positions ascend with j.  The ring holds
                     // x = word ^ (guide codes | slice << 24).
+                    // The four lowest set bits are found first and both
+                    // tables read for all four at once (an absent one reads
+                    // entry kResMaxCodes - 1, via the odd bit 2 kResMaxCodes - 1,
+                    // which no fold sets), so the lookups share one LDS
+                    // round trip.
                     const uint32_t x = wv;
-                    uint32_t f = ((x >> 1) | x) & res_fold;
-                    const uint32_t dist = static_cast<uint32_t>(__popc(f));
+                    uint32_t fi[4];
+                    fi[0] = ((x >> 1) | x) & res_fold;
+#pragma unroll
+                    for (int i = 1; i < 4; ++i) fi[i] = fi[i - 1] & (fi[i - 1] - 1u);
+                    const uint32_t dist = static_cast<uint32_t>(__popc(fi[0]));
+                    uint32_t b2[4], p[4];
+                    uint64_t pm[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        b2[i] = static_cast<uint32_t>(__builtin_ctz(fi[i] | (1u << (2 * kResMaxCodes - 1))));
+                    const uint8_t *const rab = d_rabs + sl * kResMaxCodes;
+                    const uint64_t *const rpm = d_rpm + sl * kResMaxCodes;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        p[i] = rab[b2[i] >> 1];
+                        pm[i] = rpm[b2[i] >> 1];
+                    }
                     uint32_t ix[4];
                     uint64_t nm = 0;   // slices holding a mismatch position
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const uint32_t b2 = f ? static_cast<uint32_t>(__builtin_ctz(f)) : 0u;
-                        const uint32_t p = d_rabs[sl * kResMaxCodes + (b2 >> 1)];
-                        const uint64_t pm = d_rpm[sl * kResMaxCodes + (b2 >> 1)];
-                        ix[i] = f ? ((p << 2) | ((x >> b2) & 3u)) : kAbsent7;
-                        nm |= f ? pm : 0ULL;
-                        f &= f - 1;
+                        ix[i] = fi[i] ? ((p[i] << 2) | ((x >> b2[i]) & 3u)) : kAbsent7;
+                        nm |= fi[i] ? pm[i] : 0ULL;
                     }
                     // the first-seen rule (first_seen) on those slices
                     const uint64_t earlier = sl ? (~0ULL >> (64 - sl)) : 0ULL;
@@ -1205,14 +1232,14 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
             head += cnt;
             const uint64_t b = __ballot(hit);
             if (b) {
-                const uint32_t r = __builtin_amdgcn_mbcnt_hi(
-                    static_cast<uint32_t>(b >> 32),
-                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
                 if (hit) {
-                    const uint32_t e = (htail + r) & (kHitRing - 1);
-                    s_hpos[wave][e] = ent;
-                    s_hword[wave][e] = wv;
-                    s_hmis[wave][e] = pk;
+                    const uint32_t e = __builtin_amdgcn_mbcnt_hi(
+                        static_cast<uint32_t>(b >> 32),
+                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), htail)) & (kHitRing - 1);
+                    if constexpr (RES) s_hring[wave][e] = make_uint4(ent, pk, wv, 0u);
+                    else
+                        s_hring[wave][e] = make_uint4(ent, pk, static_cast<uint32_t>(wv),
+                                                      static_cast<uint32_t>(wv >> 32));
                 }
                 htail += static_cast<uint32_t>(__popcll(b));
             }
@@ -1296,14 +1323,18 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
                 }
                 const uint64_t b = __ballot(cand);
                 if (b) {
-                    const uint32_t r = __builtin_amdgcn_mbcnt_hi(
-                        static_cast<uint32_t>(b >> 32),
-                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
                     if (cand) {
-                        const uint32_t e = (tail + r) & (kRing - 1);
+                        const uint32_t e = __builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(b >> 32),
+                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), tail)) & (kRing - 1);
                         const uint32_t slice = RES ? (sl[d] >> 24) : sl[d];
-                        s_rpos[wave][e] = (base - a0 + 64 * d + lane) | (slice << kRelBits);
-                        s_rword[wave][e] = wv[d];
+                        const uint32_t rp = (base - a0 + 64 * d + lane) | (slice << kRelBits);
+                        if constexpr (RES) {
+                            s_rring[wave][e] = make_uint2(rp, wv[d]);
+                        } else {
+                            s_rpos[wave][e] = rp;
+                            s_rword[wave][e] = wv[d];
+                        }
                     }
                     tail += static_cast<uint32_t>(__popcll(b));
                 }
