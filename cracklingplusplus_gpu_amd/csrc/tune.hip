@@ -37,7 +37,7 @@ std::string shape_key(const issl_index *ix, uint64_t n_sample, int precision) {
     for (char &c : arch)
         if (c == ' ' || c == ':') c = '_';
     std::ostringstream k;
-    k << "t7/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
+    k << "t8/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
     for (const SliceMeta &m : ix->meta_h) k << m.width << ',';
     int q = 0;
     while ((n_sample >> q) > 1) ++q;
@@ -226,9 +226,12 @@ extern "C" int issl_autotune_precision(issl_index *ix, const uint64_t *sample, u
     const int chunks[] = {1024, 2048, 4096};
     const int depths[] = {4, 8, 16};
     const int grids[] = {4, 6, 8};
-    // The list-residual words are kept only when they beat the signature-inline
-    // words (which the other kernels read too) by more than kResMargin, above
-    // the trial-to-trial spread of one to two percent.
+    // Where the index holds the list-residual words they are used unless the
+    // signature-inline words beat them by more than kResMargin, above the
+    // trial-to-trial spread of one to two percent: they move half the bytes,
+    // which pays wherever the streams are long, and where a launch is bound by
+    // latency and issue instead the two layouts time within that spread, so a
+    // pick without the margin would follow the noise.
     constexpr double kResMargin = 0.97;
     issl_tuning layout_best[2];
     layout_best[0].us_per_guide = layout_best[1].us_per_guide = 1e300;
@@ -242,7 +245,8 @@ extern "C" int issl_autotune_precision(issl_index *ix, const uint64_t *sample, u
                     if (t.us_per_guide < layout_best[layout].us_per_guide) layout_best[layout] = t;
                 }
     }
-    ix->res_use = layout_best[1].us_per_guide < kResMargin * layout_best[0].us_per_guide;
+    ix->res_use = ix->d_res != nullptr &&
+                  !(layout_best[0].us_per_guide < kResMargin * layout_best[1].us_per_guide);
     const issl_tuning tiles_best = layout_best[ix->res_use ? 1 : 0];
     if (tiles_best.us_per_guide < top.us_per_guide) top = tiles_best;
     // (2) one wave per guide, optionally sending long guides to the tile kernel
