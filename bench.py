@@ -375,24 +375,38 @@ class Rotation:
     def __len__(self):
         return len(self.batches)
 
-    def run(self, ix, steps: int, start: int, **kw) -> None:
+    def run(self, ix, steps: int, start: int, streams=None, **kw) -> None:
+        """Steps start .. start+steps-1; step k on streams[k mod len] (None:
+        the index's own stream)."""
         for k in range(steps):
             b = (start + k) % len(self)
-            ix.score_device(self.d_g[b], self.Q, self.d_out[b], **kw)
+            s = streams[k % len(streams)] if streams else None
+            ix.score_device(self.d_g[b], self.Q, self.d_out[b], stream=s, **kw)
 
     def results(self):
         return [{k: v.to_host() for k, v in o.items()} for o in self.d_out]
 
 
-def timed(hiprt, rot, ix, steps, warmup, **kw):
+def timed(hiprt, rot, ix, steps, warmup, streams=None, **kw):
     """Warm-up steps, then `steps` timed steps between two device
-    synchronizations; ms per step."""
-    rot.run(ix, max(1, warmup), 0, **kw)
+    synchronizations (hipDeviceSynchronize: every stream); ms per step."""
+    rot.run(ix, max(1, warmup), 0, streams, **kw)
     hiprt.synchronize()
     t0 = time.perf_counter()
-    rot.run(ix, steps, 0, **kw)
+    rot.run(ix, steps, 0, streams, **kw)
     hiprt.synchronize()
     return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def pick_streams(hiprt, ix, g_tune, pool, **kw):
+    """Streams the steps run on: consecutive batches on 1, 2 or 3 streams of
+    the index (its own, then bound ones), each count timed on the held-out
+    batch; the fastest is kept.  Returns (count, {count: us per step})."""
+    rot = Rotation(hiprt, [g_tune])
+    sweep = {}
+    for ns in range(1, len(pool) + 1):
+        sweep[ns] = round(timed(hiprt, rot, ix, 30, 6, pool[:ns], **kw) * 1e3, 2)
+    return min(sweep, key=sweep.get), sweep
 
 
 def kernel_us(hiprt, rot, ix, steps, **kw):
@@ -597,7 +611,10 @@ def main() -> int:
     ap.add_argument("--plain-requests", type=int, default=100,
                     help="closed-loop PLAIN requests in the serving workload's baseline")
     ap.add_argument("--graph", action="store_true",
-                    help="replay the step's kernels from a captured hipGraph")
+                    help="replay the step's kernels from a captured hipGraph (one stream)")
+    ap.add_argument("--streams", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="streams the rotated steps go round (0: the count, 1-3, that is "
+                         "fastest on the held-out batch)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="minimum CPU scoring time (the batch is repeated)")
     ap.add_argument("--stream-slots", type=int, default=4,
@@ -690,11 +707,30 @@ def main() -> int:
     # this three-kernel sequence, so off by default).
     kw_step = dict(kw, flags=issl.FLAG_GRAPH if args.graph else 0)
 
+    # ---- batches in flight: consecutive steps go round the index's own stream
+    # and up to two streams bound to the index (issl_index_bind_stream), each
+    # with its own scoring workspace, so that one step's planner, finalizer
+    # and last waves overlap the next step's tile kernel.  The count (1-3) is
+    # picked on the held-out batch unless --streams gives it.
+    pool = [None] + [hiprt.Stream() for _ in range(2)]
+    for s_ in pool[1:]:
+        ix.bind_stream(s_)
+    if args.streams or args.graph:
+        n_streams, sweep = (1 if args.graph else args.streams), None
+    else:
+        n_streams, sweep = pick_streams(hiprt, ix, g_tune, pool, **kw_step)
+    streams = pool[:n_streams]
+    log(f"streams: {n_streams}" + (f" (held-out batch, us per step by count: {sweep})" if sweep else ""))
+
     # ---- timed region: BATCHED FP64, inputs resident in HBM, rotating batches
     warm = max(args.warmup, R if args.graph else 0)
-    ms_per_step = timed(hiprt, rot, ix, args.steps, warm, **kw_step)
+    ms_per_step = timed(hiprt, rot, ix, args.steps, warm, streams, **kw_step)
     value = Q / (ms_per_step * 1e-3)
-    log(f"batched fp64: {ms_per_step:.4f} ms/step, {value:,.0f} guides/s")
+    log(f"batched fp64: {ms_per_step:.4f} ms/step, {value:,.0f} guides/s ({n_streams} streams)")
+    # the same steps one at a time on the index's stream
+    ms_serial = (timed(hiprt, rot, ix, args.steps, warm, None, **kw_step) if n_streams > 1
+                 else ms_per_step)
+    log(f"batched fp64, one stream: {ms_serial:.4f} ms/step, {Q / (ms_serial * 1e-3):,.0f} guides/s")
 
     # ---- dominant kernel time, HIP events on each of K rotated steps' dispatches
     k_us, n_disp = kernel_us(hiprt, rot, ix, args.steps, **kw)
@@ -733,6 +769,13 @@ def main() -> int:
                    "launch": "hipGraph replay" if args.graph else "eager",
                    "postings_per_step": int(np.mean(p_step)),
                    "site_words": "list-residual, 4 B" if res_used else "signature-inline, 8 B",
+                   "streams": n_streams,
+                   "streams_note": "consecutive steps on the index's stream and bound streams "
+                                   "(issl_index_bind_stream), each with its own workspace; the "
+                                   "count picked on the held-out batch",
+                   "stream_pick_us_per_step": sweep,
+                   "one_stream": {"value": round(Q / (ms_serial * 1e-3), 1),
+                                  "ms_per_step": round(ms_serial, 4)},
                    "tuning": {"chunk": tn.chunk, "depth": tn.depth,
                               "grid_per_cu": tn.grid_per_cu,
                               "stream_depth": tn.direct_depth,
@@ -758,7 +801,7 @@ def main() -> int:
 
     # ---- compensated FP32 (the north star's precision form), same steps and shape
     kwc = dict(kw, precision=issl.PREC_FP32C)
-    ms_c = timed(hiprt, rot, ix, args.steps, args.warmup, **kwc)
+    ms_c = timed(hiprt, rot, ix, args.steps, args.warmup, streams, **kwc)
     kc_us, _ = kernel_us(hiprt, rot, ix, args.steps, **kwc)
     res_fp32c = rot.results()
     log(f"batched fp32c: {ms_c:.4f} ms/step, kernel {kc_us:.2f} us")
@@ -767,7 +810,7 @@ def main() -> int:
     other_leg = None
     if res_built:
         ix.set_residual(not res_used)
-        ms_o = timed(hiprt, rot, ix, args.steps, args.warmup, **kw)
+        ms_o = timed(hiprt, rot, ix, args.steps, args.warmup, streams, **kw)
         ko_us, _ = kernel_us(hiprt, rot, ix, args.steps, **kw)
         same_o = all(np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
                      for a, b in zip(rot.results(), res_fp64) for k in a)
@@ -793,11 +836,18 @@ def main() -> int:
 
     # ---- ORDERED form (bit-exact FP64 in the tile kernel, one wave per guide)
     kwo = dict(kw, form=issl.FORM_ORDERED)
-    ord_ms = timed(hiprt, rot, ix, args.steps, args.warmup, **kwo)
+    if args.streams:
+        ord_ns, ord_sweep = args.streams, None
+    else:
+        ord_ns, ord_sweep = pick_streams(hiprt, ix, g_tune, pool, **kwo)
+    ord_ms = timed(hiprt, rot, ix, args.steps, args.warmup, pool[:ord_ns], **kwo)
+    ord_ms1 = (timed(hiprt, rot, ix, args.steps, args.warmup, None, **kwo) if ord_ns > 1
+               else ord_ms)
     ord_k_us, _ = kernel_us(hiprt, rot, ix, args.steps, **kwo)
     ord_same = all(np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
                    for a, b in zip(rot.results()[:covered], res_plain) for k in a)
-    log(f"ordered: {ord_ms:.4f} ms/step, kernel {ord_k_us:.1f} us, bits of plain {ord_same}")
+    log(f"ordered: {ord_ms:.4f} ms/step ({ord_ns} streams; one stream {ord_ms1:.4f}), "
+        f"kernel {ord_k_us:.1f} us, bits of plain {ord_same}")
 
     # ---- threshold 75 (the module's usual setting): ORDERED and PLAIN take the
     # reference's early exit, hit by hit
@@ -927,6 +977,9 @@ def main() -> int:
                     "achieved_GBps": round(bytes_ref / (ord_k_us * 1e-6) / 1e9, 1),
                     "same_bits_as_plain": bool(ord_same), "batches_compared": covered,
                     "plain_speedup": round(plain_ms / ord_ms, 3),
+                    "streams": ord_ns, "stream_pick_us_per_step": ord_sweep,
+                    "one_stream": {"value": round(Q / (ord_ms1 * 1e-3), 1),
+                                   "ms_per_step": round(ord_ms1, 4)},
                     "mode": "ORDERED: FP64 hits added in the reference's order, one wave "
                             "per guide in the persistent tile kernel"},
         "threshold_75": {"ordered": {"value": round(Q / (ord75_ms * 1e-3), 1),

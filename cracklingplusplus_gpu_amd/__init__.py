@@ -414,7 +414,18 @@ class Index:
                                  dp(d_out.get("pass")), dp(d_out.get("mit_sum")),
                                  dp(d_out.get("cfd_sum")))
         p = self.params(method, threshold, form, precision, flags, chunk)
-        check(lib().issl_score_device(self._h, d_guides.ptr, n, byref(p), byref(res), stream))
+        check(lib().issl_score_device(self._h, d_guides.ptr, n, byref(p), byref(res),
+                                      getattr(stream, "handle", stream)))
+
+    def bind_stream(self, stream) -> None:
+        """Give `stream` (a hiprt.Stream or a raw hipStream_t) a workspace of
+        its own on this index: score_device calls on it then overlap with
+        calls on other streams (issl_index_bind_stream)."""
+        check(lib().issl_index_bind_stream(self._h, getattr(stream, "handle", stream)))
+
+    def unbind_stream(self, stream) -> None:
+        """Wait for the stream's calls on this index and free its workspace."""
+        check(lib().issl_index_unbind_stream(self._h, getattr(stream, "handle", stream)))
 
     def set_tuning(self, chunk: int = 1024, depth: int = 8, grid_per_cu: int = 6,
                    direct_depth: int = 8, heavy_kpost: int = 0, ord_depth: Optional[int] = None,

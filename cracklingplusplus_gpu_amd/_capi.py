@@ -105,6 +105,8 @@ SIGNATURES = [
     ("issl_index_get_info", c_int, [c_void_p, POINTER(issl_index_info)]),
     ("issl_index_validate", c_int, [c_void_p, POINTER(c_uint64)]),
     ("issl_index_set_residual", c_int, [c_void_p, c_int32]),
+    ("issl_index_bind_stream", c_int, [c_void_p, c_void_p]),
+    ("issl_index_unbind_stream", c_int, [c_void_p, c_void_p]),
     ("issl_index_get_residual", c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32),
                                         POINTER(c_uint32)]),
     ("issl_distinct_count", c_int, [c_void_p, POINTER(c_uint64)]),

@@ -519,6 +519,10 @@ void free_index(issl_index *ix) {
     if (!ix) return;
     (void)hipSetDevice(ix->device);
     if (ix->stream) (void)hipStreamSynchronize(ix->stream);
+    // Bound streams may still run calls that use their workspaces (and the
+    // caller may have destroyed them): wait for the device.
+    if (!ix->bound.empty()) (void)hipDeviceSynchronize();
+    ix->bound.clear();
     stream_free(ix);
     void *ptrs[] = {ix->d_sig, ix->d_post, ix->d_inline, ix->d_res, ix->d_lstart, ix->d_meta,
                     ix->d_mit64, ix->d_mit32, ix->d_mitf, ix->d_cfd64, ix->d_cfd32, ix->d_distinct};
@@ -1117,10 +1121,27 @@ int ws_release(issl_index *ix, hipStream_t st) {
 namespace {
 
 int score_on_device(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_params *p,
-                    const ScoreScalars &sc, const DevResults &o, hipStream_t st) {
+                    const ScoreScalars &sc, const DevResults &o, hipStream_t st,
+                    Workspace *ws = nullptr) {
     uint64_t *distinct = (p->flags & ISSL_FLAG_DISTINCT) ? ix->d_distinct : nullptr;
     if (ix->src) return score_streamed(ix, d_guides, n, p, sc, o, distinct, st);
-    return score_with(ix, d_guides, n, p, sc, ix->tuning, o, ix->ws, st);
+    return score_with(ix, d_guides, n, p, sc, ix->tuning, o, ws ? *ws : ix->ws, st);
+}
+
+// The workspace of a bound stream, or null.
+Workspace *bound_ws(issl_index *ix, hipStream_t st) {
+    for (auto &b : ix->bound)
+        if (b->stream == st) return &b->ws;
+    return nullptr;
+}
+
+// Whether a captured sequence's workspace is still allocated (the index's
+// own workspace or a bound stream's, at its current address).
+bool ws_live(const issl_index *ix, const void *tiles) {
+    if (tiles == ix->ws.tiles.p) return true;
+    for (const auto &b : ix->bound)
+        if (tiles == b->ws.tiles.p) return true;
+    return false;
 }
 
 }  // namespace
@@ -1212,37 +1233,43 @@ extern "C" int issl_score_device(issl_index *ix, const uint64_t *guides_device, 
     if ((params->flags & ISSL_FLAG_GRAPH) && ix->src)
         return fail(ISSL_ERR_ARG, "graph replay needs a resident index (a streamed index copies "
                                   "postings from the host during the call)");
-    if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
+    // A bound stream scores in its own workspace; any other stream shares the
+    // index's, after the work of the stream that used it last.
+    Workspace *const bws = bound_ws(ix, st);
+    Workspace &W = bws ? *bws : ix->ws;
+    if (!bws && (rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
+    auto release = [&]() -> int { return bws ? ISSL_OK : ws_release(ix, st); };
     const bool use_graph = (params->flags & ISSL_FLAG_GRAPH) && !ix->timing;
     if (use_graph) {
         // Replay a captured sequence made for exactly this call.  Entries made
-        // for another workspace address hold freed pointers: drop them.
-        const void *wsp = ix->ws.tiles.p;
+        // for a workspace address no longer allocated hold freed pointers:
+        // drop them.
+        const void *wsp = W.tiles.p;
         for (size_t i = 0; i < ix->graphs.size();) {
             GraphEntry &gr = ix->graphs[i];
-            if (gr.ws != wsp) {
+            if (!ws_live(ix, gr.ws)) {
                 (void)hipGraphExecDestroy(gr.exec);
                 ix->graphs.erase(ix->graphs.begin() + static_cast<long>(i));
                 continue;
             }
-            if (gr.guides == guides_device && gr.n == n && gr.stream == st &&
+            if (gr.ws == wsp && gr.guides == guides_device && gr.n == n && gr.stream == st &&
                 std::memcmp(&gr.params, params, sizeof(issl_params)) == 0 &&
                 std::memcmp(&gr.out, out_device, sizeof(issl_results)) == 0 &&
                 std::memcmp(&gr.tuning, &ix->tuning, sizeof(issl_tuning)) == 0) {
                 ISSL_HIP(hipGraphLaunch(gr.exec, st));
-                if ((rc = ws_release(ix, st)) != ISSL_OK) return rc;
+                if ((rc = release()) != ISSL_OK) return rc;
                 return ok();
             }
             ++i;
         }
     }
-    rc = score_on_device(ix, guides_device, n, params, sc, o, st);
+    rc = score_on_device(ix, guides_device, n, params, sc, o, st, &W);
     if (rc != ISSL_OK) return rc;
     if (use_graph) {
         // The workspace is in place now: capture the same sequence for replay.
         hipGraph_t graph = nullptr;
         ISSL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = score_on_device(ix, guides_device, n, params, sc, o, st);
+        rc = score_on_device(ix, guides_device, n, params, sc, o, st, &W);
         const hipError_t e = hipStreamEndCapture(st, &graph);
         if (rc != ISSL_OK || e != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -1259,10 +1286,52 @@ extern "C" int issl_score_device(issl_index *ix, const uint64_t *guides_device, 
             ix->graphs.erase(ix->graphs.begin());
         }
         ix->graphs.push_back(GraphEntry{guides_device, n, *params, *out_device, st, ix->tuning,
-                                        ix->ws.tiles.p, exec});
+                                        W.tiles.p, exec});
     }
-    if ((rc = ws_release(ix, st)) != ISSL_OK) return rc;
+    if ((rc = release()) != ISSL_OK) return rc;
     return ok();
+}
+
+extern "C" int issl_index_bind_stream(issl_index *ix, void *stream) {
+    if (!ix || !stream) return fail(ISSL_ERR_ARG, "null argument");
+    if (ix->src)
+        return fail(ISSL_ERR_ARG, "a streamed index stages its postings through one set of "
+                                  "buffers: its calls cannot overlap");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (st == ix->stream) return fail(ISSL_ERR_ARG, "the index's own stream uses the index's workspace");
+    if (bound_ws(ix, st)) return ok();
+    if (ix->bound.size() >= static_cast<size_t>(kMaxBoundStreams))
+        return fail(ISSL_ERR_CAPACITY, "at most " + std::to_string(kMaxBoundStreams) +
+                                           " streams bound to one index");
+    auto b = std::make_unique<issl_index::BoundWs>();
+    b->stream = st;
+    ix->bound.push_back(std::move(b));
+    return ok();
+}
+
+extern "C" int issl_index_unbind_stream(issl_index *ix, void *stream) {
+    if (!ix || !stream) return fail(ISSL_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    DeviceGuard guard(ix->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    for (size_t i = 0; i < ix->bound.size(); ++i) {
+        if (ix->bound[i]->stream != st) continue;
+        // The stream's queued calls use the workspace until they finish.
+        ISSL_HIP(hipStreamSynchronize(st));
+        const void *wsp = ix->bound[i]->ws.tiles.p;
+        for (size_t j = 0; j < ix->graphs.size();) {
+            if (ix->graphs[j].ws == wsp && wsp) {
+                (void)hipGraphExecDestroy(ix->graphs[j].exec);
+                ix->graphs.erase(ix->graphs.begin() + static_cast<long>(j));
+                continue;
+            }
+            ++j;
+        }
+        ix->bound.erase(ix->bound.begin() + static_cast<long>(i));
+        return ok();
+    }
+    return fail(ISSL_ERR_ARG, "stream not bound to this index");
 }
 
 extern "C" int issl_candidates(issl_index *ix, const uint64_t *guides, uint64_t n, uint64_t *offsets,

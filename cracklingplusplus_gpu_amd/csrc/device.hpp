@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -53,6 +54,8 @@ constexpr int32_t kHeavyNever = ISSL_HEAVY_NEVER;
 // (issl_tuning.ord_split); term entries the workspace holds for them at most.
 constexpr int32_t kSplitNever = ISSL_SPLIT_NEVER;
 constexpr int32_t kOrdSplitDefault = 65536;
+// Caller streams one index gives workspaces of their own (issl_index_bind_stream).
+constexpr int kMaxBoundStreams = 8;
 constexpr uint64_t kOrdTermCap = 1ULL << 25;
 
 // Site word of the signature-inline layout: signature in bits 0..39,
@@ -244,6 +247,14 @@ struct issl_index {
     bool ws_lazy = false;
     hipStream_t ws_stream = nullptr;
     hipEvent_t ws_done = nullptr;
+    // Caller streams bound with issl_index_bind_stream, each with a workspace
+    // of its own: calls on a bound stream use it (the stream orders them) and
+    // overlap on the device with calls on every other stream.
+    struct BoundWs {
+        hipStream_t stream = nullptr;
+        issl::Workspace ws;
+    };
+    std::vector<std::unique_ptr<BoundWs>> bound;
     // Streamed index (issl_index_open_streamed and issl_index_from_arrays_streamed):
     // d_post and d_inline are null and the postings come from here.
     issl::StreamSource *src = nullptr;

@@ -195,9 +195,9 @@ int issl_index_get_info(const issl_index *ix, issl_index_info *out);
  * posting (the signature-inline layout takes 8).  It is built when every
  * slice has at least 8 positions, all below 20.  A new index streams the
  * signature-inline words; issl_autotune times both layouts and keeps the
- * faster (the 4-byte words pay
- * where the streams are long enough to be bandwidth-bound); set overrides
- * either.  Results are bit-identical either way for an index whose sites each
+ * list-residual words unless the signature-inline words are faster by more
+ * than 3% (the 4-byte words halve the bytes, which pays where the streams
+ * are long enough to be bandwidth-bound); set overrides either.  Results are bit-identical either way for an index whose sites each
  * sit in the list of their own key (issl_index_validate); for one that does
  * not, only the signature-inline words (use = 0) take the distance on the
  * whole signature as the reference does.  get: *built, *in_use and
@@ -237,6 +237,21 @@ int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
 int issl_score_device(issl_index *ix, const uint64_t *guides_device, uint64_t n,
                       const issl_params *params, issl_results *out_device,
                       void *stream);
+
+/* Give a caller stream a scoring workspace of its own on this index, so that
+ * issl_score_device calls on it overlap on the device with calls on the
+ * index's stream and on other bound streams (calls on one stream stay in
+ * order); unbound streams share the index's workspace, each call waiting for
+ * the last one on another stream.  Overlapping batches fill the CUs that the
+ * previous batch's planner, finalizer and last waves leave idle.  Each bound
+ * stream holds its own workspace (tile plans, per-tile partials, queue) while
+ * bound; at most 8 per index; a resident index only.  Unbind waits for the
+ * stream's calls on this index and frees the workspace: unbind a stream
+ * before destroying it (closing the index waits for the whole device when a
+ * bound stream is left).  Binding a bound stream again does nothing.  No
+ * counterpart in the reference, which scores one guide list per process. */
+int issl_index_bind_stream(issl_index *ix, void *stream);
+int issl_index_unbind_stream(issl_index *ix, void *stream);
 
 /* issl_score_device without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate (code under HIP graph
