@@ -1430,9 +1430,9 @@ struct FinArgs {
 // The tile queue's counters back to zero for the next launch sequence (the
 // tile kernel has finished with them).
 __device__ __forceinline__ void reset_queue(const BatchWs &ws) {
-    if (ws.tctr && blockIdx.x == 0)
+    if (ws.tctr && blockIdx.x == 0 && blockIdx.y == 0)
         for (uint32_t i = threadIdx.x; i < kLists * kRegions; i += blockDim.x) ws.tctr[i * kCtrStride] = 0;
-    if (ws.tctr && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (ws.tctr && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
         ws.tctr[kTermCtr] = 0;
         ws.tctr[kTermCtr + 1] = 0;
         ws.tctr[kSplitCtr] = 0;
@@ -1616,6 +1616,17 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
 // wave form makes.
 constexpr uint32_t kFinSerialTiles = 8;   // more tiles: the guide's wave adds them
 
+// One 64-lane workgroup per 64 guides (blockIdx.x) and gridDim.y replicas of
+// it: replica 0 writes the guides with few tiles, one lane each, and replica
+// r adds the group's guides with many tiles whose rank among them is r
+// modulo the replica count, so that a group of long guides is not added by
+// one wave in series.  Each guide is added by one wave in tile order whatever
+// the replica count, so the results do not depend on it.
+__host__ __device__ inline uint32_t fin_replicas(uint64_t groups) {
+    const uint64_t r = 512 / (groups ? groups : 1);
+    return static_cast<uint32_t>(r < 1 ? 1 : r > 64 ? 64 : r);
+}
+
 __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
     reset_queue(a.ws);
     const uint64_t g = blockIdx.x * 64ULL + threadIdx.x;
@@ -1654,8 +1665,9 @@ __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
             exited = (st.sites & kExited) != 0;
         }
     };
+    const uint32_t R = gridDim.y, rep = blockIdx.y;
     // One lane per guide with few tiles.
-    if (mine && K <= kFinSerialTiles) {
+    if (rep == 0 && mine && K <= kFinSerialTiles) {
         double dm, dc;
         uint64_t occ_sum, sites;
         bool exited;
@@ -1674,7 +1686,9 @@ __global__ __launch_bounds__(64) void k_finalize_sum(FinArgs a) {
     // order (the same additions, add_chain), and the counts are summed over
     // the lanes (integers: any order).
     __shared__ double2 s_p[64];
-    for (uint64_t m = __ballot(mine && K > kFinSerialTiles); m; m &= m - 1) {
+    uint32_t rank = 0;
+    for (uint64_t m = __ballot(mine && K > kFinSerialTiles); m; m &= m - 1, ++rank) {
+        if (rank % R != rep) continue;
         const int l = __builtin_ctzll(m);
         const uint64_t gg = blockIdx.x * 64ULL + static_cast<uint64_t>(l);
         const uint32_t KK = static_cast<uint32_t>(__shfl(static_cast<int>(K), l));
@@ -2441,7 +2455,9 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     // guide from its tile's partial, or from nothing, and sets the queue
     // counters back to zero)
     if (!sc.use_cut || ordered)
-        hipLaunchKernelGGL(k_finalize_sum, dim3(static_cast<uint32_t>((n + 63) / 64)), dim3(64), 0, st, fa);
+        hipLaunchKernelGGL(k_finalize_sum,
+                           dim3(static_cast<uint32_t>((n + 63) / 64), fin_replicas((n + 63) / 64)),
+                           dim3(64), 0, st, fa);
     else if (fp32)
         hipLaunchKernelGGL((k_finalize<ISSL_PREC_FP32C>), dim3(gw), dim3(256), 0, st, fa);
     else
