@@ -610,9 +610,22 @@ constexpr uint32_t kHitRing = 128;
 //
 // In RES form a mismatch is carried as a CFD-table index 4p + x (position p,
 // guide ^ site = x there), 7 bits, kAbsent7 for an absent one: s_cq[kAbsent7]
-// is 1.0 (p = 31 lies past the sequence) and the MIT rank terms are read by
-// the same index (0 at kAbsent7).
-constexpr uint32_t kAbsent7 = 127;
+// is 1.0, and so is the MIT factor of its position (kAbsent7 / 4 = 20 lies
+// past the sequence: 1 - weight is 1.0 there).  The RES kernels' factor table
+// therefore holds kResCq entries per wave, the inline kernels' one entry per
+// (doubled position up to kNoPos, base).
+constexpr uint32_t kAbsent7 = 80;
+constexpr int kResCq = 84;
+static_assert(kAbsent7 / 4 == 20 && kAbsent7 < kResCq && kAbsent7 < 128, "absent code");
+
+// Waves per SIMD the tile kernels are built for: the ORDERED list-residual
+// kernels fit five (96 VGPRs, under 32 KB of LDS per workgroup with the
+// table above); the ORDERED signature-inline ones are held to four by their
+// LDS; the BATCHED kernels at depth 8 fit five, at other depths the compiler
+// is left free.
+constexpr int tile_waves_per_simd(int depth, bool queue, bool res, bool ord) {
+    return ord ? (res ? 5 : 4) : (depth == 8 && queue) ? 5 : 1;
+}
 
 // Value of lane `src` (wave-uniform) of a double.
 __device__ __forceinline__ double readlane_f64(double v, uint32_t src) {
@@ -675,7 +688,7 @@ __device__ __forceinline__ void add_chain_prefix(double *p, uint32_t n, double &
 // exit point are the reference's bit for bit; after the exit the wave stops
 // streaming the guide.  (FP64 and the tile queue only.)
 template <int PREC, int DEPTH, bool QUEUE, bool RES, bool ORD = false>
-__global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 1) void k_tiles(TileArgs a) {
+__global__ __launch_bounds__(kTileBlock, tile_waves_per_simd(DEPTH, QUEUE, RES, ORD)) void k_tiles(TileArgs a) {
     static_assert(!ORD || (PREC == ISSL_PREC_FP64 && QUEUE), "in-order tiles are FP64 and queued");
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
     // Site word as streamed and ringed; the hit ring holds the word (or, in
@@ -693,7 +706,8 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
     __shared__ uint32_t s_qpre[QUEUE ? kLists : 1][kRegions];
     __shared__ uint32_t s_qbase[kLists + 1];
     // The current tile's CFD factors by (position, guide ^ site there).
-    __shared__ Real s_cq[kTileWaves][(kNoPos + 1) * 4];
+    constexpr int CQ = RES ? kResCq : static_cast<int>(kNoPos + 1) * 4;
+    __shared__ Real s_cq[kTileWaves][CQ];
     // Arrays sized by the slice count S, in dynamic LDS (tiles_lds_bytes):
     //   delta [kTileWaves][2][S]   tile plans' posting offsets, double-buffered
     //                              (the next tile's plan lands in the other half)
@@ -806,7 +820,7 @@ __global__ __launch_bounds__(kTileBlock, DEPTH != 8 ? 1 : ORD ? 4 : QUEUE ? 5 : 
     }
     // CFD factors of absent mismatches (positions 20..kNoPos): 1.0 in every
     // tile (the tiles rewrite positions 0..19 only).
-    for (int i = 20 * 4 + lane; i < static_cast<int>(kNoPos + 1) * 4; i += 64) s_cq[wave][i] = Real(1);
+    for (int i = 20 * 4 + lane; i < CQ; i += 64) s_cq[wave][i] = Real(1);
     // RES: the code field's fold bits (bit 2j per residual position j) and the
     // occurrence escape.
     const uint32_t res_fold = RES ? (0x55555555u & ((1u << a.res_shift) - 1u)) : 0u;
