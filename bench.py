@@ -398,15 +398,28 @@ def timed(hiprt, rot, ix, steps, warmup, streams=None, **kw):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def pick_streams(hiprt, ix, g_tune, pool, **kw):
-    """Streams the steps run on: consecutive batches on 1, 2 or 3 streams of
-    the index (its own, then bound ones), each count timed on the held-out
-    batch; the fastest is kept.  Returns (count, {count: us per step})."""
+def pick_streams(hiprt, ix, g_tune, bound, **kw):
+    """Streams the steps run on: the index's own stream alone; the two bound
+    streams (each with its own workspace); or the index's stream and both
+    bound ones; with the tile kernels' workgroups per CU on bound streams
+    (ovl_grid) 2 to 5.  (Three streams of the process plus the null stream
+    fill the four hardware queues HIP gives a process by default; more
+    streams would share queues.)  Each choice is timed on the held-out batch
+    and the fastest kept (its ovl_grid set).  Returns (streams, ovl_grid,
+    {choice: us per step})."""
     rot = Rotation(hiprt, [g_tune])
-    sweep = {}
-    for ns in range(1, len(pool) + 1):
-        sweep[ns] = round(timed(hiprt, rot, ix, 30, 6, pool[:ns], **kw) * 1e3, 2)
-    return min(sweep, key=sweep.get), sweep
+    us1 = timed(hiprt, rot, ix, 30, 6, None, **kw) * 1e3
+    sweep = {"1": round(us1, 2)}
+    best = (us1, [None], 0)
+    for label, ss in (("2 bound", bound[:2]), ("index + 2 bound", [None] + bound[:2])):
+        for grid in (2, 3, 4, 5):
+            ix.set_overlap_grid(grid)
+            us = timed(hiprt, rot, ix, 30, 6, ss, **kw) * 1e3
+            sweep[f"{label}, ovl_grid {grid}"] = round(us, 2)
+            if us < best[0]:
+                best = (us, ss, grid)
+    ix.set_overlap_grid(best[2])
+    return best[1], best[2], sweep
 
 
 def kernel_us(hiprt, rot, ix, steps, **kw):
@@ -707,20 +720,24 @@ def main() -> int:
     # this three-kernel sequence, so off by default).
     kw_step = dict(kw, flags=issl.FLAG_GRAPH if args.graph else 0)
 
-    # ---- batches in flight: consecutive steps go round the index's own stream
-    # and up to two streams bound to the index (issl_index_bind_stream), each
-    # with its own scoring workspace, so that one step's planner, finalizer
-    # and last waves overlap the next step's tile kernel.  The count (1-3) is
-    # picked on the held-out batch unless --streams gives it.
-    pool = [None] + [hiprt.Stream() for _ in range(2)]
-    for s_ in pool[1:]:
+    # ---- batches in flight: consecutive steps go round 2 or 3 streams bound
+    # to the index (issl_index_bind_stream), each with its own scoring
+    # workspace, so that one step's planner, finalizer and last waves overlap
+    # the next steps' tile kernels; launches on bound streams run the tile
+    # kernel at ovl_grid workgroups per CU so that several fit at once.  One
+    # stream is the index's own, at the tuned grid.  The choice is made on
+    # the held-out batch unless --streams gives the count (then ovl_grid 3).
+    bound = [hiprt.Stream() for _ in range(2)]
+    for s_ in bound:
         ix.bind_stream(s_)
     if args.streams or args.graph:
-        n_streams, sweep = (1 if args.graph else args.streams), None
+        ns_ = 1 if args.graph else args.streams
+        streams, ovl, sweep = ([None], bound[:2], [None] + bound)[ns_ - 1], 0, None
     else:
-        n_streams, sweep = pick_streams(hiprt, ix, g_tune, pool, **kw_step)
-    streams = pool[:n_streams]
-    log(f"streams: {n_streams}" + (f" (held-out batch, us per step by count: {sweep})" if sweep else ""))
+        streams, ovl, sweep = pick_streams(hiprt, ix, g_tune, bound, **kw_step)
+    n_streams = len(streams)
+    log(f"streams: {n_streams}, ovl_grid {ovl or 'default'}" +
+        (f" (held-out batch, us per step: {sweep})" if sweep else ""))
 
     # ---- timed region: BATCHED FP64, inputs resident in HBM, rotating batches
     warm = max(args.warmup, R if args.graph else 0)
@@ -770,9 +787,11 @@ def main() -> int:
                    "postings_per_step": int(np.mean(p_step)),
                    "site_words": "list-residual, 4 B" if res_used else "signature-inline, 8 B",
                    "streams": n_streams,
-                   "streams_note": "consecutive steps on the index's stream and bound streams "
-                                   "(issl_index_bind_stream), each with its own workspace; the "
-                                   "count picked on the held-out batch",
+                   "ovl_grid": ovl,
+                   "streams_note": "consecutive steps round streams bound to the index "
+                                   "(issl_index_bind_stream), each with its own workspace, tile "
+                                   "kernels at ovl_grid workgroups per CU; count and ovl_grid "
+                                   "picked on the held-out batch",
                    "stream_pick_us_per_step": sweep,
                    "one_stream": {"value": round(Q / (ms_serial * 1e-3), 1),
                                   "ms_per_step": round(ms_serial, 4)},
@@ -837,10 +856,11 @@ def main() -> int:
     # ---- ORDERED form (bit-exact FP64 in the tile kernel, one wave per guide)
     kwo = dict(kw, form=issl.FORM_ORDERED)
     if args.streams:
-        ord_ns, ord_sweep = args.streams, None
+        ord_streams, ord_ovl, ord_sweep = streams, ovl, None
     else:
-        ord_ns, ord_sweep = pick_streams(hiprt, ix, g_tune, pool, **kwo)
-    ord_ms = timed(hiprt, rot, ix, args.steps, args.warmup, pool[:ord_ns], **kwo)
+        ord_streams, ord_ovl, ord_sweep = pick_streams(hiprt, ix, g_tune, bound, **kwo)
+    ord_ns = len(ord_streams)
+    ord_ms = timed(hiprt, rot, ix, args.steps, args.warmup, ord_streams, **kwo)
     ord_ms1 = (timed(hiprt, rot, ix, args.steps, args.warmup, None, **kwo) if ord_ns > 1
                else ord_ms)
     ord_k_us, _ = kernel_us(hiprt, rot, ix, args.steps, **kwo)
@@ -977,7 +997,8 @@ def main() -> int:
                     "achieved_GBps": round(bytes_ref / (ord_k_us * 1e-6) / 1e9, 1),
                     "same_bits_as_plain": bool(ord_same), "batches_compared": covered,
                     "plain_speedup": round(plain_ms / ord_ms, 3),
-                    "streams": ord_ns, "stream_pick_us_per_step": ord_sweep,
+                    "streams": ord_ns, "ovl_grid": ord_ovl,
+                    "stream_pick_us_per_step": ord_sweep,
                     "one_stream": {"value": round(Q / (ord_ms1 * 1e-3), 1),
                                    "ms_per_step": round(ord_ms1, 4)},
                     "mode": "ORDERED: FP64 hits added in the reference's order, one wave "

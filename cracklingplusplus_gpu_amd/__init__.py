@@ -429,15 +429,24 @@ class Index:
 
     def set_tuning(self, chunk: int = 1024, depth: int = 8, grid_per_cu: int = 6,
                    direct_depth: int = 8, heavy_kpost: int = 0, ord_depth: Optional[int] = None,
-                   ord_grid: Optional[int] = None, ord_split: int = 0) -> None:
+                   ord_grid: Optional[int] = None, ord_split: int = 0,
+                   ovl_grid: int = 0) -> None:
         """heavy_kpost: guides above heavy_kpost*1024 postings use the tile kernel,
         the rest one wave per guide (0: every guide uses tiles; HEAVY_NEVER: none).
         ord_depth / ord_grid: the ORDERED form's shape (default: depth, grid_per_cu);
         ord_split: ORDERED guides above this many postings are split over tiles
-        (0: the default threshold; SPLIT_NEVER: never)."""
+        (0: the default threshold; SPLIT_NEVER: never); ovl_grid: tile-kernel
+        workgroups per CU of launches on bound streams (0: the default)."""
         t = _capi.issl_tuning(chunk, 256, depth, grid_per_cu, direct_depth, heavy_kpost, 0.0,
                               depth if ord_depth is None else ord_depth,
-                              grid_per_cu if ord_grid is None else ord_grid, ord_split, 0)
+                              grid_per_cu if ord_grid is None else ord_grid, ord_split, ovl_grid)
+        check(lib().issl_set_tuning(self._h, byref(t)))
+
+    def set_overlap_grid(self, ovl_grid: int) -> None:
+        """Tile-kernel workgroups per CU for launches on streams bound to the
+        index (0: the default); the rest of the tuning is kept."""
+        t = self.tuning()
+        t.ovl_grid = int(ovl_grid)
         check(lib().issl_set_tuning(self._h, byref(t)))
 
 

@@ -55,7 +55,7 @@ class issl_tuning(Structure):
                 ("heavy_kpost", c_int32),
                 ("us_per_guide", c_double),
                 ("ord_depth", c_int32), ("ord_grid", c_int32), ("ord_split", c_int32),
-                ("reserved", c_int32)]
+                ("ovl_grid", c_int32)]
 
 
 HEAVY_NEVER = 0x7FFFFFFF

@@ -2515,7 +2515,7 @@ int launch_ordered(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
     issl_tuning tn = tn_in;
     if (tn.chunk <= 0) tn = default_tuning();
     tn.depth = ix->ord_depth;
-    tn.grid_per_cu = ix->ord_grid;
+    tn.grid_per_cu = tn_in.ovl_grid > 0 ? tn_in.ovl_grid : ix->ord_grid;
     // One tile per guide: the tile covers the longest possible guide.
     const uint32_t ord_chunk =
         static_cast<uint32_t>(std::max<uint64_t>(64, (ix->p_max + 63) / 64 * 64));
@@ -2555,6 +2555,7 @@ int launch_batched(issl_index *ix, const uint64_t *d_guides, uint64_t n, const S
                                   "visit 2^31 postings): use the plain form");
     issl_tuning tn = tn_in;
     if (tn.chunk <= 0) tn = default_tuning();
+    if (tn_in.ovl_grid > 0) tn.grid_per_cu = tn_in.ovl_grid;   // streams bound to the index
     if (chunk > 0) tn.chunk = chunk;
     if (tn.chunk % 64 != 0 || tn.chunk > (1 << 20))
         return fail(ISSL_ERR_ARG, "chunk must be a multiple of 64 up to 2^20");

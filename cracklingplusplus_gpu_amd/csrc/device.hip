@@ -1157,11 +1157,18 @@ int score_with(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_
                hipStream_t st) {
     uint64_t *distinct = (p->flags & ISSL_FLAG_DISTINCT) ? ix->d_distinct : nullptr;
     if (ix->src) return fail(ISSL_ERR_ARG, "a streamed index is scored through issl_score or issl_score_device");
+    // Launches on a bound stream run beside other streams' launches: the tile
+    // kernels take ovl_grid workgroups per CU (carried to the launchers in the
+    // tuning's ovl_grid; other launches, the autotuner's included, carry 0).
+    bool on_bound = false;
+    for (const auto &b : ix->bound) on_bound = on_bound || b->stream == st;
+    issl_tuning t = tn;
+    t.ovl_grid = on_bound ? (ix->ovl_grid ? ix->ovl_grid : kOvlGridDefault) : 0;
     if (p->form == ISSL_FORM_ORDERED && ordered_fits(ix))
-        return launch_ordered(ix, d_guides, n, sc, tn, o, distinct, ws, st);
+        return launch_ordered(ix, d_guides, n, sc, t, o, distinct, ws, st);
     if (p->form != ISSL_FORM_BATCHED)
         return launch_plain(ix, d_guides, n, sc, o, distinct, nullptr, nullptr, st);
-    return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, tn, o, distinct, ws, st);
+    return launch_batched(ix, d_guides, n, sc, p->precision, p->chunk, t, o, distinct, ws, st);
 }
 
 }  // namespace issl

@@ -56,6 +56,8 @@ constexpr int32_t kSplitNever = ISSL_SPLIT_NEVER;
 constexpr int32_t kOrdSplitDefault = 65536;
 // Caller streams one index gives workspaces of their own (issl_index_bind_stream).
 constexpr int kMaxBoundStreams = 8;
+// Tile-kernel workgroups per CU of launches on bound streams (issl_tuning.ovl_grid 0).
+constexpr int32_t kOvlGridDefault = 3;
 constexpr uint64_t kOrdTermCap = 1ULL << 25;
 
 // Site word of the signature-inline layout: signature in bits 0..39,
@@ -231,6 +233,8 @@ struct issl_index {
     // (one tile per guide): issl_autotune times them beside the BATCHED
     // shapes, issl_set_tuning sets them with the BATCHED ones.
     int32_t ord_depth = 4, ord_grid = 6, ord_split = issl::kOrdSplitDefault;
+    // Tile-kernel workgroups per CU of launches on bound streams (0: the default).
+    int32_t ovl_grid = 0;
     // Kernel timing: event pairs around the dominant kernel of each launch.
     int timing = 0;                 // 0 off; else dispatches per event pair
     std::vector<hipEvent_t> ev_pool;

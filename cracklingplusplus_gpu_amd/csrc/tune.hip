@@ -113,7 +113,7 @@ extern "C" int issl_get_tuning(issl_index *ix, issl_tuning *out) {
     out->ord_depth = ix->ord_depth;
     out->ord_grid = ix->ord_grid;
     out->ord_split = ix->ord_split;
-    out->reserved = 0;
+    out->ovl_grid = ix->ovl_grid;
     return ok();
 }
 
@@ -129,6 +129,7 @@ extern "C" int issl_set_tuning(issl_index *ix, const issl_tuning *in) {
     if (!(od == 0 || od == 4 || od == 8 || od == 16) || og < 0 || og > 32 ||
         !(os == 0 || os >= 64))
         return fail(ISSL_ERR_ARG, "ORDERED tuning out of range");
+    if (in->ovl_grid < 0 || in->ovl_grid > 32) return fail(ISSL_ERR_ARG, "ovl_grid out of range");
     std::lock_guard<std::mutex> lk(ix->mu);
     ix->tuning = *in;
     ix->tuning.block = 256;
@@ -137,6 +138,8 @@ extern "C" int issl_set_tuning(issl_index *ix, const issl_tuning *in) {
     ix->ord_depth = od ? od : def.ord_depth;
     ix->ord_grid = og ? og : def.ord_grid;
     ix->ord_split = os ? os : def.ord_split;
+    ix->ovl_grid = in->ovl_grid;
+    ix->tuning.ovl_grid = 0;
     return ok();
 }
 
@@ -152,6 +155,7 @@ extern "C" int issl_autotune_precision(issl_index *ix, const uint64_t *sample, u
     issl_tuning cached{};
     int cached_res = 0;
     if (cache_lookup(cache_path, key, &cached, &cached_res)) {
+        cached.ovl_grid = ix->ovl_grid;   // not part of the timed tuning: kept
         int rc = issl_set_tuning(ix, &cached);
         if (rc == ISSL_OK) {
             std::lock_guard<std::mutex> lk(ix->mu);

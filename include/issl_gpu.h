@@ -301,7 +301,12 @@ typedef struct {
                              of `chunk` postings that run in parallel and
                              write their hits' terms, which one wave then adds
                              in order; ISSL_SPLIT_NEVER = one tile per guide */
-    int32_t reserved;
+    int32_t ovl_grid;     /* persistent workgroups per CU of the tile kernels
+                             (both forms) for launches on streams bound to
+                             the index (issl_index_bind_stream), so that the
+                             launches of several streams fit on the CUs at
+                             once; 0 = 3.  Not timed by the autotuner (it
+                             times one launch at a time) */
 } issl_tuning;
 
 #define ISSL_HEAVY_NEVER 0x7FFFFFFF

@@ -126,6 +126,7 @@ def test_bound_streams_overlap_with_own_workspaces(gpu_available, form, precisio
             ix.bind_stream(s1)
             ix.bind_stream(s2)
             ix.bind_stream(s1)    # binding again does nothing
+            ix.set_overlap_grid(2)   # bound streams' tile kernels at 2 workgroups per CU
             plan = [(k % 4, s) for k, s in enumerate([s1, s2, None] * 6)]
             outs = []
             for k, s in plan:
@@ -168,6 +169,12 @@ def test_bind_stream_errors(gpu_available):
             for s in streams[:8]:
                 ix.unbind_stream(s)
             ix.bind_stream(streams[8])
+            # the bound streams' tile-kernel grid is kept beside the tuning
+            ix.set_overlap_grid(2)
+            assert ix.tuning().ovl_grid == 2
+            with pytest.raises(issl.IsslError):
+                ix.set_overlap_grid(33)
+            assert ix.tuning().ovl_grid == 2
             # closing the index with a stream still bound is allowed
     finally:
         for s in streams:
