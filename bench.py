@@ -875,11 +875,22 @@ def main() -> int:
     ord75_ms = timed(hiprt, rot, ix, args.steps, args.warmup, **dict(kw75, form=issl.FORM_ORDERED))
     ord75 = rot.results()[:covered]
     plain75_ms = timed(hiprt, rot, ix, args.plain_steps, 1, **dict(kw75, form=issl.FORM_PLAIN))
+    plain75 = rot.results()[:covered]
     ord75_same = all(np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
-                     for a, b in zip(ord75, rot.results()[:covered]) for k in a)
+                     for a, b in zip(ord75, plain75) for k in a)
     bat75_ms = timed(hiprt, rot, ix, args.steps, args.warmup, **kw75)
+    bat75 = rot.results()[:covered]
+    # threshold decisions (pass/fail calls) of the BATCHED forms against the
+    # reference's (PLAIN, bit-exact), on the batches PLAIN scored
+    rot.run(ix, covered, 0, **dict(kw75, precision=issl.PREC_FP32C))
+    hiprt.synchronize()
+    f75 = rot.results()[:covered]
+    calls = np.concatenate([r["pass"] for r in plain75])
+    agree75 = {name: float(np.mean(np.concatenate([r["pass"] for r in rs]) == calls))
+               for name, rs in (("batched_fp64", bat75), ("batched_fp32c", f75))}
+    agree75["guides"] = int(calls.shape[0])
     log(f"threshold 75: ordered {ord75_ms:.4f} ms/step (bits of plain {ord75_same}), "
-        f"plain {plain75_ms:.4f}, batched {bat75_ms:.4f}")
+        f"plain {plain75_ms:.4f}, batched {bat75_ms:.4f}; calls agreeing with plain {agree75}")
 
     # ---- precision against the in-order FP64 path (PLAIN), every batch
     cat = lambda rs, f: np.concatenate([r[f] for r in rs])  # noqa: E731
@@ -1010,6 +1021,7 @@ def main() -> int:
                                    "ms_per_step": round(plain75_ms, 4)},
                          "batched_fp64": {"value": round(Q / (bat75_ms * 1e-3), 1),
                                           "ms_per_step": round(bat75_ms, 4)},
+                         "call_agreement_vs_plain": agree75,
                          "mode": "MIT+CFD at threshold 75: the exit is armed (cut 33.3)"},
         "plain": {"value": round(Q / (plain_ms * 1e-3), 1), "unit": "guides/s",
                   "ms_per_step": round(plain_ms, 4), "kernel_us": round(plain_k_us, 2),
