@@ -1183,7 +1183,10 @@ extern "C" int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
     if (!guides) return fail(ISSL_ERR_ARG, "null guides");
     std::lock_guard<std::mutex> lk(ix->mu);
     DeviceGuard guard(ix->device);
-    const OutLayout L(n);
+    // Only the columns the caller asked for are written and copied back
+    // (the pipeline module asks for MIT, CFD and the call: 17 of the 49 bytes
+    // per guide).
+    const OutLayout L(n, *out);
     if ((rc = ix->ws.guides.reserve(n * 8)) != ISSL_OK) return rc;
     if ((rc = ix->ws.out.reserve(L.total)) != ISSL_OK) return rc;
     if ((rc = ix->ws.pinned.reserve(n * 8 + L.total)) != ISSL_OK) return rc;
@@ -1193,14 +1196,7 @@ extern "C" int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
     if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
     ISSL_HIP(hipMemcpyAsync(ix->ws.guides.p, pin, n * 8, hipMemcpyHostToDevice, st));
     char *ob = static_cast<char *>(ix->ws.out.p);
-    DevResults o;
-    o.mit = reinterpret_cast<double *>(ob + L.mit);
-    o.cfd = reinterpret_cast<double *>(ob + L.cfd);
-    o.mit_sum = reinterpret_cast<double *>(ob + L.msum);
-    o.cfd_sum = reinterpret_cast<double *>(ob + L.csum);
-    o.occ = reinterpret_cast<uint64_t *>(ob + L.occ);
-    o.sites = reinterpret_cast<uint64_t *>(ob + L.sites);
-    o.pass = ob + L.pass;
+    const DevResults o = L.bind(ob);
     if ((rc = score_on_device(ix, static_cast<const uint64_t *>(ix->ws.guides.p), n, params, sc, o,
                               st)) != ISSL_OK)
         return rc;

@@ -304,9 +304,30 @@ int score_with(issl_index *ix, const uint64_t *d_guides, uint64_t n, const issl_
                const ScoreScalars &sc, const issl_tuning &tn, const DevResults &o, Workspace &ws,
                hipStream_t stream);
 
-// Byte layout of the per-call result columns for n guides.
+// Byte layout of the per-call result columns for n guides: all seven, or
+// only the columns a caller asked for, packed one after another (the others
+// at kNone, bound to null device pointers, so the kernels skip them).
 struct OutLayout {
+    static constexpr size_t kNone = ~static_cast<size_t>(0);
     size_t mit, cfd, msum, csum, occ, sites, pass, total;
+    OutLayout(uint64_t n, const issl_results &want) {
+        const size_t d = n * 8;
+        size_t off = 0;
+        auto take = [&](const void *w, size_t bytes) {
+            if (!w) return kNone;
+            const size_t o = off;
+            off += bytes;
+            return o;
+        };
+        mit = take(want.mit, d);
+        cfd = take(want.cfd, d);
+        msum = take(want.mit_sum, d);
+        csum = take(want.cfd_sum, d);
+        occ = take(want.occ_count, d);
+        sites = take(want.site_count, d);
+        pass = take(want.pass, ((n + 255) / 256) * 256);
+        total = off ? off : 256;
+    }
     explicit OutLayout(uint64_t n) {
         const size_t d = n * 8;
         mit = 0;
@@ -319,14 +340,15 @@ struct OutLayout {
         total = pass + ((n + 255) / 256) * 256;
     }
     DevResults bind(char *base) const {
+        auto at = [&](size_t off) { return off == kNone ? nullptr : base + off; };
         DevResults o;
-        o.mit = reinterpret_cast<double *>(base + mit);
-        o.cfd = reinterpret_cast<double *>(base + cfd);
-        o.mit_sum = reinterpret_cast<double *>(base + msum);
-        o.cfd_sum = reinterpret_cast<double *>(base + csum);
-        o.occ = reinterpret_cast<uint64_t *>(base + occ);
-        o.sites = reinterpret_cast<uint64_t *>(base + sites);
-        o.pass = base + pass;
+        o.mit = reinterpret_cast<double *>(at(mit));
+        o.cfd = reinterpret_cast<double *>(at(cfd));
+        o.mit_sum = reinterpret_cast<double *>(at(msum));
+        o.cfd_sum = reinterpret_cast<double *>(at(csum));
+        o.occ = reinterpret_cast<uint64_t *>(at(occ));
+        o.sites = reinterpret_cast<uint64_t *>(at(sites));
+        o.pass = at(pass);
         return o;
     }
 };

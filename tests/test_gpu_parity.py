@@ -374,3 +374,39 @@ def test_one_tile_launches(cases, name):
             assert (r.passed == gd[key + "_pass"]).all(), what
     finally:
         ix.set_tuning()
+
+
+@pytest.mark.parametrize("form", [issl.FORM_ORDERED, issl.FORM_BATCHED, issl.FORM_PLAIN])
+def test_score_with_a_subset_of_columns(cases, form):
+    """issl_score writes and copies back only the result columns the caller
+    asks for (packed in the workspace); each subset equals the same columns
+    of a full call: the pipeline module's (MIT, CFD, call), the CLI's (MIT,
+    CFD, occurrences), single columns and none."""
+    import ctypes
+    from cracklingplusplus_gpu_amd import _capi
+    print(f"[gpu-parity] column subsets form {form}", flush=True)
+    hx, ix, gold = cases["mid50k_10x10"]
+    g = np.ascontiguousarray(gold["guides"], dtype=np.uint64)
+    n = g.shape[0]
+    kw = dict(method=issl.METHOD_MIT_AND_CFD, threshold=75.0, form=form,
+              precision=issl.PREC_FP64)
+    full = ix.score(g, **kw)
+    cols = {"mit": (np.float64, full.mit), "cfd": (np.float64, full.cfd),
+            "occ_count": (np.uint64, full.occ_count), "site_count": (np.uint64, full.site_count),
+            "pass": (np.uint8, full.passed), "mit_sum": (np.float64, full.mit_sum),
+            "cfd_sum": (np.float64, full.cfd_sum)}
+    p = ix.params(kw["method"], kw["threshold"], form, kw["precision"])
+    for subset in (("mit", "cfd", "pass"), ("mit", "cfd", "occ_count"), ("pass",),
+                   ("site_count", "cfd_sum"), ()):
+        bufs = {k: np.full(n, 0xAB, dtype=cols[k][0]) for k in subset}
+        r = _capi.issl_results(*[bufs[k].ctypes.data_as(ctypes.c_void_p) if k in bufs else None
+                                 for k in ("mit", "cfd", "occ_count", "site_count", "pass",
+                                           "mit_sum", "cfd_sum")])
+        issl.check(issl.lib().issl_score(ix.handle, g.ctypes.data_as(ctypes.c_void_p), n,
+                                         ctypes.byref(p), ctypes.byref(r)))
+        for k in subset:
+            want = cols[k][1]
+            if want.dtype == np.float64:
+                assert (bits(bufs[k]) == bits(want)).all(), (subset, k)
+            else:
+                assert (bufs[k].view(np.uint8) == np.ascontiguousarray(want).view(np.uint8)).all(), (subset, k)
