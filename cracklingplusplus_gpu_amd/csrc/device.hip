@@ -1183,34 +1183,63 @@ extern "C" int issl_score(issl_index *ix, const uint64_t *guides, uint64_t n,
     if (!guides) return fail(ISSL_ERR_ARG, "null guides");
     std::lock_guard<std::mutex> lk(ix->mu);
     DeviceGuard guard(ix->device);
-    // Only the columns the caller asked for are written and copied back
-    // (the pipeline module asks for MIT, CFD and the call: 17 of the 49 bytes
-    // per guide).
-    const OutLayout L(n, *out);
+    // A resident index scores the guides in chunks of kHostChunk, each with
+    // its guides copied into pinned staging, sent, scored and its results sent
+    // back on the index's stream: the host stages chunk j + 1 and copies out
+    // chunk j's results while the device works on later chunks.  A streamed
+    // index passes all its slices through HBM once per launch, so it scores
+    // the call in one piece.  Only the columns the caller asked for are
+    // written and copied back (the pipeline module asks for MIT, CFD and the
+    // call: 17 of the 49 bytes per guide); each chunk's columns lie packed
+    // together at j * L.total in the output buffers.
+    const uint64_t C = ix->src ? n : std::min<uint64_t>(n, kHostChunk);
+    const uint64_t nch = (n + C - 1) / C;
+    const OutLayout L(C, *out);
     if ((rc = ix->ws.guides.reserve(n * 8)) != ISSL_OK) return rc;
-    if ((rc = ix->ws.out.reserve(L.total)) != ISSL_OK) return rc;
-    if ((rc = ix->ws.pinned.reserve(n * 8 + L.total)) != ISSL_OK) return rc;
+    if ((rc = ix->ws.out.reserve(nch * L.total)) != ISSL_OK) return rc;
+    if ((rc = ix->ws.pinned.reserve(n * 8 + nch * L.total)) != ISSL_OK) return rc;
     char *pin = static_cast<char *>(ix->ws.pinned.p);
-    std::memcpy(pin, guides, n * 8);
-    hipStream_t st = ix->stream;
-    if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
-    ISSL_HIP(hipMemcpyAsync(ix->ws.guides.p, pin, n * 8, hipMemcpyHostToDevice, st));
-    char *ob = static_cast<char *>(ix->ws.out.p);
-    const DevResults o = L.bind(ob);
-    if ((rc = score_on_device(ix, static_cast<const uint64_t *>(ix->ws.guides.p), n, params, sc, o,
-                              st)) != ISSL_OK)
-        return rc;
     char *hout = pin + n * 8;
-    ISSL_HIP(hipMemcpyAsync(hout, ob, L.total, hipMemcpyDeviceToHost, st));
+    char *ob = static_cast<char *>(ix->ws.out.p);
+    uint64_t *dg = static_cast<uint64_t *>(ix->ws.guides.p);
+    hipStream_t st = ix->stream;
+    // chunk j's results are on the host once done[j] has passed
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events() {
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
+        }
+    } done;
+    if (nch > 1) {
+        done.e.assign(nch, nullptr);
+        for (hipEvent_t &x : done.e) ISSL_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    }
+    if ((rc = ws_acquire(ix, st)) != ISSL_OK) return rc;
+    for (uint64_t j = 0; j < nch; ++j) {
+        const uint64_t g0 = j * C, cn = std::min(C, n - g0);
+        std::memcpy(pin + g0 * 8, guides + g0, cn * 8);
+        ISSL_HIP(hipMemcpyAsync(dg + g0, pin + g0 * 8, cn * 8, hipMemcpyHostToDevice, st));
+        const OutLayout Lj(cn, *out);
+        if ((rc = score_on_device(ix, dg + g0, cn, params, sc, Lj.bind(ob + j * L.total), st)) != ISSL_OK)
+            return rc;
+        ISSL_HIP(hipMemcpyAsync(hout + j * L.total, ob + j * L.total, Lj.total, hipMemcpyDeviceToHost, st));
+        if (nch > 1) ISSL_HIP(hipEventRecord(done.e[j], st));
+    }
     if ((rc = ws_release(ix, st)) != ISSL_OK) return rc;
-    ISSL_HIP(hipStreamSynchronize(st));
-    if (out->mit) std::memcpy(out->mit, hout + L.mit, n * 8);
-    if (out->cfd) std::memcpy(out->cfd, hout + L.cfd, n * 8);
-    if (out->mit_sum) std::memcpy(out->mit_sum, hout + L.msum, n * 8);
-    if (out->cfd_sum) std::memcpy(out->cfd_sum, hout + L.csum, n * 8);
-    if (out->occ_count) std::memcpy(out->occ_count, hout + L.occ, n * 8);
-    if (out->site_count) std::memcpy(out->site_count, hout + L.sites, n * 8);
-    if (out->pass) std::memcpy(out->pass, hout + L.pass, n);
+    if (nch == 1) ISSL_HIP(hipStreamSynchronize(st));
+    for (uint64_t j = 0; j < nch; ++j) {
+        if (nch > 1) ISSL_HIP(hipEventSynchronize(done.e[j]));
+        const uint64_t g0 = j * C, cn = std::min(C, n - g0);
+        const OutLayout Lj(cn, *out);
+        const char *h = hout + j * L.total;
+        if (out->mit) std::memcpy(out->mit + g0, h + Lj.mit, cn * 8);
+        if (out->cfd) std::memcpy(out->cfd + g0, h + Lj.cfd, cn * 8);
+        if (out->mit_sum) std::memcpy(out->mit_sum + g0, h + Lj.msum, cn * 8);
+        if (out->cfd_sum) std::memcpy(out->cfd_sum + g0, h + Lj.csum, cn * 8);
+        if (out->occ_count) std::memcpy(out->occ_count + g0, h + Lj.occ, cn * 8);
+        if (out->site_count) std::memcpy(out->site_count + g0, h + Lj.sites, cn * 8);
+        if (out->pass) std::memcpy(out->pass + g0, h + Lj.pass, cn);
+    }
     return ok();
 }
 

@@ -56,6 +56,8 @@ constexpr int32_t kSplitNever = ISSL_SPLIT_NEVER;
 constexpr int32_t kOrdSplitDefault = 65536;
 // Caller streams one index gives workspaces of their own (issl_index_bind_stream).
 constexpr int kMaxBoundStreams = 8;
+// Guides per piece of a host-buffer call (issl_score) on a resident index.
+constexpr uint64_t kHostChunk = 1ULL << 18;
 // Tile-kernel workgroups per CU of launches on bound streams (issl_tuning.ovl_grid 0).
 constexpr int32_t kOvlGridDefault = 3;
 constexpr uint64_t kOrdTermCap = 1ULL << 25;

@@ -410,3 +410,26 @@ def test_score_with_a_subset_of_columns(cases, form):
                 assert (bits(bufs[k]) == bits(want)).all(), (subset, k)
             else:
                 assert (bufs[k].view(np.uint8) == np.ascontiguousarray(want).view(np.uint8)).all(), (subset, k)
+
+
+@pytest.mark.parametrize("form", [issl.FORM_ORDERED, issl.FORM_BATCHED, issl.FORM_PLAIN])
+def test_host_call_in_pieces(cases, form):
+    """A host-buffer call of more than 262,144 guides runs in pieces whose
+    staging and result copies overlap the device's work: 600,000 guides
+    (the golden guides repeated, three pieces) give, guide by guide, the bits
+    of a call on the golden guides alone; with the module's three columns
+    as well as all seven."""
+    print(f"[gpu-parity] host call in pieces, form {form}", flush=True)
+    hx, ix, gold = cases["mid50k_10x10"]
+    g = np.ascontiguousarray(gold["guides"], dtype=np.uint64)
+    m = g.shape[0]
+    big = np.ascontiguousarray(np.resize(g, 600_000))
+    idx = np.arange(big.shape[0]) % m
+    for t in (0.0, 75.0):
+        kw = dict(method=issl.METHOD_MIT_AND_CFD, threshold=t, form=form, precision=issl.PREC_FP64)
+        one = ix.score(g, **kw)
+        many = ix.score(big, **kw)
+        for f in ("mit", "cfd", "mit_sum", "cfd_sum"):
+            assert (bits(getattr(many, f)) == bits(getattr(one, f))[idx]).all(), (t, f)
+        for f in ("occ_count", "site_count", "passed"):
+            assert (getattr(many, f) == getattr(one, f)[idx]).all(), (t, f)
