@@ -248,6 +248,22 @@ __device__ __forceinline__ bool first_seen(const uint32_t p[4], int sl, const ui
     return (earlier & ~nm) == 0;
 }
 
+// MIT value of a site at distance k = 1..4 from the closed form's factors,
+// in its order: ((t1 * t2) * t3) * 100, t1 the product of 1 - weight over the
+// mismatch positions p (kNoPos for absent ones, whose factor is 1.0), t2 and
+// t3 by the count and the span p_last - p_0 (ISSLCreateIndex.cpp:235-261);
+// the same bits as the MIT table entry mit_rank(p, k).
+template <typename T>
+__device__ __forceinline__ T mit_closed(const uint32_t p[4], int k, const double *om,
+                                        const double2 *t23) {
+    double t1 = om[p[0]] * om[p[1]];
+    t1 = t1 * om[p[2]];
+    t1 = t1 * om[p[3]];
+    const uint32_t last = k >= 4 ? p[3] : k == 3 ? p[2] : k == 2 ? p[1] : p[0];
+    const double2 f = t23[k * 20 + static_cast<int>(last - p[0])];
+    return static_cast<T>(((t1 * f.x) * f.y) * 100.0);
+}
+
 // Fold of one site as a 20-bit mask (bit j set iff position j differs),
 // from the two 32-bit halves of guide ^ site word: positions 0..15 in the
 // low half, 16..19 in bits 0..7 of the high half (the bits above are the
@@ -636,6 +652,20 @@ __device__ __forceinline__ double readlane_f64(double v, uint32_t src) {
         __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b >> 32)), static_cast<int>(src)));
     return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
 }
+
+// Value of lane `src` (wave-uniform) of a 64-bit integer, and of a float or
+// double.
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t src) {
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), static_cast<int>(src)));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), static_cast<int>(src)));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ float readlane_real(float v, uint32_t src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), static_cast<int>(src)));
+}
+__device__ __forceinline__ double readlane_real(double v, uint32_t src) { return readlane_f64(v, src); }
 
 // Adds p[0], p[2], p[4], ... (n terms, every other double) to `a`, one at a
 // time in order, U terms read before they are added.  Lanes 0 and 1 run it
@@ -1428,6 +1458,7 @@ struct FinArgs {
     const SliceMeta *meta;
     const void *mit;
     const void *cfd;
+    const double *mitf;           // MIT closed-form factors (kMitFactorEntries)
     int S;
     uint64_t n;
     ScoreScalars sc;
@@ -1458,6 +1489,9 @@ __global__ void k_zero_queue(uint32_t *tctr) {
         tctr[i] = 0;
 }
 
+// Rows of 64 postings whose site words the exit rescan loads together.
+constexpr int kRescanRows = 8;
+
 template <int PREC>
 __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     using Real = typename std::conditional<PREC == ISSL_PREC_FP32C, float, double>::type;
@@ -1466,9 +1500,19 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
     __shared__ uint64_t s_posmask[kNoPos + 1];
     __shared__ uint64_t s_pre[W][kMaxSlices + 1];
     __shared__ uint64_t s_delta[W][kMaxSlices];
+    // The rescan's term tables: CFD factors, and the MIT closed form's
+    // factors (1 - weight by position, kNoPos giving 1.0; then the count and
+    // span factors as pairs).
+    __shared__ Real s_cfd[kCfdEntries];
+    __shared__ double s_om[kMitOneMinus];
+    __shared__ double2 s_t23[5 * 20];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     reset_queue(a.ws);
     if (tid < a.S) s_fold[tid] = a.meta[tid].fold;
+    for (int i = tid; i < kCfdEntries; i += 256) s_cfd[i] = static_cast<const Real *>(a.cfd)[i];
+    if (tid < kMitOneMinus) s_om[tid] = a.mitf[tid];
+    if (tid < 5 * 20)
+        s_t23[tid] = make_double2(a.mitf[kMitOneMinus + 2 * tid], a.mitf[kMitOneMinus + 2 * tid + 1]);
     __syncthreads();
     build_posmask(s_posmask, s_fold, a.S, tid);
     __syncthreads();
@@ -1515,72 +1559,110 @@ __global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
             mine = TilePart{};
             if (k0 + lane < K) mine = a.ws.part[static_cast<uint64_t>(k0 + lane) * a.n + g];
         }
-        const int kl = static_cast<int>(k - k0);
-        const uint64_t o = shfl64(mine.occ, kl);
-        const uint64_t si = shfl64(mine.sites, kl);
+        const uint32_t kl = k - k0;   // wave-uniform
+        const uint64_t o = readlane_u64(mine.occ, kl);
+        const uint64_t si = readlane_u64(mine.sites, kl);
         const double qm = dm, qc = dc;
-        dm += __shfl(mine.m, kl);
-        dc += __shfl(mine.c, kl);
+        dm += readlane_f64(mine.m, kl);
+        dc += readlane_f64(mine.c, kl);
         if (!(a.sc.use_cut && bexit(a.sc.method, dm, dc, a.sc.cut))) {
             occ_sum += o;
             sites += si;
             continue;
         }
         // The total meets the exit test within tile k: restore the sums before
-        // it and add its hits one at a time, in order, to stop exactly where
-        // the reference's loop stops (lib/ISSLScoringModule.cpp:365-401).
+        // it and walk its postings in order, 64 at a time.  A row whose hits'
+        // sum (a butterfly over the wave) leaves the test unmet is taken
+        // whole; in the row where the test would be met the hits are added
+        // one at a time, in order, to stop where the reference's loop stops
+        // (lib/ISSLScoringModule.cpp:365-401).  The site words of
+        // kRescanRows rows are loaded together.
         dm = qm;
         dc = qc;
         load_lists();
-        const Real *mitg = static_cast<const Real *>(a.mit);
-        const Real *cfdg = static_cast<const Real *>(a.cfd);
         const uint64_t P = s_pre[wave][S];
         const uint64_t a0 = static_cast<uint64_t>(k) * C;
         const uint64_t a1 = std::min<uint64_t>(P, a0 + C);
         int cur = 0;
         bool done = false;
-        for (uint64_t b0 = a0; b0 < a1 && !done; b0 += 64) {
-            const uint64_t pos = b0 + lane;
-            bool hit = false;
-            Real tm = 0, tc = 0;
-            uint32_t occ = 0;
-            uint64_t addr = 0;
-            if (pos < a1) {
-                while (pos >= s_pre[wave][cur + 1]) ++cur;
-                addr = pos + s_delta[wave][cur];
-                const uint64_t w = a.site[addr];
-                const uint64_t sg = w & kSigBits;
-                const uint64_t f = bfold(q ^ sg);
-                const int dist = __popcll(f);
-                uint32_t mp[4];
-                mismatches(f, mp);
-                hit = dist <= 4 && first_seen(mp, cur, s_posmask);
-                if (hit) {
-                    occ = static_cast<uint32_t>(w >> 40);
-                    if (occ == kOccEscape) occ = static_cast<uint32_t>(a.post[addr] >> 32);
-                    const Real ro = to_real<Real>(occ);
-                    if (a.sc.calc_mit && dist > 0) tm = mitg[mit_rank(mp, dist)] * ro;
-                    if (a.sc.calc_cfd) tc = cfd_product<Real>(q, sg, mp, dist, cfdg) * ro;
+        for (uint64_t r0 = a0; r0 < a1 && !done; r0 += 64 * kRescanRows) {
+            uint64_t w[kRescanRows];
+            int cs[kRescanRows];
+#pragma unroll
+            for (int r = 0; r < kRescanRows; ++r) {
+                const uint64_t pos = r0 + 64 * r + lane;
+                w[r] = 0;
+                cs[r] = -1;
+                if (pos < a1) {
+                    while (pos >= s_pre[wave][cur + 1]) ++cur;
+                    cs[r] = cur;
+                    w[r] = a.site[pos + s_delta[wave][cur]];
                 }
             }
-            uint64_t bal = __ballot(hit);
-            while (bal) {
-                const int h = __builtin_ctzll(bal);
-                bal &= bal - 1;
-                const Real hm = __shfl(tm, h), hc = __shfl(tc, h);
-                const uint32_t ho = static_cast<uint32_t>(__shfl(static_cast<int>(occ), h));
-                dm += static_cast<double>(hm);
-                dc += static_cast<double>(hc);
-                occ_sum += ho;
-                sites += 1;
-                if (a.distinct && lane == h)
+#pragma unroll
+            for (int r = 0; r < kRescanRows; ++r) {
+                if (done) break;
+                const uint64_t pos = r0 + 64 * r + lane;
+                bool hit = false;
+                Real tm = 0, tc = 0;
+                uint32_t occ = 0;
+                uint64_t addr = 0;
+                if (cs[r] >= 0) {
+                    addr = pos + s_delta[wave][cs[r]];
+                    const uint64_t sg = w[r] & kSigBits;
+                    const uint64_t f = bfold(q ^ sg);
+                    const int dist = __popcll(f);
+                    uint32_t mp[4];
+                    mismatches(f, mp);
+                    hit = dist <= 4 && first_seen(mp, cs[r], s_posmask);
+                    if (hit) {
+                        occ = static_cast<uint32_t>(w[r] >> 40);
+                        if (occ == kOccEscape) occ = static_cast<uint32_t>(a.post[addr] >> 32);
+                        const Real ro = to_real<Real>(occ);
+                        if (a.sc.calc_mit && dist > 0) tm = mit_closed<Real>(mp, dist, s_om, s_t23) * ro;
+                        if (a.sc.calc_cfd) tc = cfd_product<Real>(q, sg, mp, dist, s_cfd) * ro;
+                    }
+                }
+                const uint64_t bal = __ballot(hit);
+                if (bal == 0) continue;
+                double rm = static_cast<double>(tm), rc = static_cast<double>(tc);
+                uint64_t rocc = occ;
+                tile_reduce_step<1>(rm, rc, rocc);
+                tile_reduce_step<2>(rm, rc, rocc);
+                tile_reduce_step<4>(rm, rc, rocc);
+                tile_reduce_step<8>(rm, rc, rocc);
+                tile_reduce_step<16>(rm, rc, rocc);
+                tile_reduce_step<32>(rm, rc, rocc);
+                uint32_t last = 63;   // the row's hits up to this lane are taken
+                if (!bexit(a.sc.method, dm + rm, dc + rc, a.sc.cut)) {
+                    dm += rm;
+                    dc += rc;
+                    occ_sum += rocc;
+                    sites += static_cast<uint64_t>(__popcll(bal));
+                } else {
+                    uint64_t rest = bal;
+                    while (rest) {
+                        const uint32_t h = static_cast<uint32_t>(__builtin_ctzll(rest));
+                        rest &= rest - 1;
+                        dm += static_cast<double>(readlane_real(tm, h));
+                        dc += static_cast<double>(readlane_real(tc, h));
+                        occ_sum += static_cast<uint32_t>(
+                            __builtin_amdgcn_readlane(static_cast<int>(occ), static_cast<int>(h)));
+                        sites += 1;
+                        if (bexit(a.sc.method, dm, dc, a.sc.cut)) {
+                            done = true;
+                            last = h;
+                            break;
+                        }
+                    }
+                    // (hit by hit the sums may stay below the cut, the row's
+                    // butterfly having rounded differently: then the whole
+                    // row is taken and the walk goes on)
+                }
+                if (a.distinct && hit && static_cast<uint32_t>(lane) <= last)
                     atomicOr(reinterpret_cast<unsigned long long *>(a.distinct) +
                                  (static_cast<uint32_t>(a.post[addr]) >> 6),
                              1ULL << (static_cast<uint32_t>(a.post[addr]) & 63));
-                if (bexit(a.sc.method, dm, dc, a.sc.cut)) {
-                    done = true;
-                    break;
-                }
             }
         }
         if (done) {
@@ -2448,6 +2530,7 @@ int launch_batch(issl_index *ix, const uint64_t *d_guides, uint64_t n, const Sco
     fa.meta = ix->d_meta;
     fa.mit = mit;
     fa.cfd = cfd;
+    fa.mitf = ix->d_mitf;
     fa.S = S;
     fa.n = n;
     fa.sc = sc;
