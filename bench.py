@@ -798,7 +798,9 @@ def main() -> int:
                    "tuning": {"chunk": tn.chunk, "depth": tn.depth,
                               "grid_per_cu": tn.grid_per_cu,
                               "stream_depth": tn.direct_depth,
-                              "heavy_kpost": tn.heavy_kpost}},
+                              "heavy_kpost": tn.heavy_kpost,
+                              "ord_depth": tn.ord_depth, "ord_grid": tn.ord_grid,
+                              "ord_split": tn.ord_split}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": kernel, "kernel_us": round(k_us, 2),
