@@ -608,6 +608,8 @@ def main() -> int:
                     help="form of the timed steps with --headline-only (profiling)")
     ap.add_argument("--precision", choices=["fp64", "fp32c"], default="fp64",
                     help="precision of the timed steps with --headline-only (profiling)")
+    ap.add_argument("--threshold", type=float, default=0.0,
+                    help="score threshold of --headline-only runs (75 arms the early exit)")
     ap.add_argument("--plain-steps", type=int, default=6)
     ap.add_argument("--cpu-sample", type=int, default=10000,
                     help="guides timed on the host CPU scorer")
@@ -712,9 +714,10 @@ def main() -> int:
     if args.headline_only:
         kw.update(form={"plain": issl.FORM_PLAIN, "ordered": issl.FORM_ORDERED,
                         "batched": issl.FORM_BATCHED}[args.form],
-                  precision=issl.PREC_FP32C if args.precision == "fp32c" else issl.PREC_FP64)
-    elif (args.form, args.precision) != ("batched", "fp64"):
-        raise SystemExit("--form / --precision apply to --headline-only runs only")
+                  precision=issl.PREC_FP32C if args.precision == "fp32c" else issl.PREC_FP64,
+                  threshold=args.threshold)
+    elif (args.form, args.precision, args.threshold) != ("batched", "fp64", 0.0):
+        raise SystemExit("--form / --precision / --threshold apply to --headline-only runs only")
     # With --graph each batch's launch sequence is captured in a hipGraph by its
     # first warm-up call and replayed afterwards (slower than eager launches for
     # this three-kernel sequence, so off by default).
@@ -815,7 +818,7 @@ def main() -> int:
                                     if traffic else None)},
     }
     if args.headline_only:
-        line["config"].update(form=args.form, precision=args.precision)
+        line["config"].update(form=args.form, precision=args.precision, threshold=args.threshold)
         print(json.dumps(line), flush=True)
         ix.close()
         return 0
