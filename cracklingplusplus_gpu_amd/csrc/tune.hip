@@ -37,7 +37,10 @@ std::string shape_key(const issl_index *ix, uint64_t n_sample, int precision) {
     for (char &c : arch)
         if (c == ' ' || c == ':') c = '_';
     std::ostringstream k;
-    k << "t8/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/w";
+    // (p_max, the bound on a guide's postings, tells apart indexes of one
+    // site count and slice shape whose lists differ: the in-order split is
+    // only tried where guides can pass its threshold)
+    k << "t9/" << arch << "/cu" << cus << "/n" << ix->n << "/s" << ix->S << "/pm" << ix->p_max << "/w";
     for (const SliceMeta &m : ix->meta_h) k << m.width << ',';
     int q = 0;
     while ((n_sample >> q) > 1) ++q;

@@ -327,6 +327,11 @@ def test_autotune_keeps_results(cases, tmp_path):
     assert (after.site_count == before.site_count).all()
     t2 = ix.autotune(g, str(tmp_path / "tune.txt"))  # from the cache
     assert (t2.chunk, t2.depth, t2.grid_per_cu) == (t.chunk, t.depth, t.grid_per_cu)
+    # The key holds the bound on a guide's postings, so that indexes of one
+    # site count and slice shape with different list lengths do not share a
+    # tuning (the in-order split is only tried where guides can pass it).
+    key = (tmp_path / "tune.txt").read_text().split()[0]
+    assert key.startswith("t9/") and "/pm" in key, key
 
 
 @pytest.mark.parametrize("form", [issl.FORM_PLAIN, issl.FORM_BATCHED, issl.FORM_ORDERED])
